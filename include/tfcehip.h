@@ -1,0 +1,110 @@
+/*
+ * libtfcehip — C ABI of the MI355X permutation hot path of TFCE_mediation.
+ *
+ * Conventions
+ *   - every function returns int: 0 = OK, < 0 = error; -1..-999 are negated
+ *     hipError_t values, <= -1000 are validation codes below;
+ *     tfce_last_error() gives a thread-local message for the last failure;
+ *   - d_* arguments are device pointers owned by the caller; host arrays are
+ *     read during the call only;
+ *   - each call enqueues on the given stream and returns without a host
+ *     synchronisation, unless its comment says otherwise;
+ *   - handles are immutable after creation, so one handle may serve calls on
+ *     several streams at once (scratch memory is kept per (device, stream)).
+ */
+#ifndef TFCEHIP_H
+#define TFCEHIP_H
+
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TFCE_OK 0
+#define TFCE_EINVAL (-1000)      /* bad argument */
+#define TFCE_EADJ (-1001)        /* adjacency index out of range / malformed CSR */
+#define TFCE_EDEGENERATE (-1002) /* max(image) >= minT with a non-positive step */
+#define TFCE_ELEVELS (-1003)     /* more than 1024 threshold levels */
+#define TFCE_ENOTSUP (-1004)
+
+/* ---------------------------------------------------------------- errors */
+const char *tfce_last_error(void);
+const char *tfce_version(void);
+
+/* ------------------------------------------------------------- adjacency */
+typedef struct tfce_adj tfce_adj_t;
+
+/* Adjacency flags (tfce_adj_create_ex) */
+#define TFCE_ADJ_IMPLICIT_ZERO 2   /* the full image also holds zeros outside these elements
+                                      (masked vertices): they enter max() for the levels */
+#define TFCE_ADJ_ZERO_IN_SCALE 4   /* ... and for the max/100 rescale */
+
+/* Replaces CreateAdjSet(H, E, adjacency) (tfce_mediation/tfce.pyx:24-42): CSR
+ * copy of the ragged adjacency plus the float H and E the operator stores
+ * (tfce.pyx:27-28).  row_ptr has V+1 entries, col holds indices in [0, V).
+ * Symmetry is detected here; an asymmetric list is honoured the way
+ * fast_tfce.hpp:44-66 uses it (an edge counts when it is listed by the later-
+ * activated end).  Blocking: uploads the CSR before returning. */
+int tfce_adj_create(const int32_t *row_ptr, const int32_t *col, int32_t V, int64_t nnz,
+                    float H, float E, tfce_adj_t **out);
+int tfce_adj_create_ex(const int32_t *row_ptr, const int32_t *col, int32_t V, int64_t nnz,
+                       float H, float E, uint32_t flags, tfce_adj_t **out);
+int tfce_adj_destroy(tfce_adj_t *adj);
+int tfce_adj_info(const tfce_adj_t *adj, int32_t *V, int64_t *nnz, int32_t *symmetric);
+
+/* Replaces CreateAdjSet.run(image, enhn) (tfce.pyx:44-45 -> fast_tfce.hpp:11-95):
+ * enhn += TFCE(image) over V float32 elements (accumulate != 0), or
+ * enhn = TFCE(image) (accumulate == 0).  dh == 0 gives the reference's step
+ * fl(max/100); dh > 0 an explicit step.  minT is 0 as in tfce.pyx:45. */
+int tfce_run(tfce_adj_t *adj, const float *d_image, float *d_enhn, float dh, int accumulate,
+             hipStream_t stream);
+
+/* tfce_run with every knob: min_t, optional per-call status (levels visited, or
+ * a negative TFCE_E* code) and component labels after level labels_level
+ * (d_labels[v] = a representative vertex of v's component, -1 inactive; the
+ * caller fills it with -1 first). */
+int tfce_run_ex(tfce_adj_t *adj, const float *d_image, float *d_enhn, float min_t, float dh,
+                int accumulate, int32_t labels_level, int32_t *d_labels, int32_t *d_status,
+                hipStream_t stream);
+
+/* Batched permutation maxima for one adjacency, replacing the per-permutation
+ * pair of write_perm_maxTFCE_voxel / _vertex calls (pyfunc.py:107-126):
+ *   d_max[2p + 0] = max_v fl(fl(TFCE(+map_p)_v * fl(max(+map_p)/100)) * vd_v)
+ *   d_max[2p + 1] = same for -map_p
+ * d_maps is P x V row-major.  sign_mask bit0 = +, bit1 = - (an unrequested
+ * sign writes 0).  d_vdensity: per-element weights or NULL (= 1). */
+int tfce_batch_max(tfce_adj_t *adj, const float *d_maps, int P, int sign_mask,
+                   const float *d_vdensity, float *d_max, hipStream_t stream);
+
+/* ---------------------------------------------------------- surface sets */
+/* Several adjacencies laid out along one map row (lh|rh hemispheres, or the
+ * merged image of tm_func.py:54-123 / create_position_array :556-562). */
+typedef struct tfce_surfset tfce_surfset_t;
+int tfce_surfset_create(const tfce_adj_t *const *adjs, int32_t n, const int64_t *offsets,
+                        tfce_surfset_t **out);
+int tfce_surfset_destroy(tfce_surfset_t *set);
+/* Launch order of a set (largest surface first): user_index[k] = caller's
+ * index of the k-th surface; outputs of tfce_batch follow this order. */
+int tfce_surfset_order(const tfce_surfset_t *set, int32_t *user_index);
+
+/* One launch over every (surface, map, sign) item, largest surface first.
+ * Outputs are indexed item = (surface * P + p) * n_signs + sign_index:
+ *   d_scaled: rescaled maximum as tfce_batch_max; d_raw: max of the TFCE sum;
+ *   d_own: max of sign*map over the surface; d_status: levels or negative code.
+ * d_T0 (optional, [n_items]) replaces each item's first threshold (the shared-
+ * threshold mode of tm_func.py:54-123, where deltaT comes from the merged
+ * image).  d_vdensity (optional) holds per-element weights in map-column
+ * indexing.  Any output pointer except d_scaled may be NULL. */
+int tfce_batch(const tfce_surfset_t *set, const float *d_maps, int64_t map_stride, int P,
+               int sign_mask, const float *d_T0, const float *d_vdensity, float *d_scaled,
+               float *d_raw, float *d_own, int32_t *d_status, hipStream_t stream);
+
+/* Number of resident workgroups per CU used by the TFCE sweep (autotuner knob). */
+int tfce_set_occupancy(int32_t wgs_per_cu);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TFCEHIP_H */

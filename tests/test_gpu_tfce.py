@@ -1,0 +1,182 @@
+"""GPU parity of the TFCE sweep (libtfcehip) against the C oracle.
+
+Bar: bit-exact float32 for H=2 with E=1 (every increment is one correctly
+rounded product, as in the reference); E=0.5 / 0.67 use sqrt / pow on the GPU
+where the reference calls glibc pow, so those allow 4 ulp.  Component
+partitions must be identical at every checked level.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import tfce as otfce
+from tfce_mediation_amd import adjacency, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as _t
+    assert _t.cuda.is_available()
+    return _t
+
+
+def smooth_map(row_ptr, col, seed, passes=3, shift=0.3, V=None):
+    rng = np.random.default_rng(seed)
+    V = len(row_ptr) - 1
+    x = rng.standard_normal(V)
+    deg = np.diff(row_ptr).astype(np.int64)
+    src = np.repeat(np.arange(V), deg)
+    for _ in range(passes):
+        s = np.bincount(src, weights=x[col], minlength=V)
+        x = (x + s) / (deg + 1)
+    return (x / x.std() + shift).astype(np.float32)
+
+
+def ulp_diff(a, b):
+    ai = a.view(np.int32).astype(np.int64)
+    bi = b.view(np.int32).astype(np.int64)
+    return np.abs(ai - bi)
+
+
+def check_close(got, ref, E, H):
+    if E == 1.0 and H in (1.0, 2.0):
+        bad = np.nonzero(got.view(np.uint32) != ref.view(np.uint32))[0]
+        assert bad.size == 0, (bad[:5], got[bad[:5]], ref[bad[:5]])
+    else:
+        assert ulp_diff(got, ref).max() <= 4
+
+
+GRAPHS = {
+    "ico3": lambda: synth.icosphere_csr(3),
+    "ico5": lambda: synth.icosphere_csr(5),
+    "vox26": lambda: adjacency.voxel_csr(synth.ellipsoid_mask((21, 25, 19), (10, 12, 9), (9, 11, 8.5)), 26),
+    "vox6_walls": lambda: adjacency.voxel_csr(np.ones((7, 8, 6), bool), 6),
+}
+
+
+@pytest.mark.parametrize("gname", sorted(GRAPHS))
+@pytest.mark.parametrize("HE", [(2.0, 1.0), (2.0, 0.5), (2.0, 0.67)])
+def test_run_matches_oracle(torch, gname, HE):
+    from tfce_mediation_amd.tfce import CreateAdjSet
+    H, E = HE
+    row_ptr, col = GRAPHS[gname]()
+    say(f"[gpu] CreateAdjSet.run {gname} V={len(row_ptr)-1} H={H} E={E}")
+    op = CreateAdjSet(H, E, (row_ptr, col))
+    for seed in range(3):
+        img = smooth_map(row_ptr, col, seed)
+        if gname == "vox6_walls":
+            img = img + np.float32(1e-4) * np.arange(len(img), dtype=np.float32) / len(img)
+        for sgn in (1, -1):
+            x = (sgn * img).astype(np.float32)
+            enh = np.zeros_like(x)
+            op.run(x, enh)
+            ref, st, _ = otfce.tfce(row_ptr, col, x, H, E)
+            assert st > 0
+            check_close(enh, ref, E, H)
+
+
+def test_run_accumulates_and_degenerate(torch):
+    from tfce_mediation_amd.tfce import CreateAdjSet
+    row_ptr, col = synth.icosphere_csr(4)
+    V = len(row_ptr) - 1
+    say("[gpu] accumulate / degenerate ico4")
+    op = CreateAdjSet(2, 1, (row_ptr, col))
+    img = smooth_map(row_ptr, col, 11)
+    base = np.linspace(-1, 1, V).astype(np.float32)
+    enh = base.copy()
+    op.run(img, enh)
+    ref, _, _ = otfce.tfce(row_ptr, col, img, 2.0, 1.0, enhn=base.copy())
+    check_close(enh, ref, 1.0, 2.0)
+    enh = base.copy()
+    op.run(np.zeros(V, np.float32), enh)  # max == 0: the reference never returns; defined as no-op
+    assert np.array_equal(enh, base)
+    op.run(-np.abs(img) - 0.5, enh)  # all negative: nothing is active
+    assert np.array_equal(enh, base)
+
+
+def canon(labels):
+    out = np.full(labels.shape, -1, np.int64)
+    act = labels >= 0
+    rep = labels[act]
+    idx = np.nonzero(act)[0]
+    mins = {}
+    for r, v in zip(rep, idx):
+        if r not in mins or v < mins[r]:
+            mins[r] = v
+    out[idx] = [mins[r] for r in rep]
+    return out
+
+
+@pytest.mark.parametrize("gname", ["ico5", "vox26", "vox6_walls"])
+def test_partitions_equal_at_every_level(torch, gname):
+    from tfce_mediation_amd.tfce import Adjacency
+    row_ptr, col = GRAPHS[gname]()
+    V = len(row_ptr) - 1
+    adj = Adjacency((row_ptr, col), 2.0, 1.0)
+    img = smooth_map(row_ptr, col, 5, shift=0.0)
+    if gname == "vox6_walls":
+        img = img + np.float32(1e-4) * np.arange(V, dtype=np.float32) / V
+    lv = otfce.levels(img.max())
+    d_img = torch.from_numpy(img).cuda()
+    for level in range(0, len(lv), 7):
+        say(f"[gpu] labels {gname} level {level}")
+        enh = torch.zeros(V, dtype=torch.float32, device="cuda")
+        lab = torch.full((V,), -1, dtype=torch.int32, device="cuda")
+        st = torch.zeros(1, dtype=torch.int32, device="cuda")
+        adj.run_device(d_img, enh, accumulate=False, labels_level=level, labels=lab, status=st)
+        _, _, ref = otfce.tfce(row_ptr, col, img, 2.0, 1.0, labels_level=level)
+        assert int(st.item()) == len(lv)
+        np.testing.assert_array_equal(canon(lab.cpu().numpy()), ref)
+
+
+@pytest.mark.parametrize("gname,E", [("ico5", 1.0), ("vox26", 0.5)])
+def test_batch_max_matches_oracle(torch, gname, E):
+    from tfce_mediation_amd.tfce import Adjacency
+    row_ptr, col = GRAPHS[gname]()
+    V = len(row_ptr) - 1
+    P = 24
+    say(f"[gpu] batch_max {gname} P={P} E={E}")
+    maps = np.stack([smooth_map(row_ptr, col, 100 + p, shift=0.0) for p in range(P)])
+    adj = Adjacency((row_ptr, col), 2.0, E)
+    d = torch.from_numpy(maps).cuda()
+    out = adj.batch_max(d, 3).cpu().numpy()
+    vd = adjacency.vertex_density(row_ptr)
+    outv = adj.batch_max(d, 3, vdensity=torch.from_numpy(vd).cuda()).cpu().numpy()
+    for p in range(P):
+        for k, sgn in enumerate((1, -1)):
+            x = (sgn * maps[p]).astype(np.float32)
+            enh, _, _ = otfce.tfce(row_ptr, col, x, 2.0, E)
+            s = np.float32(x.max() / np.float32(100))
+            exp = np.float32(enh.max() * s)
+            expv = ((enh * s) * vd).max()
+            if E == 1.0:
+                assert out[p, k] == exp and outv[p, k] == expv
+            else:
+                assert ulp_diff(np.float32([out[p, k]]), np.float32([exp])).max() <= 4
+                assert ulp_diff(np.float32([outv[p, k]]), np.float32([expv])).max() <= 4
+    one = adj.batch_max(d, 2).cpu().numpy()
+    assert np.all(one[:, 0] == 0) and np.array_equal(one[:, 1], out[:, 1])
+
+
+def test_large_hemisphere_maps(torch):
+    from tfce_mediation_amd.tfce import Adjacency
+    row_ptr, col = synth.icosphere_csr(7)
+    V = len(row_ptr) - 1
+    P = 6
+    say(f"[gpu] order-7 hemisphere V={V} P={P}")
+    maps = np.stack([smooth_map(row_ptr, col, 300 + p, passes=5, shift=0.0) for p in range(P)])
+    adj = Adjacency((row_ptr, col), 2.0, 1.0)
+    out = adj.batch_max(torch.from_numpy(maps).cuda(), 3).cpu().numpy()
+    for p in range(P):
+        for k, sgn in enumerate((1, -1)):
+            x = (sgn * maps[p]).astype(np.float32)
+            enh, _, _ = otfce.tfce(row_ptr, col, x, 2.0, 1.0)
+            assert out[p, k] == np.float32(enh.max() * np.float32(x.max() / np.float32(100)))

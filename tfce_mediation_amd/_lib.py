@@ -1,0 +1,88 @@
+"""ctypes binding of libtfcehip.so (C ABI declared in include/tfcehip.h).
+
+The library is the only compute path: if it is missing, or no GPU is visible,
+every entry point raises.  There is no CPU fallback.
+"""
+import ctypes
+import os
+import threading
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libtfcehip.so")
+
+c_int = ctypes.c_int
+c_i32 = ctypes.c_int32
+c_i64 = ctypes.c_int64
+c_u32 = ctypes.c_uint32
+c_f32 = ctypes.c_float
+c_p = ctypes.c_void_p
+c_str = ctypes.c_char_p
+
+# name -> (restype, argtypes); every function declared in include/tfcehip.h
+SIGNATURES = {
+    "tfce_last_error": (c_str, []),
+    "tfce_version": (c_str, []),
+    "tfce_adj_create": (c_int, [c_p, c_p, c_i32, c_i64, c_f32, c_f32, c_p]),
+    "tfce_adj_create_ex": (c_int, [c_p, c_p, c_i32, c_i64, c_f32, c_f32, c_u32, c_p]),
+    "tfce_adj_destroy": (c_int, [c_p]),
+    "tfce_adj_info": (c_int, [c_p, c_p, c_p, c_p]),
+    "tfce_run": (c_int, [c_p, c_p, c_p, c_f32, c_int, c_p]),
+    "tfce_run_ex": (c_int, [c_p, c_p, c_p, c_f32, c_f32, c_int, c_i32, c_p, c_p, c_p]),
+    "tfce_batch_max": (c_int, [c_p, c_p, c_int, c_int, c_p, c_p, c_p]),
+    "tfce_surfset_create": (c_int, [c_p, c_i32, c_p, c_p]),
+    "tfce_surfset_destroy": (c_int, [c_p]),
+    "tfce_surfset_order": (c_int, [c_p, c_p]),
+    "tfce_batch": (c_int, [c_p, c_p, c_i64, c_int, c_int, c_p, c_p, c_p, c_p, c_p, c_p, c_p]),
+    "tfce_set_occupancy": (c_int, [c_i32]),
+}
+
+_lock = threading.Lock()
+_lib = None
+
+
+class TfceHipError(RuntimeError):
+    pass
+
+
+def load_library(path=LIB_PATH):
+    """Load the shared library without touching the GPU (symbol checks)."""
+    if not os.path.exists(path):
+        raise ImportError(f"libtfcehip.so not built at {path}: run __graft_entry__.build()")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+def lib():
+    """The library, after checking that a GPU is present (raises otherwise)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            import torch
+            if not torch.cuda.is_available():
+                raise TfceHipError("tfce_mediation_amd needs a ROCm GPU; none is visible")
+            _lib = load_library()
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = _lib.tfce_last_error().decode() if _lib is not None else ""
+        raise TfceHipError(f"libtfcehip error {rc}: {msg}")
+    return rc
+
+
+def stream_ptr(stream=None):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def ptr(t):
+    """Device pointer of a torch tensor (None -> NULL)."""
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())

@@ -1,0 +1,150 @@
+"""Adjacency builders and CSR conversion for the TFCE operator.
+
+* ``create_adjac_vertex`` - triangle-edge adjacency, as ``pyfunc.py:37-46``
+  (each face contributes its three undirected edges).
+* ``create_adjac_voxel`` - 6/26-neighbourhood adjacency of a voxel mask with the
+  reference's quirks (``pyfunc.py:48-76``): the first in-mask voxel (label 0) is
+  nobody's neighbour and has an empty list; with 6-connectivity a voxel whose
+  3x3x3 window is clipped by the volume edge keeps no neighbours.
+* ``to_csr`` - ragged adjacency (list of sets/lists, object array from
+  ``np.load``) or (row_ptr, col) -> int32 CSR.
+* ``merge_csr`` / ``merge_adjacency_array`` - concatenate surfaces with index
+  offsets (``tm_func.py:521-540``).
+* ``vertex_density`` - the degree-based weight of
+  ``STEP_1_vertex_tfce_multiple_regression.py:165-173`` and
+  ``tm_multimodality_multisurface_regression.py:449-462``.
+All of these are vectorised host code run once per analysis.
+"""
+import numpy as np
+
+
+def faces_to_csr(faces, n_vertices):
+    faces = np.asarray(faces, dtype=np.int64)
+    a, b, c = faces[:, 0], faces[:, 1], faces[:, 2]
+    src = np.concatenate([a, a, b, b, c, c])
+    dst = np.concatenate([b, c, a, c, a, b])
+    keep = src != dst
+    src, dst = src[keep], dst[keep]
+    key = np.unique(src * n_vertices + dst)
+    src = (key // n_vertices).astype(np.int64)
+    dst = (key % n_vertices).astype(np.int32)
+    row_ptr = np.zeros(n_vertices + 1, dtype=np.int64)
+    np.add.at(row_ptr, src + 1, 1)
+    row_ptr = np.cumsum(row_ptr).astype(np.int32)
+    return row_ptr, dst
+
+
+def create_adjac_vertex(vertices, faces):
+    """Ragged (list of sets) form, as ``pyfunc.py:37-46`` returns it."""
+    row_ptr, col = faces_to_csr(faces, np.asarray(vertices).shape[0])
+    return [set(col[row_ptr[i]:row_ptr[i + 1]].tolist()) for i in range(len(row_ptr) - 1)]
+
+
+_OFFSETS26 = [(dx, dy, dz) for dx in (-1, 0, 1) for dy in (-1, 0, 1) for dz in (-1, 0, 1)
+              if (dx, dy, dz) != (0, 0, 0)]
+_OFFSETS6 = [(-1, 0, 0), (1, 0, 0), (0, -1, 0), (0, 1, 0), (0, 0, -1), (0, 0, 1)]
+
+
+def voxel_csr(mask, dirtype=26):
+    """CSR of ``create_adjac_voxel(mask, mask, mask.sum(), dirtype)``.
+
+    Elements are the in-mask voxels in C order (``np.where`` order); rows are
+    sorted ascending like the reference's ``sorted(i)``.
+    """
+    mask = np.asarray(mask, dtype=bool)
+    X, Y, Z = mask.shape
+    ind = np.nonzero(mask)
+    n = ind[0].size
+    label = np.full(mask.shape, -1, dtype=np.int64)
+    label[ind] = np.arange(n)
+    x, y, z = (np.asarray(i, dtype=np.int64) for i in ind)
+    offsets = _OFFSETS26 if int(dirtype) == 26 else _OFFSETS6
+    if int(dirtype) == 6:
+        interior = (x > 0) & (x < X - 1) & (y > 0) & (y < Y - 1) & (z > 0) & (z < Z - 1)
+    srcs, dsts = [], []
+    for dx, dy, dz in offsets:
+        nx, ny, nz = x + dx, y + dy, z + dz
+        ok = (nx >= 0) & (nx < X) & (ny >= 0) & (ny < Y) & (nz >= 0) & (nz < Z)
+        if int(dirtype) == 6:
+            ok &= interior
+        src = np.nonzero(ok)[0]
+        nb = label[nx[ok], ny[ok], nz[ok]]
+        good = nb > 0  # label 0 (first voxel) and out-of-mask are never neighbours
+        srcs.append(src[good])
+        dsts.append(nb[good])
+    src = np.concatenate(srcs)
+    dst = np.concatenate(dsts)
+    keep = src != 0  # adjacency[0] = []
+    src, dst = src[keep], dst[keep]
+    order = np.lexsort((dst, src))
+    src, dst = src[order], dst[order]
+    row_ptr = np.zeros(n + 1, dtype=np.int64)
+    np.add.at(row_ptr, src + 1, 1)
+    return np.cumsum(row_ptr).astype(np.int32), dst.astype(np.int32)
+
+
+def create_adjac_voxel(data_index, data_mask, num_voxel, dirtype=26):
+    """Ragged form with the reference signature (``pyfunc.py:48-76``)."""
+    mask = np.asarray(data_index, dtype=bool)
+    row_ptr, col = voxel_csr(mask, dirtype)
+    assert row_ptr.shape[0] - 1 == int(num_voxel)
+    return [col[row_ptr[i]:row_ptr[i + 1]].tolist() for i in range(int(num_voxel))]
+
+
+def to_csr(adjacency):
+    """Ragged adjacency or (row_ptr, col) -> (row_ptr int32[V+1], col int32[nnz])."""
+    if isinstance(adjacency, tuple) and len(adjacency) == 2:
+        row_ptr = np.ascontiguousarray(adjacency[0], dtype=np.int32)
+        col = np.ascontiguousarray(adjacency[1], dtype=np.int32)
+        return row_ptr, col
+    if hasattr(adjacency, "tocsr"):
+        m = adjacency.tocsr()
+        return (np.ascontiguousarray(m.indptr, dtype=np.int32),
+                np.ascontiguousarray(m.indices, dtype=np.int32))
+    lists = [np.fromiter((int(u) for u in row), dtype=np.int64) for row in adjacency]
+    lengths = np.array([len(r) for r in lists], dtype=np.int64)
+    row_ptr = np.zeros(len(lists) + 1, dtype=np.int64)
+    np.cumsum(lengths, out=row_ptr[1:])
+    col = np.concatenate(lists) if lists and row_ptr[-1] > 0 else np.zeros(0, dtype=np.int64)
+    return row_ptr.astype(np.int32), col.astype(np.int32)
+
+
+def merge_csr(csrs):
+    """Block-diagonal concatenation of CSR graphs (tm_func.py:521-540)."""
+    rows, cols, off_v, off_e = [np.zeros(1, dtype=np.int64)], [], 0, 0
+    for rp, c in csrs:
+        rp = np.asarray(rp, dtype=np.int64)
+        rows.append(rp[1:] + off_e)
+        cols.append(np.asarray(c, dtype=np.int64) + off_v)
+        off_v += rp.shape[0] - 1
+        off_e += int(rp[-1])
+    return (np.concatenate(rows).astype(np.int32),
+            np.concatenate(cols).astype(np.int32) if cols else np.zeros(0, np.int32))
+
+
+def merge_adjacency_array(adjacent_range, adjacency_array):
+    """Ragged merge with offsets, same result as ``tm_func.py:521-540``."""
+    csrs = [to_csr(adjacency_array[e]) for e in adjacent_range]
+    rp, c = merge_csr(csrs)
+    return [c[rp[i]:rp[i + 1]].tolist() for i in range(rp.shape[0] - 1)]
+
+
+def induced_subgraph(row_ptr, col, keep):
+    """CSR restricted to the elements where ``keep`` is true, re-indexed."""
+    keep = np.asarray(keep, dtype=bool)
+    newid = np.full(keep.shape[0], -1, dtype=np.int64)
+    newid[keep] = np.arange(int(keep.sum()))
+    row_ptr = np.asarray(row_ptr, dtype=np.int64)
+    src = np.repeat(np.arange(keep.shape[0]), np.diff(row_ptr))
+    dst = np.asarray(col, dtype=np.int64)
+    ok = keep[src] & keep[dst]
+    s, d = newid[src[ok]], newid[dst[ok]]
+    rp = np.zeros(int(keep.sum()) + 1, dtype=np.int64)
+    np.add.at(rp, s + 1, 1)
+    return np.cumsum(rp).astype(np.int32), d.astype(np.int32)
+
+
+def vertex_density(row_ptr):
+    """1 - deg/max(deg) + mean(deg)/max(deg), float32 (float64 arithmetic)."""
+    deg = np.diff(np.asarray(row_ptr, dtype=np.int64)).astype(np.float64)
+    return np.array((1 - (deg / deg.max()) + (deg.mean() / deg.max())), dtype=np.float32)

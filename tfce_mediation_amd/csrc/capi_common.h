@@ -1,0 +1,21 @@
+// Shared host-side helpers of the C-ABI layer (not part of the ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <string>
+
+namespace tfcehip {
+
+struct Workspace {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  int32_t* queue = nullptr;  // 256 bytes: work counters
+};
+
+int set_error(int code, const std::string& msg);
+int current_device();
+int cu_count(int device);
+int workspace_get(hipStream_t stream, size_t bytes, Workspace** out);
+int tfce_grid(int64_t n_items);
+
+}  // namespace tfcehip

@@ -1,0 +1,361 @@
+// C-ABI host layer: adjacency handles, per-stream scratch, TFCE entry points.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/tfcehip.h"
+#include "capi_common.h"
+#include "tfce_internal.h"
+
+using namespace tfcehip;
+
+struct tfce_adj {
+  int32_t V = 0;
+  int64_t nnz = 0;
+  float H = 2.0f, E = 1.0f;
+  int32_t flags = 0;
+  int device = 0;
+  int32_t* d_row_ptr = nullptr;
+  int32_t* d_col = nullptr;
+  SurfDev* d_surf = nullptr;  // one descriptor, offset 0
+};
+
+struct tfce_surfset {
+  int32_t n = 0;
+  int device = 0;
+  int32_t v_max = 0;
+  std::vector<SurfDev> host;  // in launch order (largest first)
+  std::vector<int32_t> user_index;  // launch position -> caller's surface index
+  SurfDev* d_surfs = nullptr;
+};
+
+namespace tfcehip {
+
+thread_local std::string g_last_error;
+
+int set_error(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+static std::mutex g_ws_mutex;
+static std::map<std::pair<int, hipStream_t>, Workspace> g_ws;
+static int g_wgs_per_cu = 4;
+
+int current_device() {
+  int d = 0;
+  (void)hipGetDevice(&d);
+  return d;
+}
+
+int cu_count(int device) {
+  static std::map<int, int> cache;
+  static std::mutex m;
+  std::lock_guard<std::mutex> g(m);
+  auto it = cache.find(device);
+  if (it != cache.end()) return it->second;
+  int n = 0;
+  if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || n <= 0)
+    n = 256;
+  cache[device] = n;
+  return n;
+}
+
+// Grow-only scratch for (device, stream).  Growing synchronises the stream so
+// no kernel of that stream still uses the old buffer.
+int workspace_get(hipStream_t stream, size_t bytes, Workspace** out) {
+  const int dev = current_device();
+  std::lock_guard<std::mutex> g(g_ws_mutex);
+  Workspace& w = g_ws[std::make_pair(dev, stream)];
+  if (!w.queue) {
+    hipError_t e = hipMalloc(&w.queue, 256);
+    if (e != hipSuccess) return set_error(-(int)e, std::string("hipMalloc(queue): ") + hipGetErrorString(e));
+  }
+  if (w.bytes < bytes) {
+    if (w.ptr) {
+      (void)hipStreamSynchronize(stream);
+      (void)hipFree(w.ptr);
+      w.ptr = nullptr;
+      w.bytes = 0;
+    }
+    size_t want = std::max(bytes, (size_t)(w.bytes * 5 / 4));
+    hipError_t e = hipMalloc(&w.ptr, want);
+    if (e != hipSuccess) {
+      e = hipMalloc(&w.ptr, bytes);
+      if (e != hipSuccess)
+        return set_error(-(int)e, std::string("hipMalloc(scratch): ") + hipGetErrorString(e));
+      want = bytes;
+    }
+    w.bytes = want;
+  }
+  *out = &w;
+  return 0;
+}
+
+int tfce_grid(int64_t n_items) {
+  const int64_t slots = (int64_t)cu_count(current_device()) * g_wgs_per_cu;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(n_items, slots));
+}
+
+}  // namespace tfcehip
+
+#define HIPCALL(expr)                                                               \
+  do {                                                                              \
+    hipError_t e_ = (expr);                                                         \
+    if (e_ != hipSuccess)                                                           \
+      return set_error(-(int)e_, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+extern "C" {
+
+const char* tfce_last_error(void) { return g_last_error.c_str(); }
+const char* tfce_version(void) { return "tfcehip 0.1 gfx950"; }
+
+int tfce_set_occupancy(int32_t wgs_per_cu) {
+  if (wgs_per_cu < 1 || wgs_per_cu > 16) return set_error(TFCE_EINVAL, "occupancy must be in [1, 16]");
+  g_wgs_per_cu = wgs_per_cu;
+  return 0;
+}
+
+int tfce_adj_create_ex(const int32_t* row_ptr, const int32_t* col, int32_t V, int64_t nnz, float H,
+                       float E, uint32_t flags, tfce_adj_t** out) {
+  if (!out) return set_error(TFCE_EINVAL, "out is NULL");
+  *out = nullptr;
+  if (V <= 0 || V >= (1 << 24)) return set_error(TFCE_EINVAL, "V must be in [1, 2^24)");
+  if (nnz < 0 || nnz >= ((int64_t)1 << 31)) return set_error(TFCE_EINVAL, "nnz must be in [0, 2^31)");
+  if (!row_ptr || (nnz > 0 && !col)) return set_error(TFCE_EINVAL, "row_ptr/col is NULL");
+  if (row_ptr[0] != 0 || row_ptr[V] != nnz) return set_error(TFCE_EADJ, "row_ptr[0] must be 0 and row_ptr[V] == nnz");
+  for (int32_t v = 0; v < V; ++v)
+    if (row_ptr[v + 1] < row_ptr[v]) return set_error(TFCE_EADJ, "row_ptr must be non-decreasing");
+  for (int64_t e = 0; e < nnz; ++e)
+    if (col[e] < 0 || col[e] >= V)
+      return set_error(TFCE_EADJ, "adjacency index out of range at entry " + std::to_string(e));
+  if ((flags & ~(uint32_t)(TFCE_ADJ_IMPLICIT_ZERO | TFCE_ADJ_ZERO_IN_SCALE)) != 0)
+    return set_error(TFCE_EINVAL, "unknown adjacency flag");
+  // symmetry: every (v,u) has (u,v); self loops and repeats are harmless
+  bool symmetric = true;
+  {
+    std::vector<int64_t> fwd, bwd;
+    fwd.reserve(nnz);
+    bwd.reserve(nnz);
+    for (int32_t v = 0; v < V; ++v)
+      for (int32_t e = row_ptr[v]; e < row_ptr[v + 1]; ++e) {
+        const int64_t u = col[e];
+        if (u == v) continue;
+        fwd.push_back((int64_t)v * V + u);
+        bwd.push_back(u * V + v);
+      }
+    std::sort(fwd.begin(), fwd.end());
+    fwd.erase(std::unique(fwd.begin(), fwd.end()), fwd.end());
+    std::sort(bwd.begin(), bwd.end());
+    bwd.erase(std::unique(bwd.begin(), bwd.end()), bwd.end());
+    symmetric = (fwd == bwd);
+  }
+  tfce_adj* a = new tfce_adj();
+  a->V = V;
+  a->nnz = nnz;
+  a->H = H;
+  a->E = E;
+  a->flags = (int32_t)flags | (symmetric ? kSurfSymmetric : 0);
+  a->device = current_device();
+  auto cleanup = [&](int code, const std::string& m) {
+    if (a->d_row_ptr) (void)hipFree(a->d_row_ptr);
+    if (a->d_col) (void)hipFree(a->d_col);
+    if (a->d_surf) (void)hipFree(a->d_surf);
+    delete a;
+    return set_error(code, m);
+  };
+  hipError_t e;
+  if ((e = hipMalloc(&a->d_row_ptr, sizeof(int32_t) * (V + 1))) != hipSuccess)
+    return cleanup(-(int)e, "hipMalloc(row_ptr)");
+  if ((e = hipMalloc(&a->d_col, sizeof(int32_t) * std::max<int64_t>(nnz, 1))) != hipSuccess)
+    return cleanup(-(int)e, "hipMalloc(col)");
+  if ((e = hipMalloc(&a->d_surf, sizeof(SurfDev))) != hipSuccess) return cleanup(-(int)e, "hipMalloc(surf)");
+  if ((e = hipMemcpy(a->d_row_ptr, row_ptr, sizeof(int32_t) * (V + 1), hipMemcpyHostToDevice)) != hipSuccess)
+    return cleanup(-(int)e, "upload row_ptr");
+  if (nnz > 0 && (e = hipMemcpy(a->d_col, col, sizeof(int32_t) * nnz, hipMemcpyHostToDevice)) != hipSuccess)
+    return cleanup(-(int)e, "upload col");
+  SurfDev sd;
+  sd.row_ptr = a->d_row_ptr;
+  sd.col = a->d_col;
+  sd.offset = 0;
+  sd.V = V;
+  sd.flags = a->flags;
+  sd.H = H;
+  sd.E = E;
+  if ((e = hipMemcpy(a->d_surf, &sd, sizeof(SurfDev), hipMemcpyHostToDevice)) != hipSuccess)
+    return cleanup(-(int)e, "upload surface");
+  *out = a;
+  return 0;
+}
+
+int tfce_adj_create(const int32_t* row_ptr, const int32_t* col, int32_t V, int64_t nnz, float H, float E,
+                    tfce_adj_t** out) {
+  return tfce_adj_create_ex(row_ptr, col, V, nnz, H, E, 0u, out);
+}
+
+int tfce_adj_destroy(tfce_adj_t* a) {
+  if (!a) return 0;
+  (void)hipFree(a->d_row_ptr);
+  (void)hipFree(a->d_col);
+  (void)hipFree(a->d_surf);
+  delete a;
+  return 0;
+}
+
+int tfce_adj_info(const tfce_adj_t* a, int32_t* V, int64_t* nnz, int32_t* symmetric) {
+  if (!a) return set_error(TFCE_EINVAL, "adj is NULL");
+  if (V) *V = a->V;
+  if (nnz) *nnz = a->nnz;
+  if (symmetric) *symmetric = (a->flags & kSurfSymmetric) ? 1 : 0;
+  return 0;
+}
+
+static int run_batch(const SurfDev* d_surfs, int32_t n_surf, int32_t v_max, const float* d_maps,
+                     int64_t map_stride, int P, int sign_mask, const float* d_T0,
+                     const float* d_vdensity, bool full, float* d_enhn, float min_t, float dh,
+                     int32_t labels_level, int32_t* d_labels, float* d_scaled, float* d_raw,
+                     float* d_own, int32_t* d_status, hipStream_t stream, int out_mul = 1,
+                     int out_add = 0) {
+  if (P <= 0) return 0;
+  if (sign_mask < 1 || sign_mask > 3) return set_error(TFCE_EINVAL, "sign_mask must be 1, 2 or 3");
+  const int n_signs = (sign_mask == 3) ? 2 : 1;
+  const int64_t n_items = (int64_t)n_surf * P * n_signs;
+  Workspace* ws = nullptr;
+  const int grid = tfce_grid(n_items);
+  const int64_t slot_words = tfce_slot_words(v_max);
+  // raw_max is mandatory for the kernel: park it in scratch when not wanted
+  const size_t extra = d_raw ? 0 : (size_t)n_items * sizeof(float);
+  int rc = workspace_get(stream, (size_t)grid * slot_words * sizeof(int32_t) + extra + 256, &ws);
+  if (rc) return rc;
+  TfceBatchArgs a{};
+  a.surfs = d_surfs;
+  a.n_surf = n_surf;
+  a.n_maps = P;
+  a.n_signs = n_signs;
+  a.sign0 = (sign_mask == 2) ? -1.0f : 1.0f;
+  a.maps = d_maps;
+  a.map_stride = map_stride;
+  a.ext_T0 = d_T0;
+  a.vdensity = d_vdensity;
+  a.min_t = min_t;
+  a.dh = dh;
+  a.full_scale = 0;
+  a.scaled_max = d_scaled;
+  a.out_mul = out_mul;
+  a.out_add = out_add;
+  a.own_max = d_own;
+  a.status = d_status;
+  a.enhn = d_enhn;
+  a.labels_level = labels_level;
+  a.labels = d_labels;
+  a.scratch = reinterpret_cast<int32_t*>(ws->ptr);
+  a.slot_stride = slot_words;
+  a.v_max = v_max;
+  a.queue = ws->queue;
+  a.n_items = n_items;
+  a.raw_max = d_raw ? d_raw
+                    : reinterpret_cast<float*>(reinterpret_cast<char*>(ws->ptr) +
+                                               (size_t)grid * slot_words * sizeof(int32_t));
+  hipError_t e = launch_tfce_batch(a, full, grid, stream);
+  if (e != hipSuccess) return set_error(-(int)e, std::string("tfce launch: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int tfce_run_ex(tfce_adj_t* adj, const float* d_image, float* d_enhn, float min_t, float dh, int accumulate,
+                int32_t labels_level, int32_t* d_labels, int32_t* d_status, hipStream_t stream) {
+  if (!adj || !d_image || !d_enhn) return set_error(TFCE_EINVAL, "NULL argument");
+  if (dh < 0.0f) return set_error(TFCE_EINVAL, "dh must be >= 0");
+  if (!accumulate) {
+    hipError_t e = launch_fill_f32(d_enhn, adj->V, 0.0f, stream);
+    if (e != hipSuccess) return set_error(-(int)e, "fill enhn");
+  }
+  return run_batch(adj->d_surf, 1, adj->V, d_image, adj->V, 1, 1, nullptr, nullptr, true, d_enhn, min_t,
+                   dh, labels_level, d_labels, nullptr, nullptr, nullptr, d_status, stream);
+}
+
+int tfce_run(tfce_adj_t* adj, const float* d_image, float* d_enhn, float dh, int accumulate,
+             hipStream_t stream) {
+  return tfce_run_ex(adj, d_image, d_enhn, 0.0f, dh, accumulate, -1, nullptr, nullptr, stream);
+}
+
+int tfce_batch_max(tfce_adj_t* adj, const float* d_maps, int P, int sign_mask, const float* d_vdensity,
+                   float* d_max, hipStream_t stream) {
+  if (!adj || !d_maps || !d_max) return set_error(TFCE_EINVAL, "NULL argument");
+  if (P <= 0) return 0;
+  if (sign_mask < 1 || sign_mask > 3) return set_error(TFCE_EINVAL, "sign_mask must be 1, 2 or 3");
+  if (sign_mask != 3) {
+    // the unrequested sign reads 0; the requested one lands at 2p + (sign_mask == 2)
+    hipError_t e = launch_fill_f32(d_max, (int64_t)P * 2, 0.0f, stream);
+    if (e != hipSuccess) return set_error(-(int)e, "fill max");
+  }
+  return run_batch(adj->d_surf, 1, adj->V, d_maps, adj->V, P, sign_mask, nullptr, d_vdensity,
+                   d_vdensity != nullptr, nullptr, 0.0f, 0.0f, -1, nullptr, d_max, nullptr, nullptr, nullptr,
+                   stream, sign_mask == 3 ? 1 : 2, sign_mask == 2 ? 1 : 0);
+}
+
+int tfce_surfset_create(const tfce_adj_t* const* adjs, int32_t n, const int64_t* offsets, tfce_surfset_t** out) {
+  if (!out || !adjs || !offsets || n <= 0) return set_error(TFCE_EINVAL, "bad surface set arguments");
+  *out = nullptr;
+  tfce_surfset* s = new tfce_surfset();
+  s->n = n;
+  s->device = current_device();
+  std::vector<int32_t> idx(n);
+  for (int32_t i = 0; i < n; ++i) {
+    if (!adjs[i]) { delete s; return set_error(TFCE_EINVAL, "NULL adjacency in set"); }
+    idx[i] = i;
+  }
+  std::stable_sort(idx.begin(), idx.end(), [&](int32_t a, int32_t b) { return adjs[a]->V > adjs[b]->V; });
+  for (int32_t k = 0; k < n; ++k) {
+    const tfce_adj* a = adjs[idx[k]];
+    SurfDev sd;
+    sd.row_ptr = a->d_row_ptr;
+    sd.col = a->d_col;
+    sd.offset = offsets[idx[k]];
+    sd.V = a->V;
+    sd.flags = a->flags;
+    sd.H = a->H;
+    sd.E = a->E;
+    s->host.push_back(sd);
+    s->user_index.push_back(idx[k]);
+    s->v_max = std::max(s->v_max, a->V);
+  }
+  hipError_t e = hipMalloc(&s->d_surfs, sizeof(SurfDev) * n);
+  if (e == hipSuccess) e = hipMemcpy(s->d_surfs, s->host.data(), sizeof(SurfDev) * n, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    if (s->d_surfs) (void)hipFree(s->d_surfs);
+    delete s;
+    return set_error(-(int)e, "upload surface set");
+  }
+  *out = s;
+  return 0;
+}
+
+int tfce_surfset_destroy(tfce_surfset_t* s) {
+  if (!s) return 0;
+  (void)hipFree(s->d_surfs);
+  delete s;
+  return 0;
+}
+
+int tfce_surfset_order(const tfce_surfset_t* s, int32_t* user_index) {
+  if (!s || !user_index) return set_error(TFCE_EINVAL, "NULL argument");
+  for (int32_t k = 0; k < s->n; ++k) user_index[k] = s->user_index[k];
+  return 0;
+}
+
+int tfce_batch(const tfce_surfset_t* s, const float* d_maps, int64_t map_stride, int P, int sign_mask,
+               const float* d_T0, const float* d_vdensity, float* d_scaled, float* d_raw, float* d_own,
+               int32_t* d_status, hipStream_t stream) {
+  if (!s || !d_maps || !d_scaled) return set_error(TFCE_EINVAL, "NULL argument");
+  return run_batch(s->d_surfs, s->n, s->v_max, d_maps, map_stride, P, sign_mask, d_T0, d_vdensity,
+                   d_vdensity != nullptr, nullptr, 0.0f, 0.0f, -1, nullptr, d_scaled, d_raw, d_own, d_status,
+                   stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,67 @@
+// Internal (non-ABI) declarations shared by the HIP kernels and the C-ABI layer.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tfcehip {
+
+// Level tables live in LDS; the reference visits about 101 levels
+// (fast_tfce.hpp:34-39 with deltaT = max/100), an explicit step may ask for more.
+constexpr int kMaxLevels = 1024;
+constexpr int32_t kNever = 0x7fffffff;
+constexpr int kTfceThreads = 256;
+
+// Status codes written per work item (>= 0: number of levels visited).
+constexpr int32_t kStatusDegenerate = -2;  // max >= minT with a step <= 0: the reference never returns
+constexpr int32_t kStatusTooManyLevels = -3;
+
+// Surface flags.
+constexpr int32_t kSurfSymmetric = 1;     // every edge is listed from both ends
+constexpr int32_t kSurfZeroInLevels = 2;  // the full image holds zeros outside this element set
+constexpr int32_t kSurfZeroInScale = 4;   // the rescale max includes those zeros
+
+struct SurfDev {
+  const int32_t* row_ptr;  // V+1
+  const int32_t* col;      // nnz (indices local to the surface)
+  int64_t offset;          // first element of this surface inside a map row
+  int32_t V;
+  int32_t flags;
+  float H, E;
+};
+
+struct TfceBatchArgs {
+  const SurfDev* surfs;    // device array [n_surf]
+  int32_t n_surf;
+  int32_t n_maps;
+  int32_t n_signs;         // 1 or 2 (2 = +map then -map)
+  float sign0;             // sign used when n_signs == 1
+  const float* maps;       // map p starts at maps + p * map_stride
+  int64_t map_stride;
+  const float* ext_T0;     // optional [n_items]: first threshold (shared-threshold mode)
+  const float* vdensity;   // optional per-element weights (map column indexing), FULL mode
+  float min_t;
+  float dh;                // 0: dh = fl(T0 / 100) as the reference
+  int32_t full_scale;      // FULL mode: compute the rescaled maximum
+  // outputs, one entry per item; item = (surface * n_maps + map) * n_signs + sign
+  float* raw_max;          // max over elements of the TFCE sum (no rescale)
+  float* scaled_max;       // max over elements of fl(fl(tfce * fl(smax/100)) * vd)
+  int32_t out_mul, out_add;  // scaled_max index = item * out_mul + out_add
+  float* own_max;          // max over this surface's elements of sign*map (NaN ignored)
+  int32_t* status;
+  float* enhn;             // FULL mode: accumulate target, same layout as maps (may be null)
+  int32_t labels_level;    // FULL mode debug: component labels after this level
+  int32_t* labels;         // [V] vertex id of a component representative, -1 inactive
+  // scratch
+  int32_t* scratch;
+  int64_t slot_stride;     // int32 words per slot
+  int32_t v_max;
+  int32_t* queue;          // dynamic work counter, zeroed before launch
+  int64_t n_items;
+};
+
+hipError_t launch_tfce_batch(const TfceBatchArgs& a, bool full, int grid, hipStream_t stream);
+hipError_t launch_fill_f32(float* p, int64_t n, float v, hipStream_t stream);
+
+inline int64_t tfce_slot_words(int32_t v_max) { return (int64_t)8 * ((v_max + 63) & ~63); }
+
+}  // namespace tfcehip

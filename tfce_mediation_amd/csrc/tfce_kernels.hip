@@ -1,0 +1,393 @@
+// Batched TFCE on CDNA4 (gfx950).
+//
+// Semantics follow tfce_mediation/lib/fast_tfce.hpp:11-95 (the operator behind
+// CreateAdjSet.run, tfce.pyx:24-45): thresholds T_0 = max, T_{i+1} = fl(T_i - dT)
+// while T_i >= minT, dT = fl(max/100); a vertex is active at T iff x > T; each
+// member of a component of size s gains (float)(s^E * fl(T^H)) per level, added
+// in float in descending-T order.
+//
+// Design (not the reference's): one work item = one (surface, map, sign) image.
+// A persistent grid pulls items from an atomic queue; each workgroup owns one
+// scratch slot and runs the whole threshold sweep of its item:
+//   1. max-reduce the image, build the float threshold table in LDS;
+//   2. bucket the vertices by activation level (counting sort, LDS histogram);
+//      compact id = position in that order, so the active set at level l is the
+//      id prefix [0, end_l) and "is u active" is `pos[u] < end_l`;
+//   3. per level: hook the newly active vertices into a lock-free union-find
+//      (CAS hooking larger id under smaller, path halving), merge component
+//      sizes into the surviving roots with atomics, then
+//        MAX mode : add the level increment to one value per root; a merged
+//                   root keeps the larger value (atomicMax on the float bits);
+//                   the largest TFCE of a component is held by its earliest
+//                   vertex, so max over roots == max over vertices, bit-exact;
+//        FULL mode: add the increment of its root to every active vertex.
+// Every level touches only the new vertices and the current roots (MAX) or the
+// active prefix (FULL) - never the whole graph.
+#include "tfce_internal.h"
+
+namespace tfcehip {
+
+namespace {
+
+__device__ __forceinline__ int ld(const int32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void st(int32_t* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ int uf_find(int32_t* parent, int x) {
+  int p = ld(parent + x);
+  while (p != x) {
+    int gp = ld(parent + p);
+    if (gp != p) st(parent + x, gp);  // path halving; gp is an ancestor of x
+    x = p;
+    p = gp;
+  }
+  return x;
+}
+
+__device__ __forceinline__ void uf_union(int32_t* parent, int a, int b) {
+  while (true) {
+    a = uf_find(parent, a);
+    b = uf_find(parent, b);
+    if (a == b) return;
+    if (a < b) { int t = a; a = b; b = t; }
+    int expected = a;
+    // hook root a (larger id) under b; fails if a stopped being a root
+    if (__hip_atomic_compare_exchange_strong(parent + a, &expected, b, __ATOMIC_RELAXED,
+                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
+      return;
+  }
+}
+
+__device__ __forceinline__ float pow_h(float T, float H) {
+  if (H == 2.0f) return __fmul_rn(T, T);
+  if (H == 1.0f) return T;
+  return powf(T, H);
+}
+
+// (float)(pow((double)size, (double)E) * (double)HH), fast_tfce.hpp:77
+__device__ __forceinline__ float tfce_inc(int size, float HH, float E) {
+  if (E == 1.0f) return __fmul_rn((float)size, HH);  // exact product, one rounding
+  double s = (double)size;
+  double p = (E == 0.5f) ? sqrt(s) : pow(s, (double)E);
+  return __double2float_rn(p * (double)HH);
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// max over the workgroup; NaN inputs are dropped by fmaxf
+__device__ float block_max(float v, float* s_red) {
+  v = wave_max(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) s_red[w] = v;
+  __syncthreads();
+  float r = s_red[0];
+  for (int i = 1; i < kTfceThreads / 64; ++i) r = fmaxf(r, s_red[i]);
+  __syncthreads();
+  return r;
+}
+
+// Append `c` to list when `take` (wave-aggregated LDS counter).
+__device__ __forceinline__ void wave_append(bool take, int c, int32_t* list, int* s_count) {
+  unsigned long long m = __ballot(take);
+  if (m == 0ull) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((long long)m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(s_count, __popcll(m));
+  base = __shfl(base, leader, 64);
+  if (take) list[base + __popcll(m & ((1ull << lane) - 1ull))] = c;
+}
+
+template <bool FULL>
+__global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs a) {
+  __shared__ float s_T[kMaxLevels];
+  __shared__ float s_HH[kMaxLevels];
+  __shared__ int s_start[kMaxLevels + 1];
+  __shared__ int s_cur[kMaxLevels];
+  __shared__ float s_red[kTfceThreads / 64];
+  __shared__ int s_item_lo, s_item_hi, s_L, s_status, s_nroots_new;
+  __shared__ float s_own;
+
+  const int tid = threadIdx.x;
+  const int64_t vstride = (a.v_max + 63) & ~63;
+  int32_t* slot = a.scratch + (int64_t)blockIdx.x * a.slot_stride;
+  int32_t* pos = slot;
+  int32_t* order = slot + vstride;
+  int32_t* parent = slot + 2 * vstride;
+  int32_t* size = slot + 3 * vstride;
+  float* val = reinterpret_cast<float*>(slot + 4 * vstride);
+  int32_t* roots[2] = {slot + 5 * vstride, slot + 6 * vstride};
+  float* incv = reinterpret_cast<float*>(slot + 7 * vstride);
+
+  while (true) {
+    if (tid == 0) {
+      unsigned long long it = atomicAdd(reinterpret_cast<unsigned long long*>(a.queue), 1ull);
+      s_item_lo = (int)(it & 0x7fffffff);
+      s_item_hi = (int)(it >> 31);
+    }
+    __syncthreads();
+    const int64_t item = ((int64_t)s_item_hi << 31) | s_item_lo;
+    if (item >= a.n_items) break;  // uniform across the workgroup
+
+    const int per_surf = a.n_maps * a.n_signs;
+    const int s = (int)(item / per_surf);
+    const int rem = (int)(item - (int64_t)s * per_surf);
+    const int p = rem / a.n_signs;
+    const float sign = (a.n_signs == 2) ? ((rem % 2) ? -1.0f : 1.0f) : a.sign0;
+    const SurfDev S = a.surfs[s];
+    const int V = S.V;
+    const float* x = a.maps + (int64_t)p * a.map_stride + S.offset;
+    float* enhn = (FULL && a.enhn) ? a.enhn + (int64_t)p * a.map_stride + S.offset : nullptr;
+    const float* vd = (FULL && a.vdensity) ? a.vdensity + S.offset : nullptr;
+    const bool symmetric = (S.flags & kSurfSymmetric) != 0;
+
+    // ---- phase 0: maximum and level table -----------------------------------
+    float m = -INFINITY;
+    for (int v = tid; v < V; v += kTfceThreads) m = fmaxf(m, sign * x[v]);
+    m = block_max(m, s_red);
+    if (tid == 0) {
+      s_own = m;
+      float T0 = a.ext_T0 ? a.ext_T0[item] : ((S.flags & kSurfZeroInLevels) ? fmaxf(m, 0.0f) : m);
+      float dT = (a.dh != 0.0f) ? a.dh : __fdiv_rn(T0, 100.0f);
+      int L = 0, status = 0;
+      if (T0 >= a.min_t) {
+        if (!(dT > 0.0f)) {
+          status = kStatusDegenerate;
+        } else {
+          float T = T0;
+          while (T >= a.min_t) {
+            if (L == kMaxLevels) { status = kStatusTooManyLevels; break; }
+            s_T[L] = T;
+            s_HH[L] = pow_h(T, S.H);
+            ++L;
+            T = __fsub_rn(T, dT);
+          }
+        }
+      }
+      if (status < 0) L = 0; else status = L;
+      s_L = L;
+      s_status = status;
+    }
+    __syncthreads();
+    const int L = s_L;
+
+    // ---- phase 1: activation level of every element, histogram --------------
+    for (int l = tid; l < L; l += kTfceThreads) s_cur[l] = 0;
+    __syncthreads();
+    if (L > 0) {
+      const float Tlast = s_T[L - 1];
+      for (int v = tid; v < V; v += kTfceThreads) {
+        const float xv = sign * x[v];
+        int lev = kNever;
+        if (xv > Tlast) {  // false for NaN
+          int lo = 0, hi = L - 1;
+          while (lo < hi) {
+            int mid = (lo + hi) >> 1;
+            if (xv > s_T[mid]) hi = mid; else lo = mid + 1;
+          }
+          lev = lo;
+          atomicAdd(&s_cur[lev], 1);
+        }
+        pos[v] = lev;
+      }
+    }
+    __syncthreads();
+    // exclusive scan of the histogram by wave 0
+    if (tid < 64) {
+      const int per = (L + 63) / 64;
+      const int b = tid * per, e = min(L, b + per);
+      int sum = 0;
+      for (int l = b; l < e; ++l) sum += s_cur[l];
+      int incl = sum;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        int t = __shfl_up(incl, o, 64);
+        if (tid >= o) incl += t;
+      }
+      int run = incl - sum;
+      for (int l = b; l < e; ++l) {
+        const int c = s_cur[l];
+        s_start[l] = run;
+        s_cur[l] = run;
+        run += c;
+      }
+      if (tid == 63) s_start[L] = incl;
+    }
+    __syncthreads();
+    // ---- phase 2: scatter into level order ----------------------------------
+    if (L > 0) {
+      for (int v = tid; v < V; v += kTfceThreads) {
+        const int lev = pos[v];
+        if (lev != kNever) {
+          const int q = atomicAdd(&s_cur[lev], 1);
+          order[q] = v;
+          pos[v] = q;
+        }
+      }
+    }
+    if (tid == 0) s_nroots_new = 0;
+    __syncthreads();
+
+    // ---- phase 3: threshold sweep -------------------------------------------
+    int n_roots = 0;
+    int cur = 0;
+    for (int l = 0; l < L; ++l) {
+      const int b0 = s_start[l], b1 = s_start[l + 1];
+      const float HH = s_HH[l];
+      int32_t* rold = roots[cur];
+      int32_t* rnew = roots[cur ^ 1];
+      if (b1 > b0) {
+        // 3a: make sets for the new vertices
+        for (int i = b0 + tid; i < b1; i += kTfceThreads) {
+          parent[i] = i;
+          size[i] = 1;
+          if (FULL) val[i] = enhn ? enhn[order[i]] : 0.0f;
+          else val[i] = 0.0f;
+        }
+        __syncthreads();
+        // 3b: join with already active neighbours
+        for (int i = b0 + tid; i < b1; i += kTfceThreads) {
+          const int v = order[i];
+          const int e0 = S.row_ptr[v], e1 = S.row_ptr[v + 1];
+          const float xv = symmetric ? 0.0f : sign * x[v];
+          for (int e = e0; e < e1; ++e) {
+            const int u = S.col[e];
+            const int q = pos[u];
+            if (q >= b1) continue;  // not active (kNever is never < b1)
+            if (!symmetric && q >= b0) {
+              // same level, edge only listed from v: the reference uses it iff u
+              // was activated first (descending value, ties by ascending index)
+              const float xu = sign * x[u];
+              if (!(xu > xv || (xu == xv && u < v))) continue;
+            }
+            uf_union(parent, i, q);
+          }
+        }
+        __syncthreads();
+        // 3c: merge sizes (and MAX values) into surviving roots; collect roots
+        const int n_cand = n_roots + (b1 - b0);
+        for (int k = tid; k < n_cand + ((kTfceThreads - n_cand % kTfceThreads) % kTfceThreads);
+             k += kTfceThreads) {
+          bool take = false;
+          int c = 0;
+          if (k < n_cand) {
+            c = (k < n_roots) ? rold[k] : b0 + (k - n_roots);
+            const int r = uf_find(parent, c);
+            if (r != c) {
+              __hip_atomic_fetch_add(size + r, ld(size + c), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+              if (!FULL)
+                __hip_atomic_fetch_max(reinterpret_cast<int32_t*>(val) + r,
+                                       ld(reinterpret_cast<const int32_t*>(val) + c), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+            } else {
+              take = true;
+            }
+          }
+          wave_append(take, c, rnew, &s_nroots_new);
+        }
+        __syncthreads();
+        n_roots = s_nroots_new;
+        __syncthreads();
+        if (tid == 0) s_nroots_new = 0;
+        cur ^= 1;
+      }
+      int32_t* rl = roots[cur];
+      // 3d: level increment
+      if (!FULL) {
+        for (int k = tid; k < n_roots; k += kTfceThreads) {
+          const int r = rl[k];
+          const float cur_v = __int_as_float(ld(reinterpret_cast<const int32_t*>(val) + r));
+          val[r] = cur_v + tfce_inc(ld(size + r), HH, S.E);
+        }
+        __syncthreads();
+      } else {
+        for (int k = tid; k < n_roots; k += kTfceThreads) {
+          const int r = rl[k];
+          incv[r] = tfce_inc(ld(size + r), HH, S.E);
+        }
+        __syncthreads();
+        for (int i = tid; i < b1; i += kTfceThreads) {
+          const int r = uf_find(parent, i);
+          val[i] = val[i] + incv[r];
+        }
+        if (a.labels && l == a.labels_level) {
+          for (int i = tid; i < b1; i += kTfceThreads) a.labels[order[i]] = order[uf_find(parent, i)];
+        }
+        __syncthreads();
+      }
+    }
+
+    // ---- finalize -----------------------------------------------------------
+    const int n_act = (L > 0) ? s_start[L] : 0;
+    float best = 0.0f;
+    if (!FULL) {
+      int32_t* rl = roots[cur];
+      for (int k = tid; k < n_roots; k += kTfceThreads)
+        best = fmaxf(best, __int_as_float(ld(reinterpret_cast<const int32_t*>(val) + rl[k])));
+    } else {
+      for (int i = tid; i < n_act; i += kTfceThreads) best = fmaxf(best, val[i]);
+      if (enhn)
+        for (int i = tid; i < n_act; i += kTfceThreads) enhn[order[i]] = val[i];
+    }
+    best = block_max(best, s_red);
+    const float own = s_own;
+    const float smax = (S.flags & kSurfZeroInScale) ? fmaxf(own, 0.0f) : own;
+    const float sc = __fdiv_rn(smax, 100.0f);
+    float scaled;
+    if (FULL && vd) {
+      // max over all elements of (tfce * sc) * vd; inactive elements hold tfce = 0
+      float mv = -INFINITY;
+      for (int v = tid; v < V; v += kTfceThreads) {
+        const int q = pos[v];
+        const float t = (L > 0 && q != kNever) ? val[q] : 0.0f;
+        mv = fmaxf(mv, __fmul_rn(__fmul_rn(t, sc), vd[v]));
+      }
+      scaled = block_max(mv, s_red);
+    } else {
+      scaled = __fmul_rn(best, sc);
+    }
+    if (tid == 0) {
+      a.raw_max[item] = best;
+      if (a.scaled_max) a.scaled_max[item * a.out_mul + a.out_add] = scaled;
+      if (a.own_max) a.own_max[item] = own;
+      if (a.status) a.status[item] = s_status;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void fill_f32_kernel(float* p, int64_t n, float v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void zero_queue_kernel(int32_t* q) { q[0] = 0; q[1] = 0; }
+
+}  // namespace
+
+hipError_t launch_tfce_batch(const TfceBatchArgs& a, bool full, int grid, hipStream_t stream) {
+  hipLaunchKernelGGL(zero_queue_kernel, dim3(1), dim3(1), 0, stream, a.queue);
+  if (full)
+    hipLaunchKernelGGL(tfce_batch_kernel<true>, dim3(grid), dim3(kTfceThreads), 0, stream, a);
+  else
+    hipLaunchKernelGGL(tfce_batch_kernel<false>, dim3(grid), dim3(kTfceThreads), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_f32(float* p, int64_t n, float v, hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(fill_f32_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, p, n, v);
+  return hipGetLastError();
+}
+
+}  // namespace tfcehip

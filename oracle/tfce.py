@@ -32,7 +32,8 @@ def _load():
         lib.oracle_tfce.restype = ctypes.c_int64
         lib.oracle_tfce.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                     ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
-                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
+                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                    ctypes.c_int]
         lib.oracle_tfce_levels.restype = ctypes.c_int64
         lib.oracle_tfce_levels.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                            ctypes.c_void_p, ctypes.c_int64]
@@ -70,8 +71,13 @@ def levels(max_t, min_t=0.0, delta_t=0.0):
     return out[:n]
 
 
-def tfce(row_ptr, col, image, H, E, enhn=None, min_t=0.0, delta_t=0.0, labels_level=-1):
-    """enhn += TFCE(image); returns (enhn, n_levels_or_status, labels or None)."""
+def tfce(row_ptr, col, image, H, E, enhn=None, min_t=0.0, delta_t=0.0, labels_level=-1,
+         exact_pow=False):
+    """enhn += TFCE(image); returns (enhn, n_levels_or_status, labels or None).
+
+    exact_pow=False reproduces the reference bit for bit (libm powf for T^H);
+    exact_pow=True uses the correctly rounded T^H, the GPU's definition.
+    """
     lib = _load()
     row_ptr, col = _csr(row_ptr, col)
     image = np.ascontiguousarray(image, dtype=np.float32)
@@ -85,7 +91,7 @@ def tfce(row_ptr, col, image, H, E, enhn=None, min_t=0.0, delta_t=0.0, labels_le
     st = lib.oracle_tfce(np.float32(H), np.float32(E), np.float32(min_t), np.float32(delta_t),
                          V, row_ptr.ctypes.data, col.ctypes.data, image.ctypes.data,
                          enhn.ctypes.data, labels_level,
-                         labels.ctypes.data if labels is not None else None)
+                         labels.ctypes.data if labels is not None else None, 1 if exact_pow else 0)
     return enhn, st, labels
 
 

@@ -24,6 +24,10 @@
  *   - NaN elements never activate and do not take part in max();
  *   - ties are activated in ascending vertex order (std::sort leaves them
  *     unspecified; only an asymmetric adjacency can observe the difference).
+ * pow_mode 0 computes HH with libm powf exactly as the reference does; glibc's
+ * powf(T, 2) is one ulp away from the correctly rounded T*T for ~0.06% of
+ * float inputs.  pow_mode 1 uses the correctly rounded power
+ * (float)pow((double)T, (double)H) instead, which is what the GPU computes.
  *
  * The data structure is my own: a union-find with union by size and path
  * halving over a value-sorted activation order, not the reference's
@@ -86,7 +90,7 @@ int64_t oracle_tfce_levels(float maxT, float minT, float deltaT, float *out, int
 int64_t oracle_tfce(float H, float E, float minT, float deltaT, int32_t V,
                     const int64_t *row_ptr, const int32_t *col,
                     const float *image, float *enhn,
-                    int64_t labels_level, int32_t *labels) {
+                    int64_t labels_level, int32_t *labels, int pow_mode) {
     if (V <= 0) return 0;
     kv_t *ord = (kv_t *)malloc(sizeof(kv_t) * (size_t)V);
     int32_t *parent = (int32_t *)malloc(sizeof(int32_t) * (size_t)V);
@@ -128,7 +132,7 @@ int64_t oracle_tfce(float H, float E, float minT, float deltaT, int32_t V,
             }
             ++j;
         }
-        float HH = powf(T, H);
+        float HH = pow_mode ? (float)pow((double)T, (double)H) : powf(T, H);
         for (int32_t q = 0; q < n_active; ++q) {
             int32_t v = active_list[q];
             if (parent[v] == v) inc[v] = (float)(pow((double)size[v], (double)E) * (double)HH);

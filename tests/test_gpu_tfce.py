@@ -1,9 +1,11 @@
 """GPU parity of the TFCE sweep (libtfcehip) against the C oracle.
 
-Bar: bit-exact float32 for H=2 with E=1 (every increment is one correctly
-rounded product, as in the reference); E=0.5 / 0.67 use sqrt / pow on the GPU
-where the reference calls glibc pow, so those allow 4 ulp.  Component
-partitions must be identical at every checked level.
+Bar: bit-exact float32 against the oracle in correctly-rounded-power mode for
+H=2 with E=1 (every increment is one correctly rounded product); E=0.5 / 0.67
+use sqrt / pow on the GPU where the oracle calls glibc pow, so those allow
+4 ulp.  Against the reference-identical oracle (libm powf for T^H, one ulp off
+for ~0.06% of thresholds) the bar is 4 ulp.  Component partitions must be
+identical at every checked level.
 """
 import sys
 
@@ -78,9 +80,11 @@ def test_run_matches_oracle(torch, gname, HE):
             x = (sgn * img).astype(np.float32)
             enh = np.zeros_like(x)
             op.run(x, enh)
-            ref, st, _ = otfce.tfce(row_ptr, col, x, H, E)
+            ref, st, _ = otfce.tfce(row_ptr, col, x, H, E, exact_pow=True)
             assert st > 0
             check_close(enh, ref, E, H)
+            refpow, _, _ = otfce.tfce(row_ptr, col, x, H, E)
+            assert ulp_diff(enh, refpow).max() <= 4
 
 
 def test_run_accumulates_and_degenerate(torch):
@@ -93,7 +97,7 @@ def test_run_accumulates_and_degenerate(torch):
     base = np.linspace(-1, 1, V).astype(np.float32)
     enh = base.copy()
     op.run(img, enh)
-    ref, _, _ = otfce.tfce(row_ptr, col, img, 2.0, 1.0, enhn=base.copy())
+    ref, _, _ = otfce.tfce(row_ptr, col, img, 2.0, 1.0, enhn=base.copy(), exact_pow=True)
     check_close(enh, ref, 1.0, 2.0)
     enh = base.copy()
     op.run(np.zeros(V, np.float32), enh)  # max == 0: the reference never returns; defined as no-op
@@ -153,7 +157,7 @@ def test_batch_max_matches_oracle(torch, gname, E):
     for p in range(P):
         for k, sgn in enumerate((1, -1)):
             x = (sgn * maps[p]).astype(np.float32)
-            enh, _, _ = otfce.tfce(row_ptr, col, x, 2.0, E)
+            enh, _, _ = otfce.tfce(row_ptr, col, x, 2.0, E, exact_pow=True)
             s = np.float32(x.max() / np.float32(100))
             exp = np.float32(enh.max() * s)
             expv = ((enh * s) * vd).max()
@@ -178,5 +182,5 @@ def test_large_hemisphere_maps(torch):
     for p in range(P):
         for k, sgn in enumerate((1, -1)):
             x = (sgn * maps[p]).astype(np.float32)
-            enh, _, _ = otfce.tfce(row_ptr, col, x, 2.0, 1.0)
+            enh, _, _ = otfce.tfce(row_ptr, col, x, 2.0, 1.0, exact_pow=True)
             assert out[p, k] == np.float32(enh.max() * np.float32(x.max() / np.float32(100)))

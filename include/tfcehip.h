@@ -101,6 +101,43 @@ int tfce_batch(const tfce_surfset_t *set, const float *d_maps, int64_t map_strid
                int sign_mask, const float *d_T0, const float *d_vdensity, float *d_scaled,
                float *d_raw, float *d_own, int32_t *d_status, hipStream_t stream);
 
+/* ----------------------------------------------------------- permuted OLS */
+/* Precision of the permuted-design contraction (every mode accumulates in
+ * FP32; Y'Y, the residual sum of squares and the k x k factors are FP64). */
+#define TFCE_PREC_FP32 0    /* FP32 matrix cores */
+#define TFCE_PREC_BF16X3 1  /* hi/lo bf16 split, three products */
+#define TFCE_PREC_BF16X6 2  /* three-piece bf16 split, six products */
+
+typedef struct tfce_ols_data tfce_ols_data_t;
+typedef struct tfce_ols_model tfce_ols_model_t;
+
+/* Device-resident copy of Y (n subjects x V elements, row stride ldy, the
+ * merge_y layout of STEP_1_vertex_tfce_multiple_regression.py:263), centred
+ * per element when center != 0 (exact for models with an intercept), with
+ * FP64 y'y and the split planes the chosen precision needs.  Enqueued on
+ * stream; d_Y may be freed once the stream has passed this call. */
+int tfce_ols_data_create(const float *d_Y, int32_t n, int64_t V, int64_t ldy, int32_t precision,
+                         int32_t center, tfce_ols_data_t **out, hipStream_t stream);
+int tfce_ols_data_destroy(tfce_ols_data_t *data);
+
+/* Design X (host, n x k row-major float64, k <= 8) bound to data.  Bit j of
+ * perm_mask marks column j as permuted (the reference permutes every row of
+ * X, vertex_tfce_multiple_regression_randomise.py:105, i.e. all columns; -v
+ * permutes a subset, :93-97).  contrasts lists the columns whose t is wanted
+ * (the reference's tnum = 1..k-1, :114-117).  Blocking. */
+int tfce_ols_model_create(const tfce_ols_data_t *data, const double *X, int32_t k, uint32_t perm_mask,
+                          const int32_t *contrasts, int32_t n_contrasts, tfce_ols_model_t **out,
+                          hipStream_t stream);
+int tfce_ols_model_destroy(tfce_ols_model_t *model);
+int tfce_ols_info(const tfce_ols_model_t *model, int32_t *k, int32_t *n_fixed, int32_t *shared_gram);
+
+/* Replaces tval_int (cynumstats.pyx:59-64) on X[perm_p] for P permutations at
+ * once: d_t[(c * P + p) * ldt + v] = t of contrast c, permutation p, element v
+ * (float32, se rounded to float32 as cynumstats.pyx:49 does).  d_perm is P x n
+ * int32 (row i of the permuted design is X[perm[i]]); NULL = identity, P = 1. */
+int tfce_ols_perm_t(const tfce_ols_model_t *model, const int32_t *d_perm, int32_t P, float *d_t,
+                    int64_t ldt, hipStream_t stream);
+
 /* Number of resident workgroups per CU used by the TFCE sweep (autotuner knob). */
 int tfce_set_occupancy(int32_t wgs_per_cu);
 

@@ -1,0 +1,107 @@
+"""GPU permuted OLS t-values (tfce_ols_perm_t) against the FP64 oracle.
+
+Error metrics (t has no natural scale near 0, so two are used):
+  norm  = |t_gpu - t64|_2 / |t64|_2 over the whole batch;
+  point = max |t_gpu - t64| / |t64| over entries with |t64| >= 1.
+Bars: FP32 and BF16x6 (3-piece split, 6 products) <= 1e-5 on both;
+BF16x3 (hi/lo split, 3 products: ~2^-16 relative per operand) norm <= 2e-5.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import ols as ools
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as _t
+    assert _t.cuda.is_available()
+    return _t
+
+
+def make(n, k, V, seed, offset=0.0):
+    rng = np.random.default_rng(seed)
+    X = np.column_stack([np.ones(n)] + [rng.standard_normal(n) for _ in range(k - 2)]
+                        + [(rng.random(n) < 0.5).astype(float)])
+    Y = rng.standard_normal((n, V)) + 0.4 * X[:, -1:] * (rng.random(V) < 0.2) + offset
+    return X.astype(np.float32).astype(np.float64), Y.astype(np.float32)
+
+
+def errors(t, ref):
+    t = t.astype(np.float64)
+    norm = np.linalg.norm(t - ref) / np.linalg.norm(ref)
+    big = np.abs(ref) >= 1
+    point = np.max(np.abs(t - ref)[big] / np.abs(ref[big]))
+    return norm, point
+
+
+BARS = {"fp32": (1e-5, 1e-5), "bf16x6": (1e-5, 1e-5), "bf16x3": (2e-5, None)}
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3", "bf16x6"])
+@pytest.mark.parametrize("n,k,V,P", [(100, 4, 3000, 37), (160, 6, 2500, 20), (75, 5, 1030, 9)])
+def test_perm_t_matches_fp64(torch, prec, n, k, V, P):
+    from tfce_mediation_amd.ols import PermutedOLS
+    say(f"[gpu] ols {prec} n={n} k={k} V={V} P={P}")
+    X, Y = make(n, k, V, seed=n + k)
+    rng = np.random.default_rng(7)
+    perms = np.stack([rng.permutation(n) for _ in range(P)]).astype(np.int32)
+    con = list(range(1, k))
+    model = PermutedOLS(torch.from_numpy(Y).cuda(), X, contrasts=con, precision=prec)
+    t = model.t(torch.from_numpy(perms).cuda()).cpu().numpy()
+    ref = ools.perm_t(X, Y, perms, con)
+    norm, point = errors(t, ref)
+    say(f"    norm={norm:.2e} point={point:.2e}")
+    nb, pb = BARS[prec]
+    assert norm <= nb
+    if pb is not None:
+        assert point <= pb
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_partial_permutation_and_offset(torch, prec):
+    """-v mode (only columns 2..3 move: per-permutation Gram factors) and a
+    large per-element mean (centring)."""
+    from tfce_mediation_amd.ols import PermutedOLS
+    n, k, V, P = 90, 5, 1500, 12
+    say(f"[gpu] ols partial permutation {prec}")
+    X, Y = make(n, k, V, seed=3, offset=40.0)
+    rng = np.random.default_rng(2)
+    perms = np.stack([rng.permutation(n) for _ in range(P)]).astype(np.int32)
+    con = [1, 2, 3, 4]
+    model = PermutedOLS(torch.from_numpy(Y).cuda(), X, contrasts=con, perm_columns=[2, 3], precision=prec)
+    t = model.t(torch.from_numpy(perms).cuda()).cpu().numpy()
+    ref = ools.perm_t(X, Y, perms, con, perm_columns=[2, 3])
+    norm, point = errors(t, ref)
+    say(f"    norm={norm:.2e} point={point:.2e}")
+    assert norm <= BARS[prec][0] * 2
+
+
+def test_identity_is_tval_int_with_intercept(torch):
+    from tfce_mediation_amd.ols import PermutedOLS
+    n, k, V = 50, 3, 700
+    say("[gpu] ols identity incl. intercept contrast")
+    X, Y = make(n, k, V, seed=11, offset=3.0)
+    model = PermutedOLS(torch.from_numpy(Y).cuda(), X, contrasts=[0, 1, 2], precision="fp32")
+    t = model.t().cpu().numpy()[:, 0]
+    ref = ools.tval_int(X, np.linalg.inv(X.T @ X), Y.astype(np.float64), n, k, V)
+    np.testing.assert_allclose(t, ref, rtol=2e-5, atol=1e-5)
+
+
+def test_constant_element_gives_nan_like_reference(torch):
+    from tfce_mediation_amd.ols import PermutedOLS
+    n, k, V = 40, 3, 64
+    X, Y = make(n, k, V, seed=4)
+    Y[:, 5] = 2.0
+    model = PermutedOLS(torch.from_numpy(Y).cuda(), X, contrasts=[1, 2], precision="bf16x3")
+    t = model.t().cpu().numpy()
+    assert np.all(np.isnan(t[:, 0, 5]))
+    assert np.all(np.isfinite(np.delete(t[:, 0, :], 5, axis=1)))

@@ -1,0 +1,90 @@
+"""Pins the FP64 OLS oracle (oracle/ols.py) with computations that share no
+code with it: least squares through QR / lstsq, scipy's linregress for the
+one-predictor case, and invariances the t statistic must satisfy."""
+import numpy as np
+import pytest
+from scipy import stats
+
+from oracle import ols
+
+
+def _data(n=60, k=4, V=50, seed=0):
+    rng = np.random.default_rng(seed)
+    X = np.column_stack([np.ones(n)] + [rng.standard_normal(n) for _ in range(k - 2)]
+                        + [(rng.random(n) < 0.5).astype(float)])
+    Y = rng.standard_normal((n, V)) + 0.3 * X[:, -1:] * rng.standard_normal(V)
+    return X, Y
+
+
+def _qr_t(X, Y):
+    Q, R = np.linalg.qr(X)
+    a = np.linalg.solve(R, Q.T @ Y)
+    res = Y - X @ a
+    s2 = (res ** 2).sum(0) / (X.shape[0] - X.shape[1])
+    Rinv = np.linalg.inv(R)
+    d = (Rinv ** 2).sum(1)
+    return a / np.sqrt(s2[None, :] * d[:, None])
+
+
+def test_tval_int_matches_qr():
+    X, Y = _data()
+    t = ols.tval_int(X, np.linalg.inv(X.T @ X), Y, X.shape[0], X.shape[1], Y.shape[1])
+    np.testing.assert_allclose(t, _qr_t(X, Y), rtol=2e-7)  # se is float32 in the reference
+
+
+def test_single_predictor_matches_linregress():
+    rng = np.random.default_rng(3)
+    n = 40
+    x = rng.standard_normal(n)
+    y = 0.4 * x + rng.standard_normal(n)
+    X = np.column_stack([np.ones(n), x])
+    t = ols.tval_int(X, np.linalg.inv(X.T @ X), y[:, None], n, 2, 1)
+    lr = stats.linregress(x, y)
+    assert t[1, 0] == pytest.approx(lr.slope / lr.stderr, rel=1e-6)
+    beta, se = ols.calc_beta_se(x, y[:, None], n, 1)
+    assert beta[0] == pytest.approx(lr.slope, rel=1e-10)
+    assert se[1, 0] == pytest.approx(lr.stderr, rel=1e-6)
+
+
+def test_invariances():
+    X, Y = _data(seed=5)
+    n, k = X.shape
+    t0 = ols.tval_int(X, np.linalg.inv(X.T @ X), Y, n, k, Y.shape[1])
+    X2 = X.copy(); X2[:, 1] *= -3.0
+    t1 = ols.tval_int(X2, np.linalg.inv(X2.T @ X2), Y, n, k, Y.shape[1])
+    np.testing.assert_allclose(t1[1], -t0[1], rtol=1e-6)
+    np.testing.assert_allclose(t1[2:], t0[2:], rtol=1e-6)
+    Y2 = Y + 2.5 * X[:, 2:3]  # add an in-model column: only that coefficient moves
+    t2 = ols.tval_int(X, np.linalg.inv(X.T @ X), Y2, n, k, Y.shape[1])
+    np.testing.assert_allclose(t2[[1, 3]], t0[[1, 3]], rtol=1e-6)
+    perm = np.random.default_rng(1).permutation(n)
+    tp = ols.perm_t(X, Y, [perm], [1, 2, 3])[:, 0]
+    nx = X[perm]
+    np.testing.assert_allclose(tp, _qr_t(nx, Y)[1:], rtol=2e-7)
+
+
+def test_resid_covars_orthogonal():
+    X, Y = _data(seed=2)
+    r = ols.resid_covars(X, Y.T)
+    assert np.abs(X.T @ r).max() < 1e-10
+    np.testing.assert_allclose(r, Y - X @ np.linalg.lstsq(X, Y, rcond=None)[0], atol=1e-10)
+
+
+def test_sobel_forms():
+    # equal path t-values: closed forms of the three variants
+    t = np.array([2.0, 3.0, 5.0])
+    np.testing.assert_allclose(ols.sobel_from_t(t, t, "sobel"), t / np.sqrt(2))
+    np.testing.assert_allclose(ols.sobel_from_t(t, t, "aroian"), 1 / np.sqrt(2 / t ** 2 + 1 / t ** 4))
+    np.testing.assert_allclose(ols.sobel_from_t(t, t, "goodman"), 1 / np.sqrt(2 / t ** 2 - 1 / t ** 4))
+    # type M: path b is the partial t of the mediator given x (symmetric in M and y)
+    rng = np.random.default_rng(9)
+    n = 80
+    x = rng.standard_normal(n)
+    m = 0.5 * x + rng.standard_normal(n)
+    y = 0.4 * m + rng.standard_normal(n)
+    z = ols.calc_sobelz("M", x, y, m[:, None], n, 1)
+    ta = stats.linregress(x, m).slope / stats.linregress(x, m).stderr
+    tb = _qr_t(np.column_stack([np.ones(n), y, x]), m[:, None])[1, 0]
+    tb2 = _qr_t(np.column_stack([np.ones(n), m, x]), y[:, None])[1, 0]
+    assert tb == pytest.approx(tb2, rel=1e-9)
+    assert z[0] == pytest.approx(1 / np.sqrt(1 / tb ** 2 + 1 / ta ** 2 + 1 / (ta ** 2 * tb ** 2)), rel=1e-6)

@@ -34,7 +34,15 @@ SIGNATURES = {
     "tfce_surfset_order": (c_int, [c_p, c_p]),
     "tfce_batch": (c_int, [c_p, c_p, c_i64, c_int, c_int, c_p, c_p, c_p, c_p, c_p, c_p, c_p]),
     "tfce_set_occupancy": (c_int, [c_i32]),
+    "tfce_ols_data_create": (c_int, [c_p, c_i32, c_i64, c_i64, c_i32, c_i32, c_p, c_p]),
+    "tfce_ols_data_destroy": (c_int, [c_p]),
+    "tfce_ols_model_create": (c_int, [c_p, c_p, c_i32, c_u32, c_p, c_i32, c_p, c_p]),
+    "tfce_ols_model_destroy": (c_int, [c_p]),
+    "tfce_ols_info": (c_int, [c_p, c_p, c_p, c_p]),
+    "tfce_ols_perm_t": (c_int, [c_p, c_p, c_i32, c_p, c_i64, c_p]),
 }
+
+PREC = {"fp32": 0, "bf16x3": 1, "bf16x6": 2}
 
 _lock = threading.Lock()
 _lib = None

@@ -1,0 +1,452 @@
+// C-ABI host layer of the permuted-OLS path: prepared data, design models,
+// batched permuted t-values.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/tfcehip.h"
+#include "capi_common.h"
+#include "ols_internal.h"
+
+using namespace tfcehip;
+
+struct tfce_ols_data {
+  int32_t n = 0, Kp = 0, precision = 0, pieces = 0, center = 0, device = 0;
+  int64_t V = 0, Npad = 0;
+  double* mu = nullptr;
+  double* yy = nullptr;
+  float* plane32 = nullptr;
+  uint16_t* bf[3] = {nullptr, nullptr, nullptr};
+};
+
+struct tfce_ols_model {
+  const tfce_ols_data* data = nullptr;
+  int32_t k = 0, f = 0, m = 0, n_con = 0;
+  int32_t con[kOlsMaxContrasts] = {0};
+  int32_t con_is_intercept[kOlsMaxContrasts] = {0};
+  bool shared = false;
+  double* F = nullptr;   // [n][f]
+  double* Z = nullptr;   // [n][m]
+  double* FF = nullptr;  // [f][f]
+  double* ZZ = nullptr;  // [m][m]
+  float* Z32 = nullptr;  // [n][m]
+  double* colsumZ = nullptr;
+  float* gF = nullptr;
+  float* hF = nullptr;
+  double* r0 = nullptr;
+  float* Linv1 = nullptr;  // shared factors
+  double* dc1 = nullptr;
+};
+
+namespace {
+
+std::mutex g_ols_mutex;
+struct OlsWs {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+std::map<std::pair<int, hipStream_t>, OlsWs> g_ols_ws;
+
+int ols_ws(hipStream_t stream, size_t bytes, void** out) {
+  std::lock_guard<std::mutex> g(g_ols_mutex);
+  OlsWs& w = g_ols_ws[std::make_pair(current_device(), stream)];
+  if (w.bytes < bytes) {
+    if (w.ptr) {
+      (void)hipStreamSynchronize(stream);
+      (void)hipFree(w.ptr);
+      w.ptr = nullptr;
+      w.bytes = 0;
+    }
+    hipError_t e = hipMalloc(&w.ptr, bytes);
+    if (e != hipSuccess) return set_error(-(int)e, std::string("hipMalloc(ols scratch): ") + hipGetErrorString(e));
+    w.bytes = bytes;
+  }
+  *out = w.ptr;
+  return 0;
+}
+
+template <class T>
+int dalloc(T** p, size_t count, const char* what) {
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * (count ? count : 1));
+  if (e != hipSuccess) return set_error(-(int)e, std::string("hipMalloc(") + what + "): " + hipGetErrorString(e));
+  return 0;
+}
+
+template <class T>
+int upload(T* d, const T* h, size_t count, const char* what) {
+  if (!count) return 0;
+  hipError_t e = hipMemcpy(d, h, sizeof(T) * count, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return set_error(-(int)e, std::string("upload ") + what + ": " + hipGetErrorString(e));
+  return 0;
+}
+
+// Lower Cholesky factor and its inverse (row-major k x k); false if not SPD.
+bool chol_inv(const std::vector<double>& G, int k, std::vector<double>& Li) {
+  std::vector<double> L(k * k, 0.0);
+  Li.assign(k * k, 0.0);
+  for (int j = 0; j < k; ++j) {
+    double d = G[j * k + j];
+    for (int q = 0; q < j; ++q) d -= L[j * k + q] * L[j * k + q];
+    if (!(d > 0.0)) return false;
+    d = std::sqrt(d);
+    L[j * k + j] = d;
+    for (int i = j + 1; i < k; ++i) {
+      double s = G[i * k + j];
+      for (int q = 0; q < j; ++q) s -= L[i * k + q] * L[j * k + q];
+      L[i * k + j] = s / d;
+    }
+  }
+  for (int j = 0; j < k; ++j) {
+    Li[j * k + j] = 1.0 / L[j * k + j];
+    for (int i = j + 1; i < k; ++i) {
+      double s = 0.0;
+      for (int q = j; q < i; ++q) s += L[i * k + q] * Li[q * k + j];
+      Li[i * k + j] = -s / L[i * k + i];
+    }
+  }
+  return true;
+}
+
+void free_model(tfce_ols_model* m) {
+  if (!m) return;
+  for (void* p : {(void*)m->F, (void*)m->Z, (void*)m->FF, (void*)m->ZZ, (void*)m->Z32, (void*)m->colsumZ,
+                  (void*)m->gF, (void*)m->hF, (void*)m->r0, (void*)m->Linv1, (void*)m->dc1})
+    if (p) (void)hipFree(p);
+  delete m;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tfce_ols_data_create(const float* d_Y, int32_t n, int64_t V, int64_t ldy, int32_t precision,
+                         int32_t center, tfce_ols_data_t** out, hipStream_t stream) {
+  if (!out || !d_Y) return set_error(TFCE_EINVAL, "NULL argument");
+  *out = nullptr;
+  if (n < 2 || V < 1 || ldy < V) return set_error(TFCE_EINVAL, "need n >= 2, V >= 1, ldy >= V");
+  if (precision < kPrecFP32 || precision > kPrecBF16x6) return set_error(TFCE_EINVAL, "unknown precision");
+  tfce_ols_data* d = new tfce_ols_data();
+  d->n = n;
+  d->Kp = (n + 15) / 16 * 16;
+  d->V = V;
+  d->Npad = (V + kColAlign - 1) / kColAlign * kColAlign;
+  d->precision = precision;
+  d->pieces = precision == kPrecFP32 ? 0 : ols_pieces(precision);
+  d->center = center ? 1 : 0;
+  d->device = current_device();
+  const size_t plane = (size_t)d->Kp * d->Npad;
+  int rc = 0;
+  if ((rc = dalloc(&d->mu, V, "mu")) || (rc = dalloc(&d->yy, V, "yy")) ||
+      (rc = dalloc(&d->plane32, plane, "plane32"))) {
+    tfce_ols_data_destroy(d);
+    return rc;
+  }
+  for (int p = 0; p < d->pieces; ++p)
+    if ((rc = dalloc(&d->bf[p], plane, "bf16 plane"))) {
+      tfce_ols_data_destroy(d);
+      return rc;
+    }
+  OlsPrepArgs a{};
+  a.Y = d_Y;
+  a.n = n;
+  a.Kp = d->Kp;
+  a.V = V;
+  a.ldy = ldy;
+  a.Npad = d->Npad;
+  a.center = d->center;
+  a.pieces = d->pieces;
+  a.mu = d->mu;
+  a.yy = d->yy;
+  a.plane32 = d->plane32;
+  for (int p = 0; p < 3; ++p) a.bf[p] = d->bf[p];
+  hipError_t e = launch_ols_prep(a, stream);
+  if (e != hipSuccess) {
+    tfce_ols_data_destroy(d);
+    return set_error(-(int)e, "ols prep launch");
+  }
+  *out = d;
+  return 0;
+}
+
+int tfce_ols_data_destroy(tfce_ols_data_t* d) {
+  if (!d) return 0;
+  for (void* p : {(void*)d->mu, (void*)d->yy, (void*)d->plane32, (void*)d->bf[0], (void*)d->bf[1], (void*)d->bf[2]})
+    if (p) (void)hipFree(p);
+  delete d;
+  return 0;
+}
+
+int tfce_ols_model_create(const tfce_ols_data_t* data, const double* X, int32_t k, uint32_t perm_mask,
+                          const int32_t* contrasts, int32_t n_contrasts, tfce_ols_model_t** out,
+                          hipStream_t stream) {
+  if (!out || !data || !X || !contrasts) return set_error(TFCE_EINVAL, "NULL argument");
+  *out = nullptr;
+  const int n = data->n;
+  if (k < 1 || k > kOlsMaxK) return set_error(TFCE_EINVAL, "design must have 1..8 columns");
+  if (n <= k) return set_error(TFCE_EINVAL, "need more subjects than design columns");
+  if (n_contrasts < 1 || n_contrasts > kOlsMaxContrasts) return set_error(TFCE_EINVAL, "1..8 contrasts");
+  if (perm_mask >= (1u << k)) return set_error(TFCE_EINVAL, "perm_mask names a column beyond k");
+  std::vector<int> order, is_fixed(k);
+  for (int j = 0; j < k; ++j)
+    if (!((perm_mask >> j) & 1u)) order.push_back(j);
+  const int f = (int)order.size();
+  for (int j = 0; j < k; ++j)
+    if ((perm_mask >> j) & 1u) order.push_back(j);
+  const int m = k - f;
+  if (m < 1) return set_error(TFCE_EINVAL, "at least one column must be permuted");
+  std::vector<int> internal(k);
+  for (int i = 0; i < k; ++i) internal[order[i]] = i;
+  // constant columns: the intercept the centred data relies on
+  std::vector<double> constval(k, 0.0);
+  bool has_const = false;
+  for (int j = 0; j < k; ++j) {
+    bool c = X[j] != 0.0;
+    for (int i = 1; i < n && c; ++i) c = X[i * k + j] == X[j];
+    if (c) {
+      constval[j] = X[j];
+      has_const = true;
+    }
+  }
+  if (data->center && !has_const)
+    return set_error(TFCE_EINVAL, "data was centred but the design has no intercept column");
+  tfce_ols_model* md = new tfce_ols_model();
+  md->data = data;
+  md->k = k;
+  md->f = f;
+  md->m = m;
+  md->n_con = n_contrasts;
+  for (int c = 0; c < n_contrasts; ++c) {
+    if (contrasts[c] < 0 || contrasts[c] >= k) {
+      free_model(md);
+      return set_error(TFCE_EINVAL, "contrast index out of range");
+    }
+    md->con[c] = internal[contrasts[c]];
+    md->con_is_intercept[c] = (data->center && constval[contrasts[c]] != 0.0) ? 1 : 0;
+    if (md->con_is_intercept[c] && constval[contrasts[c]] != 1.0) {
+      free_model(md);
+      return set_error(TFCE_ENOTSUP, "a contrast on a constant column must use the value 1");
+    }
+  }
+  std::vector<double> F(n * std::max(f, 1)), Z(n * m), FF(std::max(f * f, 1)), ZZ(m * m), colsum(m, 0.0);
+  std::vector<float> Z32(n * m);
+  for (int i = 0; i < n; ++i) {
+    for (int q = 0; q < f; ++q) F[i * f + q] = X[i * k + order[q]];
+    for (int q = 0; q < m; ++q) {
+      Z[i * m + q] = X[i * k + order[f + q]];
+      Z32[i * m + q] = (float)Z[i * m + q];
+      colsum[q] += (double)Z32[i * m + q];
+    }
+  }
+  for (int a = 0; a < f; ++a)
+    for (int b = 0; b < f; ++b) {
+      double s = 0.0;
+      for (int i = 0; i < n; ++i) s += F[i * f + a] * F[i * f + b];
+      FF[a * f + b] = s;
+    }
+  for (int a = 0; a < m; ++a)
+    for (int b = 0; b < m; ++b) {
+      double s = 0.0;
+      for (int i = 0; i < n; ++i) s += Z[i * m + a] * Z[i * m + b];
+      ZZ[a * m + b] = s;
+    }
+  std::vector<double> LFF;
+  if (f > 0 && !chol_inv(FF, f, LFF)) {
+    free_model(md);
+    return set_error(TFCE_EINVAL, "fixed design columns are rank deficient");
+  }
+  // the Gram matrix of X[perm] is the same for every permutation when every
+  // fixed column is constant (F'PZ = F'Z)
+  bool shared = true;
+  for (int q = 0; q < f; ++q) shared = shared && constval[order[q]] != 0.0;
+  md->shared = shared;
+  int rc = 0;
+  const int64_t V = data->V;
+  if ((rc = dalloc(&md->F, F.size(), "F")) || (rc = dalloc(&md->Z, Z.size(), "Z")) ||
+      (rc = dalloc(&md->FF, FF.size(), "FF")) || (rc = dalloc(&md->ZZ, ZZ.size(), "ZZ")) ||
+      (rc = dalloc(&md->Z32, Z32.size(), "Z32")) || (rc = dalloc(&md->colsumZ, m, "colsum")) ||
+      (rc = dalloc(&md->gF, (size_t)std::max(f, 1) * V, "gF")) ||
+      (rc = dalloc(&md->hF, (size_t)std::max(f, 1) * V, "hF")) || (rc = dalloc(&md->r0, V, "r0")) ||
+      (rc = upload(md->F, F.data(), F.size(), "F")) || (rc = upload(md->Z, Z.data(), Z.size(), "Z")) ||
+      (rc = upload(md->FF, FF.data(), FF.size(), "FF")) || (rc = upload(md->ZZ, ZZ.data(), ZZ.size(), "ZZ")) ||
+      (rc = upload(md->Z32, Z32.data(), Z32.size(), "Z32")) ||
+      (rc = upload(md->colsumZ, colsum.data(), (size_t)m, "colsum"))) {
+    free_model(md);
+    return rc;
+  }
+  double* dLFF = nullptr;
+  if ((rc = dalloc(&dLFF, std::max<size_t>(LFF.size(), 1), "LFF")) ||
+      (rc = upload(dLFF, LFF.data(), LFF.size(), "LFF"))) {
+    if (dLFF) (void)hipFree(dLFF);
+    free_model(md);
+    return rc;
+  }
+  OlsFixedArgs fa{};
+  fa.plane32 = data->plane32;
+  fa.Kp = data->Kp;
+  fa.n = n;
+  fa.V = V;
+  fa.Npad = data->Npad;
+  fa.yy = data->yy;
+  fa.F = md->F;
+  fa.f = f;
+  fa.LFF = dLFF;
+  fa.gF = md->gF;
+  fa.hF = md->hF;
+  fa.r0 = md->r0;
+  hipError_t e = launch_ols_fixed(fa, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  (void)hipFree(dLFF);
+  if (e != hipSuccess) {
+    free_model(md);
+    return set_error(-(int)e, "ols fixed-column kernel");
+  }
+  if (shared) {
+    if ((rc = dalloc(&md->Linv1, 64, "Linv")) || (rc = dalloc(&md->dc1, kOlsMaxContrasts, "dc"))) {
+      free_model(md);
+      return rc;
+    }
+    OlsFactorArgs ka{};
+    ka.F = md->F;
+    ka.Z = md->Z;
+    ka.FF = md->FF;
+    ka.ZZ = md->ZZ;
+    ka.perm = nullptr;
+    ka.n = n;
+    ka.f = f;
+    ka.m = m;
+    ka.P = 1;
+    ka.n_con = md->n_con;
+    for (int c = 0; c < kOlsMaxContrasts; ++c) ka.con[c] = md->con[c];
+    ka.Linv = md->Linv1;
+    ka.dc = md->dc1;
+    e = launch_ols_factors(ka, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) {
+      free_model(md);
+      return set_error(-(int)e, "ols factor kernel");
+    }
+  }
+  *out = md;
+  return 0;
+}
+
+int tfce_ols_model_destroy(tfce_ols_model_t* m) {
+  free_model(m);
+  return 0;
+}
+
+int tfce_ols_perm_t(const tfce_ols_model_t* md, const int32_t* d_perm, int32_t P, float* d_t, int64_t ldt,
+                    hipStream_t stream) {
+  if (!md || !d_t) return set_error(TFCE_EINVAL, "NULL argument");
+  const tfce_ols_data* d = md->data;
+  if (!d_perm) P = 1;
+  if (P < 1) return set_error(TFCE_EINVAL, "P must be >= 1");
+  if (ldt < d->V) return set_error(TFCE_EINVAL, "ldt must be >= V");
+  const int rpp = md->m <= 4 ? 4 : 8;
+  const int per_tile = kGemmRows / rpp;
+  const int64_t Ppad = ((int64_t)P + per_tile - 1) / per_tile * per_tile;
+  const int64_t Mpad = Ppad * rpp;
+  const int pieces = d->pieces;
+  const size_t abytes = (size_t)d->Kp * Mpad * (pieces ? 2 * pieces : 4);
+  const size_t fbytes = md->shared ? 0 : (size_t)P * (64 * sizeof(float) + kOlsMaxContrasts * sizeof(double));
+  void* ws = nullptr;
+  int rc = ols_ws(stream, abytes + fbytes + 512, &ws);
+  if (rc) return rc;
+  char* base = reinterpret_cast<char*>(ws);
+  OlsBuildAArgs ba{};
+  ba.Z32 = md->Z32;
+  ba.perm = d_perm;
+  ba.n = d->n;
+  ba.m = md->m;
+  ba.P = P;
+  ba.Kp = d->Kp;
+  ba.rpp = rpp;
+  ba.Mpad = Mpad;
+  ba.pieces = pieces;
+  const size_t plane_a = (size_t)d->Kp * Mpad;
+  if (pieces == 0) {
+    ba.a32 = reinterpret_cast<float*>(base);
+  } else {
+    for (int p = 0; p < pieces; ++p) ba.abf[p] = reinterpret_cast<uint16_t*>(base) + p * plane_a;
+  }
+  hipError_t e = launch_ols_build_a(ba, stream);
+  if (e != hipSuccess) return set_error(-(int)e, "ols build-A launch");
+  const float* Linv = md->Linv1;
+  const double* dc = md->dc1;
+  if (!md->shared) {
+    char* fb = base + ((abytes + 255) / 256 * 256);
+    float* Lw = reinterpret_cast<float*>(fb);
+    double* dw = reinterpret_cast<double*>(fb + (size_t)P * 64 * sizeof(float));
+    OlsFactorArgs ka{};
+    ka.F = md->F;
+    ka.Z = md->Z;
+    ka.FF = md->FF;
+    ka.ZZ = md->ZZ;
+    ka.perm = d_perm;
+    ka.n = d->n;
+    ka.f = md->f;
+    ka.m = md->m;
+    ka.P = P;
+    ka.n_con = md->n_con;
+    for (int c = 0; c < kOlsMaxContrasts; ++c) ka.con[c] = md->con[c];
+    ka.Linv = Lw;
+    ka.dc = dw;
+    e = launch_ols_factors(ka, stream);
+    if (e != hipSuccess) return set_error(-(int)e, "ols factor launch");
+    Linv = Lw;
+    dc = dw;
+  }
+  OlsGemmArgs ga{};
+  ga.a32 = ba.a32;
+  for (int p = 0; p < 3; ++p) {
+    ga.abf[p] = ba.abf[p];
+    ga.bbf[p] = d->bf[p];
+  }
+  ga.b32 = d->plane32;
+  ga.Kp = d->Kp;
+  ga.Mpad = Mpad;
+  ga.Npad = d->Npad;
+  ga.V = d->V;
+  ga.rpp = rpp;
+  ga.P = P;
+  ga.n = d->n;
+  ga.k = md->k;
+  ga.f = md->f;
+  ga.m = md->m;
+  ga.n_con = md->n_con;
+  for (int c = 0; c < kOlsMaxContrasts; ++c) {
+    ga.con[c] = md->con[c];
+    ga.con_is_intercept[c] = md->con_is_intercept[c];
+  }
+  ga.shared_factor = md->shared ? 1 : 0;
+  ga.Linv = Linv;
+  ga.dc = dc;
+  ga.gF = md->gF;
+  ga.hF = md->hF;
+  ga.r0 = md->r0;
+  ga.mu = d->mu;
+  ga.centered = d->center;
+  ga.colsumZ = md->colsumZ;
+  ga.t = d_t;
+  ga.ldt = ldt;
+  ga.n_mtiles = (int32_t)(Mpad / kGemmRows);
+  ga.n_ntiles = (int32_t)(d->Npad / kGemmCols);
+  e = launch_ols_gemm(ga, d->precision, stream);
+  if (e != hipSuccess) return set_error(-(int)e, "ols gemm launch");
+  return 0;
+}
+
+int tfce_ols_info(const tfce_ols_model_t* md, int32_t* k, int32_t* n_fixed, int32_t* shared_gram) {
+  if (!md) return set_error(TFCE_EINVAL, "NULL model");
+  if (k) *k = md->k;
+  if (n_fixed) *n_fixed = md->f;
+  if (shared_gram) *shared_gram = md->shared ? 1 : 0;
+  return 0;
+}
+
+}  // extern "C"

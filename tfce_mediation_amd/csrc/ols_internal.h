@@ -1,0 +1,103 @@
+// Internal declarations of the permuted-OLS kernels (not part of the ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tfcehip {
+
+// Contraction precision of the permuted design with Y.
+constexpr int kPrecFP32 = 0;    // FP32 MFMA (v_mfma_f32_32x32x2_f32)
+constexpr int kPrecBF16x3 = 1;  // hi/lo bf16 split, hi*hi + hi*lo + lo*hi, FP32 accumulate
+constexpr int kPrecBF16x6 = 2;  // three-piece bf16 split, every product of order <= 2
+
+constexpr int kOlsMaxK = 8;         // design columns
+constexpr int kOlsMaxContrasts = 8;
+constexpr int kGemmRows = 256;      // rows (permutation x coefficient) per workgroup
+constexpr int kGemmCols = 128;      // elements per workgroup
+constexpr int kColAlign = 1024;     // element padding: 8 XCDs x 128 columns
+
+inline int ols_pieces(int precision) { return precision == kPrecBF16x6 ? 3 : (precision == kPrecBF16x3 ? 2 : 1); }
+
+struct OlsPrepArgs {
+  const float* Y;   // [n][ldy]
+  int32_t n;
+  int32_t Kp;       // n rounded up to 16
+  int64_t V, ldy, Npad;
+  int32_t center;
+  int32_t pieces;   // bf16 pieces to write (0 = none)
+  double* mu;       // [V]
+  double* yy;       // [V]
+  float* plane32;   // [Kp/8][Npad][8] centred float32 (always written)
+  uint16_t* bf[3];  // [Kp/8][Npad][8] bf16 pieces
+};
+
+struct OlsFixedArgs {
+  const float* plane32;  // data
+  int32_t Kp, n;
+  int64_t V, Npad;
+  const double* yy;
+  const double* F;       // [n][f] fixed columns (internal order)
+  int32_t f;
+  const double* LFF;     // [f][f] inverse Cholesky factor of F'F (lower)
+  float* gF;             // [f][V]  F'Y
+  float* hF;             // [f][V]  LFF * gF
+  double* r0;            // [V]     yy - |hF|^2
+};
+
+struct OlsFactorArgs {
+  const double* F;       // [n][f]
+  const double* Z;       // [n][m]
+  const double* FF;      // [f][f]
+  const double* ZZ;      // [m][m]
+  const int32_t* perm;   // [P][n] or null (identity)
+  int32_t n, f, m, P;
+  int32_t n_con;
+  int32_t con[kOlsMaxContrasts];  // internal column index of each contrast
+  float* Linv;           // [P][8][8]
+  double* dc;            // [P][kOlsMaxContrasts]
+};
+
+struct OlsBuildAArgs {
+  const float* Z32;      // [n][m] permuted columns, float
+  const int32_t* perm;   // [P][n] or null
+  int32_t n, m, P, Kp, rpp;
+  int64_t Mpad;
+  int32_t pieces;
+  float* a32;            // [Kp/8][Mpad][8] (FP32 mode)
+  uint16_t* abf[3];      // [Kp/8][Mpad][8] bf16 pieces
+};
+
+struct OlsGemmArgs {
+  const float* a32;
+  const uint16_t* abf[3];
+  const float* b32;
+  const uint16_t* bbf[3];
+  int32_t Kp;
+  int64_t Mpad, Npad, V;
+  int32_t rpp;           // rows per permutation (4 or 8)
+  int32_t P;             // valid permutations
+  int32_t n, k, f, m;
+  int32_t n_con;
+  int32_t con[kOlsMaxContrasts];
+  int32_t con_is_intercept[kOlsMaxContrasts];
+  int32_t shared_factor; // 1: factors at index 0 serve every permutation
+  const float* Linv;     // [P or 1][8][8]
+  const double* dc;      // [P or 1][kOlsMaxContrasts]
+  const float* gF;       // [f][V]
+  const float* hF;       // [f][V]
+  const double* r0;      // [V]
+  const double* mu;      // [V]
+  int32_t centered;
+  const double* colsumZ; // [m]  (uncentred mode correction)
+  float* t;              // [n_con][P][ldt]
+  int64_t ldt;
+  int32_t n_mtiles, n_ntiles;
+};
+
+hipError_t launch_ols_prep(const OlsPrepArgs& a, hipStream_t s);
+hipError_t launch_ols_fixed(const OlsFixedArgs& a, hipStream_t s);
+hipError_t launch_ols_factors(const OlsFactorArgs& a, hipStream_t s);
+hipError_t launch_ols_build_a(const OlsBuildAArgs& a, hipStream_t s);
+hipError_t launch_ols_gemm(const OlsGemmArgs& a, int precision, hipStream_t s);
+
+}  // namespace tfcehip

@@ -1,0 +1,85 @@
+"""Batched permuted OLS t-values on the GPU (tfce_ols_* of libtfcehip).
+
+``PermutedOLS(Y, X, ...)`` prepares Y once (centring, FP64 y'y, split planes)
+and binds the design; ``t(perms)`` returns the t-values of the requested
+contrasts for every permutation of a batch, ``[n_contrasts, P, V]`` float32 on
+the device.  Row i of the permuted design is ``X[perm[i]]`` as in
+``vertex_tfce_multiple_regression_randomise.py:105`` (all columns) or, with
+``perm_columns``, only the listed columns move (``-v``, :93-97).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+
+class PermutedOLS:
+    def __init__(self, Y, X, contrasts=None, perm_columns=None, precision="bf16x3", center=None,
+                 stream=None):
+        import torch
+        lib = _lib.lib()
+        X = np.ascontiguousarray(np.asarray(X, dtype=np.float64))
+        if X.ndim == 1:
+            X = X[:, None]
+        n, k = X.shape
+        if not (isinstance(Y, torch.Tensor) and Y.is_cuda):
+            Y = torch.as_tensor(np.ascontiguousarray(Y, dtype=np.float32)).cuda()
+        Y = Y.to(torch.float32)
+        if Y.dim() == 1:
+            Y = Y[:, None]
+        if Y.stride(1) != 1:
+            Y = Y.contiguous()
+        assert Y.shape[0] == n, "Y must be [n_subjects, n_elements]"
+        self.n, self.k, self.V = n, k, int(Y.shape[1])
+        const = [bool(np.all(X[:, j] == X[0, j]) and X[0, j] != 0) for j in range(k)]
+        if center is None:
+            center = any(const)
+        if contrasts is None:
+            contrasts = [j for j in range(k) if not const[j]]
+        self.contrasts = [int(c) for c in contrasts]
+        if perm_columns is None:
+            perm_columns = [j for j in range(k) if not const[j]]
+        mask = 0
+        for j in perm_columns:
+            mask |= 1 << int(j)
+        self.precision = precision
+        self._Y = Y  # kept alive until the prep kernel has run
+        data = ctypes.c_void_p()
+        _lib.check(lib.tfce_ols_data_create(_lib.ptr(Y), n, self.V, Y.stride(0), _lib.PREC[precision],
+                                            1 if center else 0, ctypes.byref(data),
+                                            _lib.stream_ptr(stream)))
+        self._data = data
+        con = np.ascontiguousarray(self.contrasts, dtype=np.int32)
+        model = ctypes.c_void_p()
+        _lib.check(lib.tfce_ols_model_create(self._data, X.ctypes.data, k, mask, con.ctypes.data,
+                                             len(self.contrasts), ctypes.byref(model),
+                                             _lib.stream_ptr(stream)))
+        self._model = model
+        self._Y = None
+
+    def __del__(self):
+        if _lib._lib is None:
+            return
+        m = getattr(self, "_model", None)
+        if m is not None and m.value:
+            _lib._lib.tfce_ols_model_destroy(m)
+        d = getattr(self, "_data", None)
+        if d is not None and d.value:
+            _lib._lib.tfce_ols_data_destroy(d)
+
+    def t(self, perms=None, out=None, stream=None):
+        """t-values [n_contrasts, P, V] for permutation indices perms [P, n] (int32, cuda)."""
+        import torch
+        if perms is None:
+            P = 1
+        else:
+            assert perms.is_cuda and perms.dtype == torch.int32 and perms.is_contiguous()
+            assert perms.dim() == 2 and perms.shape[1] == self.n
+            P = int(perms.shape[0])
+        if out is None:
+            out = torch.empty((len(self.contrasts), P, self.V), dtype=torch.float32, device="cuda")
+        assert out.is_contiguous() and out.shape == (len(self.contrasts), P, self.V)
+        _lib.check(_lib.lib().tfce_ols_perm_t(self._model, _lib.ptr(perms), P, _lib.ptr(out), self.V,
+                                             _lib.stream_ptr(stream)))
+        return out

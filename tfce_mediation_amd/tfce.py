@@ -108,3 +108,65 @@ class CreateAdjSet:
         d_enh = torch.from_numpy(enhn).to(dev)
         self._adj.run_device(d_img, d_enh, accumulate=True)
         enhn[:] = d_enh.cpu().numpy()
+
+
+class SurfaceSet:
+    """Several adjacencies laid out along one map row (tfce_surfset_t).
+
+    ``batch(maps)`` runs every (surface, map, sign) item in one persistent
+    launch and returns the rescaled maxima as ``[n_surf, P, n_signs]`` in the
+    caller's surface order (optionally also the raw TFCE maxima, the maxima of
+    the maps themselves and the per-item status).
+    """
+
+    def __init__(self, adjacencies, offsets):
+        lib = _lib.lib()
+        self.adjs = list(adjacencies)
+        self.offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        n = len(self.adjs)
+        arr = (ctypes.c_void_p * n)(*[a.handle.value for a in self.adjs])
+        h = ctypes.c_void_p()
+        _lib.check(lib.tfce_surfset_create(arr, n, self.offsets.ctypes.data, ctypes.byref(h)))
+        self.handle = h
+        order = np.zeros(n, dtype=np.int32)
+        _lib.check(lib.tfce_surfset_order(self.handle, order.ctypes.data))
+        self.launch_order = order  # launch position -> caller's index
+        self._inv = np.argsort(order)
+        self.n = n
+        self.width = int(max(o + a.V for o, a in zip(self.offsets, self.adjs)))
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value and _lib._lib is not None:
+            _lib._lib.tfce_surfset_destroy(h)
+            self.handle = None
+
+    def batch(self, maps, sign_mask=3, T0=None, vdensity=None, want_raw=False, want_own=False,
+              want_status=False, stream=None):
+        import torch
+        assert maps.is_cuda and maps.dtype == torch.float32 and maps.stride(-1) == 1
+        P = maps.shape[0]
+        stride = maps.stride(0)
+        assert stride >= self.width
+        ns = 2 if sign_mask == 3 else 1
+        n_items = self.n * P * ns
+        dev = maps.device
+        scaled = torch.empty(n_items, dtype=torch.float32, device=dev)
+        raw = torch.empty(n_items, dtype=torch.float32, device=dev) if want_raw else None
+        own = torch.empty(n_items, dtype=torch.float32, device=dev) if want_own else None
+        st = torch.empty(n_items, dtype=torch.int32, device=dev) if want_status else None
+        if T0 is not None:
+            assert T0.is_cuda and T0.numel() == n_items
+        if vdensity is not None:
+            assert vdensity.is_cuda and vdensity.dtype == torch.float32 and vdensity.numel() >= self.width
+        _lib.check(_lib.lib().tfce_batch(self.handle, _lib.ptr(maps), stride, P, int(sign_mask),
+                                        _lib.ptr(T0), _lib.ptr(vdensity), _lib.ptr(scaled), _lib.ptr(raw),
+                                        _lib.ptr(own), _lib.ptr(st), _lib.stream_ptr(stream)))
+        inv = torch.as_tensor(self._inv, device=dev, dtype=torch.long)
+
+        def user(x):
+            return None if x is None else x.view(self.n, P, ns).index_select(0, inv)
+        out = user(scaled)
+        if not (want_raw or want_own or want_status):
+            return out
+        return out, user(raw), user(own), user(st)

@@ -138,6 +138,20 @@ int tfce_ols_info(const tfce_ols_model_t *model, int32_t *k, int32_t *n_fixed, i
 int tfce_ols_perm_t(const tfce_ols_model_t *model, const int32_t *d_perm, int32_t P, float *d_t,
                     int64_t ldt, hipStream_t stream);
 
+/* ------------------------------------------------------------------ FWER */
+/* 1 - p(FWE) of each element against the sorted (ascending) null, FP64.
+ * mode 0: calculate_fweP_vertex.py:37-69 / calculate_fweP_voxel.py:23-57
+ *   (elements with value > 0; idx = #(null < value); (idx-1)/N floored at 0;
+ *   other elements 0).
+ * mode 1: the nearest-neighbour find_nearest of tm_func.py:566-573 (used by
+ *   apply_mfwer, tm_func.py:470-487) for every element. */
+int tfce_fwe_p(const double *d_sorted_null, int32_t n_null, const float *d_values, int64_t V,
+               int32_t mode, double *d_out, hipStream_t stream);
+/* The same lookups for float64 element values (apply_mfwer's standardised
+ * log-TFCE, tm_func.py:431-446). */
+int tfce_fwe_p64(const double *d_sorted_null, int32_t n_null, const double *d_values, int64_t V,
+                 int32_t mode, double *d_out, hipStream_t stream);
+
 /* Number of resident workgroups per CU used by the TFCE sweep (autotuner knob). */
 int tfce_set_occupancy(int32_t wgs_per_cu);
 

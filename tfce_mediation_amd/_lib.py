@@ -40,6 +40,8 @@ SIGNATURES = {
     "tfce_ols_model_destroy": (c_int, [c_p]),
     "tfce_ols_info": (c_int, [c_p, c_p, c_p, c_p]),
     "tfce_ols_perm_t": (c_int, [c_p, c_p, c_i32, c_p, c_i64, c_p]),
+    "tfce_fwe_p": (c_int, [c_p, c_i32, c_p, c_i64, c_i32, c_p, c_p]),
+    "tfce_fwe_p64": (c_int, [c_p, c_i32, c_p, c_i64, c_i32, c_p, c_p]),
 }
 
 PREC = {"fp32": 0, "bf16x3": 1, "bf16x6": 2}

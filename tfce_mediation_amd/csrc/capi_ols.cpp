@@ -122,7 +122,34 @@ void free_model(tfce_ols_model* m) {
 
 }  // namespace
 
+namespace tfcehip {
+hipError_t launch_fwe_lookup(const double* sorted, int n, const float* vals, int64_t V, int mode, double* out,
+                             hipStream_t s);
+hipError_t launch_fwe_lookup64(const double* sorted, int n, const double* vals, int64_t V, int mode, double* out,
+                               hipStream_t s);
+}
+
 extern "C" {
+
+int tfce_fwe_p(const double* d_sorted_null, int32_t n_null, const float* d_values, int64_t V, int32_t mode,
+               double* d_out, hipStream_t stream) {
+  if (!d_sorted_null || !d_values || !d_out) return set_error(TFCE_EINVAL, "NULL argument");
+  if (n_null < 1) return set_error(TFCE_EINVAL, "the null needs at least one value");
+  if (mode != 0 && mode != 1) return set_error(TFCE_EINVAL, "mode must be 0 (left) or 1 (nearest)");
+  hipError_t e = launch_fwe_lookup(d_sorted_null, n_null, d_values, V, mode, d_out, stream);
+  if (e != hipSuccess) return set_error(-(int)e, "fwe lookup launch");
+  return 0;
+}
+
+int tfce_fwe_p64(const double* d_sorted_null, int32_t n_null, const double* d_values, int64_t V, int32_t mode,
+                 double* d_out, hipStream_t stream) {
+  if (!d_sorted_null || !d_values || !d_out) return set_error(TFCE_EINVAL, "NULL argument");
+  if (n_null < 1) return set_error(TFCE_EINVAL, "the null needs at least one value");
+  if (mode != 0 && mode != 1) return set_error(TFCE_EINVAL, "mode must be 0 (left) or 1 (nearest)");
+  hipError_t e = launch_fwe_lookup64(d_sorted_null, n_null, d_values, V, mode, d_out, stream);
+  if (e != hipSuccess) return set_error(-(int)e, "fwe lookup launch");
+  return 0;
+}
 
 int tfce_ols_data_create(const float* d_Y, int32_t n, int64_t V, int64_t ldy, int32_t precision,
                          int32_t center, tfce_ols_data_t** out, hipStream_t stream) {

@@ -62,6 +62,7 @@ struct TfceBatchArgs {
 hipError_t launch_tfce_batch(const TfceBatchArgs& a, bool full, int grid, hipStream_t stream);
 hipError_t launch_fill_f32(float* p, int64_t n, float v, hipStream_t stream);
 
+// Slot layout: 8 arrays of the largest surface's element count.
 inline int64_t tfce_slot_words(int32_t v_max) { return (int64_t)8 * ((v_max + 63) & ~63); }
 
 }  // namespace tfcehip

@@ -95,7 +95,8 @@ __device__ float block_max(float v, float* s_red) {
   return r;
 }
 
-// Append `c` to list when `take` (wave-aggregated LDS counter).
+// Append `c` to list when `take` (wave-aggregated LDS counter).  Every lane
+// of the wave must call it (it ballots).
 __device__ __forceinline__ void wave_append(bool take, int c, int32_t* list, int* s_count) {
   unsigned long long m = __ballot(take);
   if (m == 0ull) return;
@@ -107,6 +108,92 @@ __device__ __forceinline__ void wave_append(bool take, int c, int32_t* list, int
   if (take) list[base + __popcll(m & ((1ull << lane) - 1ull))] = c;
 }
 
+constexpr int kIlp = 4;  // independent union-find chains per thread
+
+// Roots of up to kIlp elements at once; the loads of the chains interleave.
+// Path splitting: a visited node is re-pointed to its grandparent.
+__device__ __forceinline__ void find_batch(int32_t* parent, int (&x)[kIlp], const bool (&live)[kIlp]) {
+  bool go[kIlp];
+  int prev[kIlp];
+#pragma unroll
+  for (int j = 0; j < kIlp; ++j) {
+    go[j] = live[j];
+    prev[j] = -1;
+  }
+  while (go[0] || go[1] || go[2] || go[3]) {
+    int p[kIlp];
+#pragma unroll
+    for (int j = 0; j < kIlp; ++j) p[j] = go[j] ? ld(parent + x[j]) : 0;
+#pragma unroll
+    for (int j = 0; j < kIlp; ++j) {
+      if (!go[j]) continue;
+      if (p[j] == x[j]) {
+        go[j] = false;
+      } else {
+        if (prev[j] >= 0) st(parent + prev[j], p[j]);
+        prev[j] = x[j];
+        x[j] = p[j];
+      }
+    }
+  }
+}
+
+// Join the pairs (ua[j], ub[j]) for live[j]: kIlp unions in flight, CAS
+// hooking the larger root under the smaller.
+__device__ __forceinline__ void union_pairs(int32_t* parent, int (&ua)[kIlp], int (&ub)[kIlp], bool (&live)[kIlp]) {
+  while (live[0] || live[1] || live[2] || live[3]) {
+    int pa[kIlp], pb[kIlp];
+#pragma unroll
+    for (int j = 0; j < kIlp; ++j) {
+      pa[j] = live[j] ? ld(parent + ua[j]) : 0;
+      pb[j] = live[j] ? ld(parent + ub[j]) : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < kIlp; ++j) {
+      if (!live[j]) continue;
+      const bool ra = pa[j] == ua[j], rb = pb[j] == ub[j];
+      if (!ra) ua[j] = pa[j];
+      if (!rb) ub[j] = pb[j];
+      if (ra && rb) {
+        if (ua[j] == ub[j]) {
+          live[j] = false;
+        } else {
+          const int hi = max(ua[j], ub[j]), lo = min(ua[j], ub[j]);
+          int expected = hi;
+          // on failure hi gained a parent; the next round climbs from it
+          if (__hip_atomic_compare_exchange_strong(parent + hi, &expected, lo, __ATOMIC_RELAXED,
+                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
+            live[j] = false;
+        }
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ int wave_incl_scan(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(v, o, 64);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+
+// First level l with x > T[l] (T strictly decreasing, L levels), kNever if none.
+__device__ __forceinline__ int level_of(float xv, const float* s_T, int L, float T0, float dT) {
+  if (!(xv > s_T[L - 1])) return kNever;  // also NaN
+  int l = 0;
+  if (xv < T0) {
+    const float g = (T0 - xv) / dT;  // estimate; fixed up against the table below
+    l = g < (float)(L - 1) ? (int)g : L - 1;
+    if (l < 0) l = 0;
+  }
+  while (l > 0 && xv > s_T[l - 1]) --l;
+  while (!(xv > s_T[l])) ++l;
+  return l;
+}
+
 template <bool FULL>
 __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs a) {
   __shared__ float s_T[kMaxLevels];
@@ -115,7 +202,7 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
   __shared__ int s_cur[kMaxLevels];
   __shared__ float s_red[kTfceThreads / 64];
   __shared__ int s_item_lo, s_item_hi, s_L, s_status, s_nroots_new;
-  __shared__ float s_own;
+  __shared__ float s_own, s_T0, s_dT;
 
   const int tid = threadIdx.x;
   const int64_t vstride = (a.v_max + 63) & ~63;
@@ -151,9 +238,18 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
     const bool symmetric = (S.flags & kSurfSymmetric) != 0;
 
     // ---- phase 0: maximum and level table -----------------------------------
-    float m = -INFINITY;
-    for (int v = tid; v < V; v += kTfceThreads) m = fmaxf(m, sign * x[v]);
-    m = block_max(m, s_red);
+    float m0 = -INFINITY, m1 = -INFINITY, m2 = -INFINITY, m3 = -INFINITY;
+    {
+      int v = tid;
+      for (; v + 3 * kTfceThreads < V; v += 4 * kTfceThreads) {
+        m0 = fmaxf(m0, sign * x[v]);
+        m1 = fmaxf(m1, sign * x[v + kTfceThreads]);
+        m2 = fmaxf(m2, sign * x[v + 2 * kTfceThreads]);
+        m3 = fmaxf(m3, sign * x[v + 3 * kTfceThreads]);
+      }
+      for (; v < V; v += kTfceThreads) m0 = fmaxf(m0, sign * x[v]);
+    }
+    const float m = block_max(fmaxf(fmaxf(m0, m1), fmaxf(m2, m3)), s_red);
     if (tid == 0) {
       s_own = m;
       float T0 = a.ext_T0 ? a.ext_T0[item] : ((S.flags & kSurfZeroInLevels) ? fmaxf(m, 0.0f) : m);
@@ -176,6 +272,8 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
       if (status < 0) L = 0; else status = L;
       s_L = L;
       s_status = status;
+      s_T0 = T0;
+      s_dT = dT;
     }
     __syncthreads();
     const int L = s_L;
@@ -184,19 +282,10 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
     for (int l = tid; l < L; l += kTfceThreads) s_cur[l] = 0;
     __syncthreads();
     if (L > 0) {
-      const float Tlast = s_T[L - 1];
+      const float T0 = s_T0, dT = s_dT;
       for (int v = tid; v < V; v += kTfceThreads) {
-        const float xv = sign * x[v];
-        int lev = kNever;
-        if (xv > Tlast) {  // false for NaN
-          int lo = 0, hi = L - 1;
-          while (lo < hi) {
-            int mid = (lo + hi) >> 1;
-            if (xv > s_T[mid]) hi = mid; else lo = mid + 1;
-          }
-          lev = lo;
-          atomicAdd(&s_cur[lev], 1);
-        }
+        const int lev = level_of(sign * x[v], s_T, L, T0, dT);
+        if (lev != kNever) atomicAdd(&s_cur[lev], 1);
         pos[v] = lev;
       }
     }
@@ -207,12 +296,7 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
       const int b = tid * per, e = min(L, b + per);
       int sum = 0;
       for (int l = b; l < e; ++l) sum += s_cur[l];
-      int incl = sum;
-#pragma unroll
-      for (int o = 1; o < 64; o <<= 1) {
-        int t = __shfl_up(incl, o, 64);
-        if (tid >= o) incl += t;
-      }
+      const int incl = wave_incl_scan(sum);
       int run = incl - sum;
       for (int l = b; l < e; ++l) {
         const int c = s_cur[l];
@@ -223,7 +307,7 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
       if (tid == 63) s_start[L] = incl;
     }
     __syncthreads();
-    // ---- phase 2: scatter into level order ----------------------------------
+    // ---- phase 2: level order ----------------------------------------------
     if (L > 0) {
       for (int v = tid; v < V; v += kTfceThreads) {
         const int lev = pos[v];
@@ -236,8 +320,9 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
     }
     if (tid == 0) s_nroots_new = 0;
     __syncthreads();
+    const int A = (L > 0) ? s_start[L] : 0;
 
-    // ---- phase 3: threshold sweep -------------------------------------------
+    // ---- phase 4: threshold sweep -------------------------------------------
     int n_roots = 0;
     int cur = 0;
     for (int l = 0; l < L; ++l) {
@@ -246,53 +331,94 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
       int32_t* rold = roots[cur];
       int32_t* rnew = roots[cur ^ 1];
       if (b1 > b0) {
-        // 3a: make sets for the new vertices
+        // 4a: one set per new vertex
         for (int i = b0 + tid; i < b1; i += kTfceThreads) {
           parent[i] = i;
           size[i] = 1;
-          if (FULL) val[i] = enhn ? enhn[order[i]] : 0.0f;
-          else val[i] = 0.0f;
+          val[i] = (FULL && enhn) ? enhn[order[i]] : 0.0f;
         }
         __syncthreads();
-        // 3b: join with already active neighbours
-        for (int i = b0 + tid; i < b1; i += kTfceThreads) {
-          const int v = order[i];
-          const int e0 = S.row_ptr[v], e1 = S.row_ptr[v + 1];
-          const float xv = symmetric ? 0.0f : sign * x[v];
-          for (int e = e0; e < e1; ++e) {
-            const int u = S.col[e];
-            const int q = pos[u];
-            if (q >= b1) continue;  // not active (kNever is never < b1)
-            if (!symmetric && q >= b0) {
-              // same level, edge only listed from v: the reference uses it iff u
-              // was activated first (descending value, ties by ascending index)
-              const float xu = sign * x[u];
-              if (!(xu > xv || (xu == xv && u < v))) continue;
-            }
-            uf_union(parent, i, q);
+        // 4b: join new vertices with their active neighbours.  Neighbour
+        // positions of two vertices are gathered together, then the pairs are
+        // joined kIlp at a time.
+        for (int i0 = b0; i0 < b1; i0 += 2 * kTfceThreads) {
+          int iv[2], vv[2], e0[2], e1[2];
+          float xv[2];
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            iv[h] = i0 + h * kTfceThreads + tid;
+            vv[h] = iv[h] < b1 ? order[iv[h]] : -1;
           }
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            e0[h] = vv[h] >= 0 ? S.row_ptr[vv[h]] : 0;
+            e1[h] = vv[h] >= 0 ? S.row_ptr[vv[h] + 1] : 0;
+            xv[h] = (!symmetric && vv[h] >= 0) ? sign * x[vv[h]] : 0.0f;
+          }
+          int ua[kIlp], ub[kIlp];
+          bool live[kIlp];
+          int nq = 0;
+#pragma unroll
+          for (int j = 0; j < kIlp; ++j) live[j] = false;
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            for (int e = e0[h]; e < e1[h]; e += 8) {
+              int u[8], q[8];
+#pragma unroll
+              for (int j = 0; j < 8; ++j) u[j] = (e + j < e1[h]) ? S.col[e + j] : -1;
+#pragma unroll
+              for (int j = 0; j < 8; ++j) q[j] = u[j] >= 0 ? pos[u[j]] : kNever;
+#pragma unroll
+              for (int j = 0; j < 8; ++j) {
+                bool take = q[j] < b1 && q[j] != iv[h];
+                if (take && !symmetric && q[j] >= b0) {
+                  // same level, entry listed from v only: the reference uses it
+                  // iff u was activated first (descending value, ties by index)
+                  const float xu = sign * x[u[j]];
+                  take = xu > xv[h] || (xu == xv[h] && u[j] < vv[h]);
+                }
+                if (take && symmetric && q[j] >= b0) take = q[j] < iv[h];  // same-level edge once
+                if (take) {
+                  // append to the pending pairs; flush when kIlp are queued
+#pragma unroll
+                  for (int k = 0; k < kIlp; ++k)
+                    if (k == nq) { ua[k] = iv[h]; ub[k] = q[j]; live[k] = true; }
+                  if (++nq == kIlp) {
+                    union_pairs(parent, ua, ub, live);
+                    nq = 0;
+                  }
+                }
+              }
+            }
+          }
+          if (nq > 0) union_pairs(parent, ua, ub, live);
         }
         __syncthreads();
-        // 3c: merge sizes (and MAX values) into surviving roots; collect roots
+        // 4b: merge sizes (and MAX values) into surviving roots; collect roots
         const int n_cand = n_roots + (b1 - b0);
-        for (int k = tid; k < n_cand + ((kTfceThreads - n_cand % kTfceThreads) % kTfceThreads);
-             k += kTfceThreads) {
-          bool take = false;
-          int c = 0;
-          if (k < n_cand) {
-            c = (k < n_roots) ? rold[k] : b0 + (k - n_roots);
-            const int r = uf_find(parent, c);
-            if (r != c) {
-              __hip_atomic_fetch_add(size + r, ld(size + c), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-              if (!FULL)
-                __hip_atomic_fetch_max(reinterpret_cast<int32_t*>(val) + r,
-                                       ld(reinterpret_cast<const int32_t*>(val) + c), __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-            } else {
-              take = true;
-            }
+        for (int k0 = 0; k0 < n_cand; k0 += kIlp * kTfceThreads) {
+          int c[kIlp], r[kIlp];
+          bool live[kIlp];
+#pragma unroll
+          for (int j = 0; j < kIlp; ++j) {
+            const int k = k0 + j * kTfceThreads + tid;
+            live[j] = k < n_cand;
+            c[j] = live[j] ? ((k < n_roots) ? rold[k] : b0 + (k - n_roots)) : 0;
+            r[j] = c[j];
           }
-          wave_append(take, c, rnew, &s_nroots_new);
+          find_batch(parent, r, live);
+#pragma unroll
+          for (int j = 0; j < kIlp; ++j) {
+            const bool hooked = live[j] && r[j] != c[j];
+            if (hooked) {
+              __hip_atomic_fetch_add(size + r[j], ld(size + c[j]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+              if (!FULL)
+                __hip_atomic_fetch_max(reinterpret_cast<int32_t*>(val) + r[j],
+                                       ld(reinterpret_cast<const int32_t*>(val) + c[j]), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+            wave_append(live[j] && !hooked, c[j], rnew, &s_nroots_new);
+          }
         }
         __syncthreads();
         n_roots = s_nroots_new;
@@ -301,12 +427,24 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
         cur ^= 1;
       }
       int32_t* rl = roots[cur];
-      // 3d: level increment
+      // 4c: level increment
       if (!FULL) {
-        for (int k = tid; k < n_roots; k += kTfceThreads) {
-          const int r = rl[k];
-          const float cur_v = __int_as_float(ld(reinterpret_cast<const int32_t*>(val) + r));
-          val[r] = cur_v + tfce_inc(ld(size + r), HH, S.E);
+        for (int k0 = 0; k0 < n_roots; k0 += kIlp * kTfceThreads) {
+          int r[kIlp], sz[kIlp];
+          float cv[kIlp];
+#pragma unroll
+          for (int j = 0; j < kIlp; ++j) {
+            const int k = k0 + j * kTfceThreads + tid;
+            r[j] = k < n_roots ? rl[k] : -1;
+          }
+#pragma unroll
+          for (int j = 0; j < kIlp; ++j) {
+            sz[j] = r[j] >= 0 ? ld(size + r[j]) : 0;
+            cv[j] = r[j] >= 0 ? __int_as_float(ld(reinterpret_cast<const int32_t*>(val) + r[j])) : 0.0f;
+          }
+#pragma unroll
+          for (int j = 0; j < kIlp; ++j)
+            if (r[j] >= 0) val[r[j]] = cv[j] + tfce_inc(sz[j], HH, S.E);
         }
         __syncthreads();
       } else {
@@ -315,28 +453,39 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
           incv[r] = tfce_inc(ld(size + r), HH, S.E);
         }
         __syncthreads();
-        for (int i = tid; i < b1; i += kTfceThreads) {
-          const int r = uf_find(parent, i);
-          val[i] = val[i] + incv[r];
-        }
-        if (a.labels && l == a.labels_level) {
-          for (int i = tid; i < b1; i += kTfceThreads) a.labels[order[i]] = order[uf_find(parent, i)];
+        for (int i0 = 0; i0 < b1; i0 += kIlp * kTfceThreads) {
+          int r[kIlp];
+          bool live[kIlp];
+#pragma unroll
+          for (int j = 0; j < kIlp; ++j) {
+            const int i = i0 + j * kTfceThreads + tid;
+            live[j] = i < b1;
+            r[j] = live[j] ? i : 0;
+          }
+          find_batch(parent, r, live);
+#pragma unroll
+          for (int j = 0; j < kIlp; ++j) {
+            const int i = i0 + j * kTfceThreads + tid;
+            if (live[j]) {
+              val[i] = val[i] + incv[r[j]];
+              if (a.labels && l == a.labels_level) a.labels[order[i]] = order[r[j]];
+            }
+          }
         }
         __syncthreads();
       }
     }
 
     // ---- finalize -----------------------------------------------------------
-    const int n_act = (L > 0) ? s_start[L] : 0;
     float best = 0.0f;
     if (!FULL) {
       int32_t* rl = roots[cur];
       for (int k = tid; k < n_roots; k += kTfceThreads)
         best = fmaxf(best, __int_as_float(ld(reinterpret_cast<const int32_t*>(val) + rl[k])));
     } else {
-      for (int i = tid; i < n_act; i += kTfceThreads) best = fmaxf(best, val[i]);
+      for (int i = tid; i < A; i += kTfceThreads) best = fmaxf(best, val[i]);
       if (enhn)
-        for (int i = tid; i < n_act; i += kTfceThreads) enhn[order[i]] = val[i];
+        for (int i = tid; i < A; i += kTfceThreads) enhn[order[i]] = val[i];
     }
     best = block_max(best, s_red);
     const float own = s_own;

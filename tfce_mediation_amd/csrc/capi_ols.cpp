@@ -9,7 +9,7 @@
 #include <utility>
 #include <vector>
 
-#include "../../include/tfcehip.h"
+#include "tfcehip.h"
 #include "capi_common.h"
 #include "ols_internal.h"
 

@@ -8,7 +8,7 @@
 #include <utility>
 #include <vector>
 
-#include "../../include/tfcehip.h"
+#include "tfcehip.h"
 #include "capi_common.h"
 #include "tfce_internal.h"
 
@@ -22,7 +22,8 @@ struct tfce_adj {
   int device = 0;
   int32_t* d_row_ptr = nullptr;
   int32_t* d_col = nullptr;
-  SurfDev* d_surf = nullptr;  // one descriptor, offset 0
+  double* d_sizepow = nullptr;  // (double)s^E for s in [0, V] when E != 1
+  SurfDev* d_surf = nullptr;    // one descriptor, offset 0
 };
 
 struct tfce_surfset {
@@ -167,6 +168,7 @@ int tfce_adj_create_ex(const int32_t* row_ptr, const int32_t* col, int32_t V, in
   auto cleanup = [&](int code, const std::string& m) {
     if (a->d_row_ptr) (void)hipFree(a->d_row_ptr);
     if (a->d_col) (void)hipFree(a->d_col);
+    if (a->d_sizepow) (void)hipFree(a->d_sizepow);
     if (a->d_surf) (void)hipFree(a->d_surf);
     delete a;
     return set_error(code, m);
@@ -181,9 +183,16 @@ int tfce_adj_create_ex(const int32_t* row_ptr, const int32_t* col, int32_t V, in
     return cleanup(-(int)e, "upload row_ptr");
   if (nnz > 0 && (e = hipMemcpy(a->d_col, col, sizeof(int32_t) * nnz, hipMemcpyHostToDevice)) != hipSuccess)
     return cleanup(-(int)e, "upload col");
+  if (E != 1.0f) {
+    if ((e = hipMalloc(&a->d_sizepow, sizeof(double) * ((size_t)V + 1))) != hipSuccess)
+      return cleanup(-(int)e, "hipMalloc(sizepow)");
+    if ((e = launch_sizepow(a->d_sizepow, V, E, nullptr)) != hipSuccess || (e = hipDeviceSynchronize()) != hipSuccess)
+      return cleanup(-(int)e, "sizepow table");
+  }
   SurfDev sd;
   sd.row_ptr = a->d_row_ptr;
   sd.col = a->d_col;
+  sd.sizepow = a->d_sizepow;
   sd.offset = 0;
   sd.V = V;
   sd.flags = a->flags;
@@ -204,6 +213,7 @@ int tfce_adj_destroy(tfce_adj_t* a) {
   if (!a) return 0;
   (void)hipFree(a->d_row_ptr);
   (void)hipFree(a->d_col);
+  if (a->d_sizepow) (void)hipFree(a->d_sizepow);
   (void)hipFree(a->d_surf);
   delete a;
   return 0;
@@ -230,10 +240,13 @@ static int run_batch(const SurfDev* d_surfs, int32_t n_surf, int32_t v_max, int6
   Workspace* ws = nullptr;
   const int grid = tfce_grid(n_items);
   const int64_t slot_words = tfce_slot_words(v_max);
+  const size_t slots_bytes = (size_t)grid * slot_words * sizeof(int32_t);
+  const size_t prep_bytes = (size_t)n_items * kPrepWords * sizeof(int32_t);
   // raw_max is mandatory for the kernel: park it in scratch when not wanted
-  const size_t extra = d_raw ? 0 : (size_t)n_items * sizeof(float);
-  int rc = workspace_get(stream, (size_t)grid * slot_words * sizeof(int32_t) + extra + 256, &ws);
+  const size_t raw_bytes = (size_t)n_items * sizeof(float);
+  int rc = workspace_get(stream, slots_bytes + prep_bytes + raw_bytes + 256, &ws);
   if (rc) return rc;
+  char* wbase = reinterpret_cast<char*>(ws->ptr);
   TfceBatchArgs a{};
   a.surfs = d_surfs;
   a.n_surf = n_surf;
@@ -260,9 +273,8 @@ static int run_batch(const SurfDev* d_surfs, int32_t n_surf, int32_t v_max, int6
   a.v_max = v_max;
   a.queue = ws->queue;
   a.n_items = n_items;
-  a.raw_max = d_raw ? d_raw
-                    : reinterpret_cast<float*>(reinterpret_cast<char*>(ws->ptr) +
-                                               (size_t)grid * slot_words * sizeof(int32_t));
+  a.prep = reinterpret_cast<int32_t*>(wbase + slots_bytes);
+  a.raw_max = d_raw ? d_raw : reinterpret_cast<float*>(wbase + slots_bytes + prep_bytes);
   hipError_t e = launch_tfce_batch(a, full, grid, stream);
   if (e != hipSuccess) return set_error(-(int)e, std::string("tfce launch: ") + hipGetErrorString(e));
   return 0;
@@ -317,6 +329,7 @@ int tfce_surfset_create(const tfce_adj_t* const* adjs, int32_t n, const int64_t*
     SurfDev sd;
     sd.row_ptr = a->d_row_ptr;
     sd.col = a->d_col;
+    sd.sizepow = a->d_sizepow;
     sd.offset = offsets[idx[k]];
     sd.V = a->V;
     sd.flags = a->flags;

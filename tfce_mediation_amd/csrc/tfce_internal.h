@@ -23,6 +23,7 @@ constexpr int32_t kSurfZeroInScale = 4;   // the rescale max includes those zero
 struct SurfDev {
   const int32_t* row_ptr;  // V+1
   const int32_t* col;      // nnz (indices local to the surface)
+  const double* sizepow;   // [V+1] (double)s^E, null when E == 1
   int64_t offset;          // first element of this surface inside a map row
   int32_t V;
   int32_t flags;
@@ -56,10 +57,20 @@ struct TfceBatchArgs {
   int64_t slot_stride;     // int32 words per slot
   int32_t v_max;
   int32_t* queue;          // dynamic work counter, zeroed before launch
+  int32_t* prep;           // [n_items][kPrepWords] per-item maximum and level tables
   int64_t n_items;
 };
 
+// Per-item record written by the prepare kernel (int32 words).
+constexpr int kPrepL = 0;     // levels visited, or a negative status
+constexpr int kPrepOwn = 1;   // max of sign*map over the surface (float bits)
+constexpr int kPrepT0 = 2;    // first threshold (float bits)
+constexpr int kPrepDT = 3;    // step (float bits)
+constexpr int kPrepT = 4;     // kMaxLevels thresholds, then kMaxLevels values of T^H
+constexpr int kPrepWords = kPrepT + 2 * kMaxLevels;
+
 hipError_t launch_tfce_batch(const TfceBatchArgs& a, bool full, int grid, hipStream_t stream);
+hipError_t launch_sizepow(double* out, int32_t n, float E, hipStream_t stream);
 hipError_t launch_fill_f32(float* p, int64_t n, float v, hipStream_t stream);
 
 // Slot layout: 8 arrays of the largest surface's element count.

@@ -7,22 +7,26 @@
 // in float in descending-T order.
 //
 // Design (not the reference's): one work item = one (surface, map, sign) image.
-// A persistent grid pulls items from an atomic queue; each workgroup owns one
-// scratch slot and runs the whole threshold sweep of its item:
-//   1. max-reduce the image, build the float threshold table in LDS;
-//   2. bucket the vertices by activation level (counting sort, LDS histogram);
-//      compact id = position in that order, so the active set at level l is the
-//      id prefix [0, end_l) and "is u active" is `pos[u] < end_l`;
-//   3. per level: hook the newly active vertices into a lock-free union-find
-//      (CAS hooking larger id under smaller, path halving), merge component
-//      sizes into the surviving roots with atomics, then
-//        MAX mode : add the level increment to one value per root; a merged
-//                   root keeps the larger value (atomicMax on the float bits);
-//                   the largest TFCE of a component is held by its earliest
-//                   vertex, so max over roots == max over vertices, bit-exact;
-//        FULL mode: add the increment of its root to every active vertex.
+//   tfce_prepare_kernel - one workgroup per item, all items at once: the
+//     image maximum and the float threshold / T^H tables of the item.
+//   tfce_batch_kernel   - a persistent grid pulls items from an atomic queue;
+//     each workgroup owns one scratch slot and runs the item's sweep:
+//     1. bucket the vertices by activation level (counting sort, LDS
+//        histogram); compact id = position in that order, so the active set
+//        at level l is the id prefix [0, end_l) and "is u active" is
+//        `pos[u] < end_l`;
+//     2. per level: hook the newly active vertices into a lock-free union-find
+//        (CAS hooking the larger root under the smaller), merge component sizes
+//        into the surviving roots with atomics, then
+//          MAX mode : add the level increment to one value per root; a merged
+//                     root keeps the larger value (atomicMax on float bits); the
+//                     largest TFCE of a component is held by its earliest vertex,
+//                     so max over roots == max over vertices, bit for bit;
+//          FULL mode: add the increment of its root to every active vertex.
 // Every level touches only the new vertices and the current roots (MAX) or the
-// active prefix (FULL) - never the whole graph.
+// active prefix (FULL).  The sweep kernel carries no transcendental math (T^H
+// comes from the prepare kernel, s^E from a per-adjacency table) so it keeps
+// few registers and many waves resident: it is latency bound.
 #include "tfce_internal.h"
 
 namespace tfcehip {
@@ -61,20 +65,12 @@ __device__ __forceinline__ void uf_union(int32_t* parent, int a, int b) {
   }
 }
 
-// fl(T^H), correctly rounded (T*T is exact in double).  The reference calls libm
-// powf, which is one ulp off for ~0.06% of inputs.
-__device__ __forceinline__ float pow_h(float T, float H) {
-  if (H == 2.0f) return __fmul_rn(T, T);
-  if (H == 1.0f) return T;
-  return __double2float_rn(pow((double)T, (double)H));
-}
-
-// (float)(pow((double)size, (double)E) * (double)HH), fast_tfce.hpp:77
-__device__ __forceinline__ float tfce_inc(int size, float HH, float E) {
-  if (E == 1.0f) return __fmul_rn((float)size, HH);  // exact product, one rounding
-  double s = (double)size;
-  double p = (E == 0.5f) ? sqrt(s) : pow(s, (double)E);
-  return __double2float_rn(p * (double)HH);
+// (float)(s^E * HH) for one component: E == 1 is one correctly rounded float
+// product (exact in double, as fast_tfce.hpp:77 computes it); otherwise the
+// per-adjacency table holds (double)s^E.
+__device__ __forceinline__ float tfce_inc(int size, float HH, float E, const double* sizepow) {
+  if (E == 1.0f) return __fmul_rn((float)size, HH);
+  return __double2float_rn(sizepow[size] * (double)HH);
 }
 
 __device__ __forceinline__ float wave_max(float v) {
@@ -108,7 +104,7 @@ __device__ __forceinline__ void wave_append(bool take, int c, int32_t* list, int
   if (take) list[base + __popcll(m & ((1ull << lane) - 1ull))] = c;
 }
 
-constexpr int kIlp = 4;  // independent union-find chains per thread
+constexpr int kIlp = 4;  // independent find chains per thread in the merge phases
 
 // Roots of up to kIlp elements at once; the loads of the chains interleave.
 // Path splitting: a visited node is re-pointed to its grandparent.
@@ -138,38 +134,6 @@ __device__ __forceinline__ void find_batch(int32_t* parent, int (&x)[kIlp], cons
   }
 }
 
-// Join the pairs (ua[j], ub[j]) for live[j]: kIlp unions in flight, CAS
-// hooking the larger root under the smaller.
-__device__ __forceinline__ void union_pairs(int32_t* parent, int (&ua)[kIlp], int (&ub)[kIlp], bool (&live)[kIlp]) {
-  while (live[0] || live[1] || live[2] || live[3]) {
-    int pa[kIlp], pb[kIlp];
-#pragma unroll
-    for (int j = 0; j < kIlp; ++j) {
-      pa[j] = live[j] ? ld(parent + ua[j]) : 0;
-      pb[j] = live[j] ? ld(parent + ub[j]) : 0;
-    }
-#pragma unroll
-    for (int j = 0; j < kIlp; ++j) {
-      if (!live[j]) continue;
-      const bool ra = pa[j] == ua[j], rb = pb[j] == ub[j];
-      if (!ra) ua[j] = pa[j];
-      if (!rb) ub[j] = pb[j];
-      if (ra && rb) {
-        if (ua[j] == ub[j]) {
-          live[j] = false;
-        } else {
-          const int hi = max(ua[j], ub[j]), lo = min(ua[j], ub[j]);
-          int expected = hi;
-          // on failure hi gained a parent; the next round climbs from it
-          if (__hip_atomic_compare_exchange_strong(parent + hi, &expected, lo, __ATOMIC_RELAXED,
-                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
-            live[j] = false;
-        }
-      }
-    }
-  }
-}
-
 __device__ __forceinline__ int wave_incl_scan(int v) {
   const int lane = threadIdx.x & 63;
 #pragma unroll
@@ -194,6 +158,75 @@ __device__ __forceinline__ int level_of(float xv, const float* s_T, int L, float
   return l;
 }
 
+struct ItemRef {
+  int s, p;
+  float sign;
+};
+
+__device__ __forceinline__ ItemRef decode_item(const TfceBatchArgs& a, int64_t item) {
+  const int per_surf = a.n_maps * a.n_signs;
+  ItemRef r;
+  r.s = (int)(item / per_surf);
+  const int rem = (int)(item - (int64_t)r.s * per_surf);
+  r.p = rem / a.n_signs;
+  r.sign = (a.n_signs == 2) ? ((rem % 2) ? -1.0f : 1.0f) : a.sign0;
+  return r;
+}
+
+// fl(T^H), correctly rounded (T*T is exact in double).  The reference calls libm
+// powf, which is one ulp off for ~0.06% of inputs.
+__device__ __forceinline__ float pow_h(float T, float H) {
+  if (H == 2.0f) return __fmul_rn(T, T);
+  if (H == 1.0f) return T;
+  return __double2float_rn(pow((double)T, (double)H));
+}
+
+// One workgroup per item: maximum of sign*map over the surface (NaN ignored),
+// first threshold, step, and the level tables.
+__global__ __launch_bounds__(kTfceThreads) void tfce_prepare_kernel(TfceBatchArgs a) {
+  __shared__ float s_red[kTfceThreads / 64];
+  const int64_t item = blockIdx.x;
+  const ItemRef it = decode_item(a, item);
+  const SurfDev S = a.surfs[it.s];
+  const float* x = a.maps + (int64_t)it.p * a.map_stride + S.offset;
+  const int V = S.V;
+  float m0 = -INFINITY, m1 = -INFINITY, m2 = -INFINITY, m3 = -INFINITY;
+  int v = threadIdx.x;
+  for (; v + 3 * kTfceThreads < V; v += 4 * kTfceThreads) {
+    m0 = fmaxf(m0, it.sign * x[v]);
+    m1 = fmaxf(m1, it.sign * x[v + kTfceThreads]);
+    m2 = fmaxf(m2, it.sign * x[v + 2 * kTfceThreads]);
+    m3 = fmaxf(m3, it.sign * x[v + 3 * kTfceThreads]);
+  }
+  for (; v < V; v += kTfceThreads) m0 = fmaxf(m0, it.sign * x[v]);
+  const float m = block_max(fmaxf(fmaxf(m0, m1), fmaxf(m2, m3)), s_red);
+  if (threadIdx.x != 0) return;
+  int32_t* rec = a.prep + item * kPrepWords;
+  float* T = reinterpret_cast<float*>(rec + kPrepT);
+  float* HH = reinterpret_cast<float*>(rec + kPrepT + kMaxLevels);
+  const float T0 = a.ext_T0 ? a.ext_T0[item] : ((S.flags & kSurfZeroInLevels) ? fmaxf(m, 0.0f) : m);
+  const float dT = (a.dh != 0.0f) ? a.dh : __fdiv_rn(T0, 100.0f);
+  int L = 0, status = 0;
+  if (T0 >= a.min_t) {
+    if (!(dT > 0.0f)) {
+      status = kStatusDegenerate;
+    } else {
+      float t = T0;
+      while (t >= a.min_t) {
+        if (L == kMaxLevels) { status = kStatusTooManyLevels; break; }
+        T[L] = t;
+        HH[L] = pow_h(t, S.H);
+        ++L;
+        t = __fsub_rn(t, dT);
+      }
+    }
+  }
+  rec[kPrepL] = status < 0 ? status : L;
+  rec[kPrepOwn] = __float_as_int(m);
+  rec[kPrepT0] = __float_as_int(T0);
+  rec[kPrepDT] = __float_as_int(dT);
+}
+
 template <bool FULL>
 __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs a) {
   __shared__ float s_T[kMaxLevels];
@@ -201,8 +234,7 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
   __shared__ int s_start[kMaxLevels + 1];
   __shared__ int s_cur[kMaxLevels];
   __shared__ float s_red[kTfceThreads / 64];
-  __shared__ int s_item_lo, s_item_hi, s_L, s_status, s_nroots_new;
-  __shared__ float s_own, s_T0, s_dT;
+  __shared__ int s_item_lo, s_item_hi, s_nroots_new;
 
   const int tid = threadIdx.x;
   const int64_t vstride = (a.v_max + 63) & ~63;
@@ -225,64 +257,29 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
     const int64_t item = ((int64_t)s_item_hi << 31) | s_item_lo;
     if (item >= a.n_items) break;  // uniform across the workgroup
 
-    const int per_surf = a.n_maps * a.n_signs;
-    const int s = (int)(item / per_surf);
-    const int rem = (int)(item - (int64_t)s * per_surf);
-    const int p = rem / a.n_signs;
-    const float sign = (a.n_signs == 2) ? ((rem % 2) ? -1.0f : 1.0f) : a.sign0;
-    const SurfDev S = a.surfs[s];
+    const ItemRef itr = decode_item(a, item);
+    const float sign = itr.sign;
+    const SurfDev S = a.surfs[itr.s];
     const int V = S.V;
-    const float* x = a.maps + (int64_t)p * a.map_stride + S.offset;
-    float* enhn = (FULL && a.enhn) ? a.enhn + (int64_t)p * a.map_stride + S.offset : nullptr;
+    const float* x = a.maps + (int64_t)itr.p * a.map_stride + S.offset;
+    float* enhn = (FULL && a.enhn) ? a.enhn + (int64_t)itr.p * a.map_stride + S.offset : nullptr;
     const float* vd = (FULL && a.vdensity) ? a.vdensity + S.offset : nullptr;
     const bool symmetric = (S.flags & kSurfSymmetric) != 0;
+    const int32_t* rec = a.prep + item * kPrepWords;
+    const int status = rec[kPrepL];
+    const int L = status > 0 ? status : 0;
+    const float T0 = __int_as_float(rec[kPrepT0]);
+    const float dT = __int_as_float(rec[kPrepDT]);
+    const float own = __int_as_float(rec[kPrepOwn]);
 
-    // ---- phase 0: maximum and level table -----------------------------------
-    float m0 = -INFINITY, m1 = -INFINITY, m2 = -INFINITY, m3 = -INFINITY;
-    {
-      int v = tid;
-      for (; v + 3 * kTfceThreads < V; v += 4 * kTfceThreads) {
-        m0 = fmaxf(m0, sign * x[v]);
-        m1 = fmaxf(m1, sign * x[v + kTfceThreads]);
-        m2 = fmaxf(m2, sign * x[v + 2 * kTfceThreads]);
-        m3 = fmaxf(m3, sign * x[v + 3 * kTfceThreads]);
-      }
-      for (; v < V; v += kTfceThreads) m0 = fmaxf(m0, sign * x[v]);
+    // ---- phase 1: level tables, activation level of every element ------------
+    for (int l = tid; l < L; l += kTfceThreads) {
+      s_T[l] = __int_as_float(rec[kPrepT + l]);
+      s_HH[l] = __int_as_float(rec[kPrepT + kMaxLevels + l]);
+      s_cur[l] = 0;
     }
-    const float m = block_max(fmaxf(fmaxf(m0, m1), fmaxf(m2, m3)), s_red);
-    if (tid == 0) {
-      s_own = m;
-      float T0 = a.ext_T0 ? a.ext_T0[item] : ((S.flags & kSurfZeroInLevels) ? fmaxf(m, 0.0f) : m);
-      float dT = (a.dh != 0.0f) ? a.dh : __fdiv_rn(T0, 100.0f);
-      int L = 0, status = 0;
-      if (T0 >= a.min_t) {
-        if (!(dT > 0.0f)) {
-          status = kStatusDegenerate;
-        } else {
-          float T = T0;
-          while (T >= a.min_t) {
-            if (L == kMaxLevels) { status = kStatusTooManyLevels; break; }
-            s_T[L] = T;
-            s_HH[L] = pow_h(T, S.H);
-            ++L;
-            T = __fsub_rn(T, dT);
-          }
-        }
-      }
-      if (status < 0) L = 0; else status = L;
-      s_L = L;
-      s_status = status;
-      s_T0 = T0;
-      s_dT = dT;
-    }
-    __syncthreads();
-    const int L = s_L;
-
-    // ---- phase 1: activation level of every element, histogram --------------
-    for (int l = tid; l < L; l += kTfceThreads) s_cur[l] = 0;
     __syncthreads();
     if (L > 0) {
-      const float T0 = s_T0, dT = s_dT;
       for (int v = tid; v < V; v += kTfceThreads) {
         const int lev = level_of(sign * x[v], s_T, L, T0, dT);
         if (lev != kNever) atomicAdd(&s_cur[lev], 1);
@@ -322,7 +319,7 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
     __syncthreads();
     const int A = (L > 0) ? s_start[L] : 0;
 
-    // ---- phase 4: threshold sweep -------------------------------------------
+    // ---- phase 3: threshold sweep -------------------------------------------
     int n_roots = 0;
     int cur = 0;
     for (int l = 0; l < L; ++l) {
@@ -331,70 +328,44 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
       int32_t* rold = roots[cur];
       int32_t* rnew = roots[cur ^ 1];
       if (b1 > b0) {
-        // 4a: one set per new vertex
+        // 3a: one set per new vertex
         for (int i = b0 + tid; i < b1; i += kTfceThreads) {
           parent[i] = i;
           size[i] = 1;
           val[i] = (FULL && enhn) ? enhn[order[i]] : 0.0f;
         }
         __syncthreads();
-        // 4b: join new vertices with their active neighbours.  Neighbour
-        // positions of two vertices are gathered together, then the pairs are
-        // joined kIlp at a time.
-        for (int i0 = b0; i0 < b1; i0 += 2 * kTfceThreads) {
-          int iv[2], vv[2], e0[2], e1[2];
-          float xv[2];
+        // 3b: join new vertices with their active neighbours; the positions of
+        // up to 8 neighbours are gathered before the first union
+        for (int i = b0 + tid; i < b1; i += kTfceThreads) {
+          const int v = order[i];
+          const int e0 = S.row_ptr[v], e1 = S.row_ptr[v + 1];
+          const float xv = symmetric ? 0.0f : sign * x[v];
+          for (int e = e0; e < e1; e += 8) {
+            int u[8], q[8];
 #pragma unroll
-          for (int h = 0; h < 2; ++h) {
-            iv[h] = i0 + h * kTfceThreads + tid;
-            vv[h] = iv[h] < b1 ? order[iv[h]] : -1;
-          }
+            for (int j = 0; j < 8; ++j) u[j] = (e + j < e1) ? S.col[e + j] : -1;
 #pragma unroll
-          for (int h = 0; h < 2; ++h) {
-            e0[h] = vv[h] >= 0 ? S.row_ptr[vv[h]] : 0;
-            e1[h] = vv[h] >= 0 ? S.row_ptr[vv[h] + 1] : 0;
-            xv[h] = (!symmetric && vv[h] >= 0) ? sign * x[vv[h]] : 0.0f;
-          }
-          int ua[kIlp], ub[kIlp];
-          bool live[kIlp];
-          int nq = 0;
+            for (int j = 0; j < 8; ++j) q[j] = u[j] >= 0 ? pos[u[j]] : kNever;
 #pragma unroll
-          for (int j = 0; j < kIlp; ++j) live[j] = false;
-#pragma unroll
-          for (int h = 0; h < 2; ++h) {
-            for (int e = e0[h]; e < e1[h]; e += 8) {
-              int u[8], q[8];
-#pragma unroll
-              for (int j = 0; j < 8; ++j) u[j] = (e + j < e1[h]) ? S.col[e + j] : -1;
-#pragma unroll
-              for (int j = 0; j < 8; ++j) q[j] = u[j] >= 0 ? pos[u[j]] : kNever;
-#pragma unroll
-              for (int j = 0; j < 8; ++j) {
-                bool take = q[j] < b1 && q[j] != iv[h];
-                if (take && !symmetric && q[j] >= b0) {
-                  // same level, entry listed from v only: the reference uses it
-                  // iff u was activated first (descending value, ties by index)
+            for (int j = 0; j < 8; ++j) {
+              bool take = q[j] < b1 && q[j] != i;
+              if (take && q[j] >= b0) {
+                if (symmetric) {
+                  take = q[j] < i;  // a same-level edge is joined once
+                } else {
+                  // listed from v only: the reference uses it iff u was
+                  // activated first (descending value, ties by index)
                   const float xu = sign * x[u[j]];
-                  take = xu > xv[h] || (xu == xv[h] && u[j] < vv[h]);
-                }
-                if (take && symmetric && q[j] >= b0) take = q[j] < iv[h];  // same-level edge once
-                if (take) {
-                  // append to the pending pairs; flush when kIlp are queued
-#pragma unroll
-                  for (int k = 0; k < kIlp; ++k)
-                    if (k == nq) { ua[k] = iv[h]; ub[k] = q[j]; live[k] = true; }
-                  if (++nq == kIlp) {
-                    union_pairs(parent, ua, ub, live);
-                    nq = 0;
-                  }
+                  take = xu > xv || (xu == xv && u[j] < v);
                 }
               }
+              if (take) uf_union(parent, i, q[j]);
             }
           }
-          if (nq > 0) union_pairs(parent, ua, ub, live);
         }
         __syncthreads();
-        // 4b: merge sizes (and MAX values) into surviving roots; collect roots
+        // 3c: merge sizes (and MAX values) into surviving roots; collect roots
         const int n_cand = n_roots + (b1 - b0);
         for (int k0 = 0; k0 < n_cand; k0 += kIlp * kTfceThreads) {
           int c[kIlp], r[kIlp];
@@ -427,7 +398,7 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
         cur ^= 1;
       }
       int32_t* rl = roots[cur];
-      // 4c: level increment
+      // 3d: level increment
       if (!FULL) {
         for (int k0 = 0; k0 < n_roots; k0 += kIlp * kTfceThreads) {
           int r[kIlp], sz[kIlp];
@@ -444,13 +415,13 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
           }
 #pragma unroll
           for (int j = 0; j < kIlp; ++j)
-            if (r[j] >= 0) val[r[j]] = cv[j] + tfce_inc(sz[j], HH, S.E);
+            if (r[j] >= 0) val[r[j]] = cv[j] + tfce_inc(sz[j], HH, S.E, S.sizepow);
         }
         __syncthreads();
       } else {
         for (int k = tid; k < n_roots; k += kTfceThreads) {
           const int r = rl[k];
-          incv[r] = tfce_inc(ld(size + r), HH, S.E);
+          incv[r] = tfce_inc(ld(size + r), HH, S.E, S.sizepow);
         }
         __syncthreads();
         for (int i0 = 0; i0 < b1; i0 += kIlp * kTfceThreads) {
@@ -488,7 +459,6 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
         for (int i = tid; i < A; i += kTfceThreads) enhn[order[i]] = val[i];
     }
     best = block_max(best, s_red);
-    const float own = s_own;
     const float smax = (S.flags & kSurfZeroInScale) ? fmaxf(own, 0.0f) : own;
     const float sc = __fdiv_rn(smax, 100.0f);
     float scaled;
@@ -508,10 +478,17 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
       a.raw_max[item] = best;
       if (a.scaled_max) a.scaled_max[item * a.out_mul + a.out_add] = scaled;
       if (a.own_max) a.own_max[item] = own;
-      if (a.status) a.status[item] = s_status;
+      if (a.status) a.status[item] = status;
     }
     __syncthreads();
   }
+}
+
+__global__ void sizepow_kernel(double* out, int32_t n, double E) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > n) return;
+  const double s = (double)i;
+  out[i] = (E == 0.5) ? sqrt(s) : pow(s, E);
 }
 
 __global__ void fill_f32_kernel(float* p, int64_t n, float v) {
@@ -526,10 +503,16 @@ __global__ void zero_queue_kernel(int32_t* q) { q[0] = 0; q[1] = 0; }
 
 hipError_t launch_tfce_batch(const TfceBatchArgs& a, bool full, int grid, hipStream_t stream) {
   hipLaunchKernelGGL(zero_queue_kernel, dim3(1), dim3(1), 0, stream, a.queue);
+  hipLaunchKernelGGL(tfce_prepare_kernel, dim3((unsigned)a.n_items), dim3(kTfceThreads), 0, stream, a);
   if (full)
     hipLaunchKernelGGL(tfce_batch_kernel<true>, dim3(grid), dim3(kTfceThreads), 0, stream, a);
   else
     hipLaunchKernelGGL(tfce_batch_kernel<false>, dim3(grid), dim3(kTfceThreads), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_sizepow(double* out, int32_t n, float E, hipStream_t stream) {
+  hipLaunchKernelGGL(sizepow_kernel, dim3((unsigned)((n + 256) / 256)), dim3(256), 0, stream, out, n, (double)E);
   return hipGetLastError();
 }
 

@@ -138,6 +138,29 @@ int tfce_ols_info(const tfce_ols_model_t *model, int32_t *k, int32_t *n_fixed, i
 int tfce_ols_perm_t(const tfce_ols_model_t *model, const int32_t *d_perm, int32_t P, float *d_t,
                     int64_t ldt, hipStream_t stream);
 
+/* One design, FP64 throughout (the numpy path of cynumstats.pyx:28-112 for a
+ * single X): coefficients with inv(X'X) computed here, residuals, SSE and
+ * se_j = fl32(sqrt(SSE/(n-k) * S[j][j])) with S = invXX_se (the invXX the
+ * caller passes to tval_int, cynumstats.pyx:59-64) or inv(X'X) when NULL;
+ * t = a / se.  Also the per-element sum and sum of squares of y (for calcF's
+ * grand-mean TSS, :31-36).  X is host n x k row-major (n <= 2048, k <= 8);
+ * outputs are device arrays [k][V] (coef, se, t), [n][V] (resid), [V], any
+ * may be NULL.  Blocking. */
+int tfce_ols_direct(const float *d_Y, int32_t n, int64_t V, int64_t ldy, const double *X, int32_t k,
+                    const double *invXX_se, double *d_coef, float *d_se, double *d_t, double *d_resid,
+                    double *d_sse, double *d_sumy, double *d_sumyy, hipStream_t stream);
+
+/* se[j][v] = fl32(sqrt(sigma2[v] * invXX[j][j])) for j < k (cynumstats.pyx:47-52);
+ * invXX is host k x k, sigma2 and se are device arrays.  Blocking. */
+int tfce_se_of_slope(const double *d_sigma2, const double *invXX, int32_t k, int64_t V, float *d_se,
+                     hipStream_t stream);
+
+/* Indirect-effect z from the path t-values, elementwise (pyfunc.py:150-161):
+ * alg 0 Aroian 1/sqrt(1/tb^2 + 1/ta^2 + 1/(ta^2 tb^2)), 1 Sobel, 2 Goodman.
+ * d_ta == NULL uses the scalar ta_const (mediation type Y, pyfunc.py:142). */
+int tfce_sobel_z(const float *d_ta, const float *d_tb, int64_t n, int32_t alg, float ta_const,
+                 float *d_z, hipStream_t stream);
+
 /* ------------------------------------------------------------------ FWER */
 /* 1 - p(FWE) of each element against the sorted (ascending) null, FP64.
  * mode 0: calculate_fweP_vertex.py:37-69 / calculate_fweP_voxel.py:23-57
@@ -151,6 +174,20 @@ int tfce_fwe_p(const double *d_sorted_null, int32_t n_null, const float *d_value
  * log-TFCE, tm_func.py:431-446). */
 int tfce_fwe_p64(const double *d_sorted_null, int32_t n_null, const double *d_values, int64_t V,
                  int32_t mode, double *d_out, hipStream_t stream);
+
+/* tfce_batch with options:
+ *   TFCE_BATCH_SHARED_THRESHOLDS - every surface of a (map, sign) uses the
+ *     threshold sequence of the maximum over the whole set, as when the
+ *     surfaces are TFCE'd as one merged image (tm_func.py:54-123 with its
+ *     merged adjacency, tm_multimodality_multisurface_regression.py:443-444);
+ *     the rescale still uses each surface's own maximum (tm_func.py:83-91);
+ *   TFCE_BATCH_ZERO_IN_IMAGE - that merged image also holds zeros (vertex
+ *     masks in create_full_mask, tm_func.py:502-510). */
+#define TFCE_BATCH_SHARED_THRESHOLDS 1
+#define TFCE_BATCH_ZERO_IN_IMAGE 2
+int tfce_batch_ex(const tfce_surfset_t *set, const float *d_maps, int64_t map_stride, int P,
+                  int sign_mask, uint32_t flags, const float *d_vdensity, float *d_scaled, float *d_raw,
+                  float *d_own, int32_t *d_status, hipStream_t stream);
 
 /* Number of resident workgroups per CU used by the TFCE sweep (autotuner knob). */
 int tfce_set_occupancy(int32_t wgs_per_cu);

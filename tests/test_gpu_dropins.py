@@ -1,0 +1,246 @@
+"""Drop-in modules (cynumstats, pyfunc, tm_func) and the batched randomise
+drivers against the oracle on small synthetic data.
+
+Bars: FP64 statistics (cynumstats) rtol 1e-9 against numpy FP64 (the GPU sums
+in a different order); TFCE-based values are bit-exact when the GPU is fed the
+oracle's own float32 statistic, and otherwise (end to end, FP32 contraction)
+agree to rtol 1e-4 with the median difference below 1e-6.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import fwe as ofwe
+from oracle import ols as ools
+from oracle import tfce as otfce
+from tfce_mediation_amd import adjacency, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as _t
+    assert _t.cuda.is_available()
+    return _t
+
+
+def design(n, k, seed):
+    rng = np.random.default_rng(seed)
+    return np.column_stack([np.ones(n)] + [rng.standard_normal(n) for _ in range(k - 2)]
+                           + [(rng.random(n) < 0.5).astype(float)])
+
+
+def smooth_data(row_ptr, col, n, seed, passes=3):
+    rng = np.random.default_rng(seed)
+    V = len(row_ptr) - 1
+    Y = rng.standard_normal((n, V))
+    deg = np.diff(row_ptr).astype(np.int64)
+    src = np.repeat(np.arange(V), deg)
+    for _ in range(passes):
+        S = np.zeros_like(Y)
+        np.add.at(S.T, src, Y.T[col])
+        Y = (Y + S) / (deg + 1)
+    Y = (Y - Y.mean(0)) / Y.std(0)
+    return Y.astype(np.float32)
+
+
+# ---------------------------------------------------------------- cynumstats
+def test_cynumstats_match_fp64(torch):
+    from tfce_mediation_amd import cynumstats as cs
+    say("[gpu] cynumstats")
+    n, k, V = 80, 4, 900
+    X = design(n, k, 1)
+    rng = np.random.default_rng(2)
+    Y = (rng.standard_normal((n, V)) + 5.0).astype(np.float32)
+    invXX = np.linalg.inv(X.T @ X)
+    np.testing.assert_allclose(cs.tval_int(X, invXX, Y, n, k, V), ools.tval_int(X, invXX, Y, n, k, V), rtol=1e-9)
+    np.testing.assert_allclose(cs.cy_lin_lstsqr_mat(X, Y), ools.cy_lin_lstsqr_mat(X, Y), rtol=1e-9, atol=1e-12)
+    a, rss = cs.cy_lin_lstsqr_mat_residual(X, Y)
+    res = Y - X @ ools.cy_lin_lstsqr_mat(X, Y)
+    np.testing.assert_allclose(rss, (res ** 2).sum(0), rtol=1e-9)
+    b, se = cs.calc_beta_se(X[:, 1], Y, n, V)
+    rb, rse = ools.calc_beta_se(X[:, 1], Y, n, V)
+    np.testing.assert_allclose(b, rb, rtol=1e-9)
+    assert se.dtype == np.float32
+    np.testing.assert_allclose(se, rse, rtol=1e-6)
+    r = cs.resid_covars(X, Y.T)
+    np.testing.assert_allclose(r, ools.resid_covars(X, Y.T.astype(np.float64)), atol=1e-9)
+    F = cs.calcF(X, Y, n, k)
+    np.testing.assert_allclose(F, ools.calcF(X, Y.astype(np.float64), n, k), rtol=1e-8)
+    s2 = rng.random(V) + 0.5
+    np.testing.assert_array_equal(cs.se_of_slope(V, invXX, s2, k), ools.se_of_slope(V, invXX, s2, k))
+
+
+def test_calc_sobelz_matches_reference_formula(torch):
+    from tfce_mediation_amd import pyfunc
+    say("[gpu] calc_sobelz")
+    rng = np.random.default_rng(5)
+    n, V = 120, 700
+    x = rng.standard_normal(n)
+    y = rng.standard_normal(n)
+    M = (0.4 * x[:, None] + rng.standard_normal((n, V))).astype(np.float32)
+    for mt in ("M", "I"):
+        got = pyfunc.calc_sobelz(mt, x, y, M, n, V)
+        exp = ools.calc_sobelz(mt, x, y, M.astype(np.float64), n, V)
+        np.testing.assert_allclose(got, exp, rtol=2e-6)
+
+
+# ------------------------------------------------------------ TFCE writers
+def _hemis(order=3, seed=0):
+    row_ptr, col = synth.icosphere_csr(order)
+    V = len(row_ptr) - 1
+    rng = np.random.default_rng(seed)
+    m_lh = rng.random(V) < 0.9
+    m_rh = rng.random(V) < 0.85
+    return row_ptr, col, m_lh, m_rh
+
+
+def _oracle_vertex_max(stat, m_lh, m_rh, row_ptr, col, vd_lh=1, vd_rh=1):
+    nl = int(m_lh.sum())
+    out = []
+    for m, part, vd in ((m_lh, stat[:nl], vd_lh), (m_rh, stat[nl:], vd_rh)):
+        full = np.zeros(len(m), np.float32)
+        full[m] = part
+        enh, _, _ = otfce.tfce(row_ptr, col, full, 2.0, 1.0, exact_pow=True)
+        out.append((enh * (full.max() / np.float32(100)) * vd).max())
+    return np.array(out).max()
+
+
+def test_write_perm_max_vertex_and_voxel(torch, tmp_path):
+    from tfce_mediation_amd import pyfunc
+    from tfce_mediation_amd.tfce import CreateAdjSet
+    say("[gpu] write_perm_maxTFCE_vertex/voxel")
+    row_ptr, col, m_lh, m_rh = _hemis()
+    lists = [col[row_ptr[i]:row_ptr[i + 1]].tolist() for i in range(len(row_ptr) - 1)]
+    op = CreateAdjSet(2, 1, lists)
+    rng = np.random.default_rng(3)
+    stat = rng.standard_normal(int(m_lh.sum() + m_rh.sum())).astype(np.float32)
+    vd = adjacency.vertex_density(row_ptr)
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        for s in (stat, -stat):
+            got = pyfunc.write_perm_maxTFCE_vertex("tstat_con1", s, int(m_lh.sum()), m_lh, m_rh, op, op, vd, vd)
+            assert got == _oracle_vertex_max(s, m_lh, m_rh, row_ptr, col, vd, vd)
+        lines = open("perm_tstat_con1_TFCE_maxVertex.csv").read().split()
+        assert len(lines) == 2 and all(len(l.split(".")[1]) == 4 for l in lines)
+        vox = rng.standard_normal(len(row_ptr) - 1).astype(np.float32)
+        got = pyfunc.write_perm_maxTFCE_voxel("tstat_con1", vox, op)
+        enh, _, _ = otfce.tfce(row_ptr, col, vox, 2.0, 1.0, exact_pow=True)
+        assert got == ofwe.perm_max_voxel(enh, vox)
+    finally:
+        os.chdir(cwd)
+
+
+# --------------------------------------------------------- randomise drivers
+def test_vertex_randomise_null_matches_oracle_loop(torch, tmp_path):
+    from tfce_mediation_amd import randomise
+    say("[gpu] randomise.vertex_regress")
+    row_ptr, col, m_lh, m_rh = _hemis(order=3, seed=4)
+    nl, nr = int(m_lh.sum()), int(m_rh.sum())
+    n = 40
+    full = smooth_data(row_ptr, col, n, 6)
+    ny = np.concatenate([full[:, m_lh], smooth_data(row_ptr, col, n, 7)[:, m_rh]], 1)
+    X = design(n, 3, 8)
+    perms = randomise.permutation_indices(n, range(1, 13), seed=5)
+    lists = [col[row_ptr[i]:row_ptr[i + 1]].tolist() for i in range(len(row_ptr) - 1)]
+    nulls = randomise.vertex_regress(ny, X[:, 1:], lists, lists, m_lh, m_rh, [2, 1], perms,
+                                     out_dir=str(tmp_path), precision="fp32")
+    for c in (1, 2):
+        got = nulls[c]
+        exp = np.zeros_like(got)
+        for p, perm in enumerate(perms):
+            t = ools.perm_t(X, ny, [perm], [c])[0, 0].astype(np.float32)
+            exp[p, 0] = _oracle_vertex_max(t, m_lh, m_rh, row_ptr, col)
+            exp[p, 1] = _oracle_vertex_max(-t, m_lh, m_rh, row_ptr, col)
+        rel = np.abs(got - exp) / np.abs(exp)
+        say(f"    con{c}: max rel {rel.max():.2e} median {np.median(rel):.2e}")
+        assert rel.max() < 1e-4 and np.median(rel) < 1e-6
+        vals = np.loadtxt(os.path.join(tmp_path, "perm_tstat_con%d_TFCE_maxVertex.csv" % c))
+        np.testing.assert_allclose(vals, got.ravel(), atol=5e-5)
+
+
+def test_mediation_model_matches_calc_sobelz(torch):
+    from tfce_mediation_amd import randomise
+    say("[gpu] MediationModel")
+    rng = np.random.default_rng(11)
+    n, V = 90, 1500
+    x = rng.standard_normal(n)
+    y = rng.standard_normal(n)
+    M = (0.5 * x[:, None] + rng.standard_normal((n, V))).astype(np.float32)
+    perms = randomise.permutation_indices(n, range(6), seed=3)
+    for mt in ("M", "I", "Y"):
+        model = randomise.MediationModel(torch.from_numpy(M).cuda(), x, y, mt, precision="fp32")
+        z = model.z(torch.from_numpy(perms).cuda()).cpu().numpy()
+        for p, perm in enumerate(perms):
+            if mt == "Y":
+                from scipy.stats import linregress
+                xp, yp = x[perm], y[perm]
+                lr = linregress(xp, yp)
+                ta = lr.slope / lr.stderr
+                bb, bse = ools.calc_beta_se(np.column_stack([yp, xp]), M.astype(np.float64), n, V)
+                exp = ools.sobel_from_t(ta, bb / bse[1])
+            else:
+                exp = ools.calc_sobelz(mt, x[perm], y, M.astype(np.float64), n, V)
+            np.testing.assert_allclose(z[p], exp, rtol=2e-5, atol=1e-6)
+
+
+def test_mmr_merged_thresholds_match_merged_adjacency(torch, tmp_path):
+    from tfce_mediation_amd import randomise
+    say("[gpu] randomise.mmr merged")
+    surfs = [synth.icosphere_csr(2), synth.icosphere_csr(3),
+             adjacency.voxel_csr(synth.ellipsoid_mask((9, 9, 9), (4, 4, 4), (4, 4, 3.5)), 26)]
+    sizes = [len(s[0]) - 1 for s in surfs]
+    offs = list(np.cumsum([0] + sizes[:-1]))
+    n = 30
+    rng = np.random.default_rng(9)
+    Y = rng.standard_normal((n, sum(sizes))).astype(np.float32)
+    X = design(n, 3, 10)
+    perms = randomise.permutation_indices(n, range(1, 6), seed=2)
+    res = randomise.mmr(Y, X[:, 1:], surfs, offs, [(2.0, 1.0)] * 3, perms, merged=True, precision="fp32",
+                        out_dir=str(tmp_path))
+    mrp, mcol = adjacency.merge_csr(surfs)
+    for c in (1, 2):
+        for p, perm in enumerate(perms):
+            t = ools.perm_t(X, Y, [perm], [c])[0, 0].astype(np.float32)
+            for k, sgn in enumerate((1, -1)):
+                img = (sgn * t).astype(np.float32)
+                enh, _, _ = otfce.tfce(mrp, mcol, img, 2.0, 1.0, exact_pow=True)
+                for s in range(3):
+                    a, b = offs[s], offs[s] + sizes[s]
+                    exp = (enh[a:b] * (img[a:b].max() / np.float32(100))).max()
+                    got = res[(s, c)][p, k]
+                    assert abs(got - exp) <= 1e-4 * abs(exp) + 1e-6, (c, p, k, s, got, exp)
+    assert os.path.exists(os.path.join(tmp_path, "perm_maxTFCE_surf2_tcon2.csv"))
+
+
+def test_tm_func_calculate_tfce_observed(torch):
+    from tfce_mediation_amd import tm_func
+    from tfce_mediation_amd.tfce import CreateAdjSet
+    say("[gpu] tm_func.calculate_tfce")
+    surfs = [synth.icosphere_csr(2), synth.icosphere_csr(2)]
+    sizes = [len(s[0]) - 1 for s in surfs]
+    mrp, mcol = adjacency.merge_csr(surfs)
+    masks = [np.ones((sizes[0], 1, 1), bool), np.ones((sizes[1], 1, 1), bool)]
+    n = 25
+    rng = np.random.default_rng(12)
+    Y = rng.standard_normal((n, sum(sizes))).astype(np.float32)
+    x = rng.standard_normal(n)
+    op = CreateAdjSet(2.0, 1.0, (mrp, mcol))
+    pos_arr = tm_func.create_position_array(masks)
+    full = tm_func.create_full_mask(masks)
+    tv, tp, tn = tm_func.calculate_tfce(Y, masks, x, op, 1, pos_arr, full)
+    X = np.column_stack([np.ones(n), x])
+    t = ools.tval_int(X, np.linalg.inv(X.T @ X), Y.astype(np.float64), n, 2, Y.shape[1])[1].astype(np.float32)
+    np.testing.assert_allclose(tv[0], t, rtol=1e-6, atol=1e-6)
+    enh, _, _ = otfce.tfce(mrp, mcol, tv[0], 2.0, 1.0, exact_pow=True)
+    for s in range(2):
+        a, b = pos_arr[s], pos_arr[s + 1]
+        np.testing.assert_array_equal(tp[0, a:b], enh[a:b] * (tv[0][a:b].max() / np.float32(100)))

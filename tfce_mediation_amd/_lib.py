@@ -34,6 +34,7 @@ SIGNATURES = {
     "tfce_surfset_order": (c_int, [c_p, c_p]),
     "tfce_batch": (c_int, [c_p, c_p, c_i64, c_int, c_int, c_p, c_p, c_p, c_p, c_p, c_p, c_p]),
     "tfce_set_occupancy": (c_int, [c_i32]),
+    "tfce_batch_ex": (c_int, [c_p, c_p, c_i64, c_int, c_int, c_u32, c_p, c_p, c_p, c_p, c_p, c_p]),
     "tfce_ols_data_create": (c_int, [c_p, c_i32, c_i64, c_i64, c_i32, c_i32, c_p, c_p]),
     "tfce_ols_data_destroy": (c_int, [c_p]),
     "tfce_ols_model_create": (c_int, [c_p, c_p, c_i32, c_u32, c_p, c_i32, c_p, c_p]),
@@ -42,6 +43,10 @@ SIGNATURES = {
     "tfce_ols_perm_t": (c_int, [c_p, c_p, c_i32, c_p, c_i64, c_p]),
     "tfce_fwe_p": (c_int, [c_p, c_i32, c_p, c_i64, c_i32, c_p, c_p]),
     "tfce_fwe_p64": (c_int, [c_p, c_i32, c_p, c_i64, c_i32, c_p, c_p]),
+    "tfce_ols_direct": (c_int, [c_p, c_i32, c_i64, c_i64, c_p, c_i32, c_p, c_p, c_p, c_p, c_p, c_p, c_p,
+                                c_p, c_p]),
+    "tfce_se_of_slope": (c_int, [c_p, c_p, c_i32, c_i64, c_p, c_p]),
+    "tfce_sobel_z": (c_int, [c_p, c_p, c_i64, c_i32, c_f32, c_p, c_p]),
 }
 
 PREC = {"fp32": 0, "bf16x3": 1, "bf16x6": 2}

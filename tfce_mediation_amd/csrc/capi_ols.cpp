@@ -468,6 +468,85 @@ int tfce_ols_perm_t(const tfce_ols_model_t* md, const int32_t* d_perm, int32_t P
   return 0;
 }
 
+int tfce_ols_direct(const float* d_Y, int32_t n, int64_t V, int64_t ldy, const double* X, int32_t k,
+                    const double* invXX_se, double* d_coef, float* d_se, double* d_t, double* d_resid,
+                    double* d_sse, double* d_sumy, double* d_sumyy, hipStream_t stream) {
+  if (!d_Y || !X) return set_error(TFCE_EINVAL, "NULL argument");
+  if (k < 1 || k > kOlsMaxK || n <= k || n > kOlsDirectMaxN || V < 1 || ldy < V)
+    return set_error(TFCE_EINVAL, "need 1 <= k <= 8, k < n <= 2048, V >= 1, ldy >= V");
+  // inv(X'X) for the coefficients, as cy_lin_lstsqr_mat does (cynumstats.pyx:28-29)
+  std::vector<double> G(k * k, 0.0), Li, Ainv(k * k, 0.0);
+  for (int a = 0; a < k; ++a)
+    for (int b = 0; b < k; ++b) {
+      double s = 0.0;
+      for (int i = 0; i < n; ++i) s += X[i * k + a] * X[i * k + b];
+      G[a * k + b] = s;
+    }
+  if (!chol_inv(G, k, Li)) return set_error(TFCE_EINVAL, "X'X is singular");
+  for (int a = 0; a < k; ++a)
+    for (int b = 0; b < k; ++b) {
+      double s = 0.0;
+      for (int q = std::max(a, b); q < k; ++q) s += Li[q * k + a] * Li[q * k + b];
+      Ainv[a * k + b] = s;
+    }
+  double *dX = nullptr, *dA = nullptr, *dS = nullptr;
+  int rc = 0;
+  if ((rc = dalloc(&dX, (size_t)n * k, "X")) || (rc = dalloc(&dA, (size_t)k * k, "Ainv")) ||
+      (rc = dalloc(&dS, (size_t)k * k, "Sinv")) || (rc = upload(dX, X, (size_t)n * k, "X")) ||
+      (rc = upload(dA, Ainv.data(), (size_t)k * k, "Ainv")) ||
+      (rc = upload(dS, invXX_se ? invXX_se : Ainv.data(), (size_t)k * k, "Sinv"))) {
+    for (void* p : {(void*)dX, (void*)dA, (void*)dS}) if (p) (void)hipFree(p);
+    return rc;
+  }
+  OlsDirectArgs a{};
+  a.Y = d_Y;
+  a.n = n;
+  a.k = k;
+  a.V = V;
+  a.ldy = ldy;
+  a.X = dX;
+  a.Ainv = dA;
+  a.Sinv = dS;
+  a.coef = d_coef;
+  a.se = d_se;
+  a.t = d_t;
+  a.resid = d_resid;
+  a.sse = d_sse;
+  a.sumy = d_sumy;
+  a.sumyy = d_sumyy;
+  hipError_t e = launch_ols_direct(a, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  for (void* p : {(void*)dX, (void*)dA, (void*)dS}) (void)hipFree(p);
+  if (e != hipSuccess) return set_error(-(int)e, "ols direct kernel");
+  return 0;
+}
+
+int tfce_se_of_slope(const double* d_sigma2, const double* invXX, int32_t k, int64_t V, float* d_se,
+                     hipStream_t stream) {
+  if (!d_sigma2 || !invXX || !d_se) return set_error(TFCE_EINVAL, "NULL argument");
+  if (k < 1 || k > kOlsMaxK) return set_error(TFCE_EINVAL, "k must be in [1, 8]");
+  double* dd = nullptr;
+  std::vector<double> diag(k);
+  for (int j = 0; j < k; ++j) diag[j] = invXX[j * k + j];
+  int rc = dalloc(&dd, (size_t)k, "diag");
+  if (!rc) rc = upload(dd, diag.data(), (size_t)k, "diag");
+  if (rc) { if (dd) (void)hipFree(dd); return rc; }
+  hipError_t e = launch_se_of_slope(d_sigma2, dd, k, V, d_se, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  (void)hipFree(dd);
+  if (e != hipSuccess) return set_error(-(int)e, "se_of_slope kernel");
+  return 0;
+}
+
+int tfce_sobel_z(const float* d_ta, const float* d_tb, int64_t n, int32_t alg, float ta_const, float* d_z,
+                 hipStream_t stream) {
+  if (!d_tb || !d_z) return set_error(TFCE_EINVAL, "NULL argument");
+  if (alg < 0 || alg > 2) return set_error(TFCE_EINVAL, "alg must be 0 (aroian), 1 (sobel) or 2 (goodman)");
+  hipError_t e = launch_sobel(d_ta, d_tb, n, alg, ta_const, d_z, stream);
+  if (e != hipSuccess) return set_error(-(int)e, "sobel launch");
+  return 0;
+}
+
 int tfce_ols_info(const tfce_ols_model_t* md, int32_t* k, int32_t* n_fixed, int32_t* shared_gram) {
   if (!md) return set_error(TFCE_EINVAL, "NULL model");
   if (k) *k = md->k;

@@ -232,7 +232,7 @@ static int run_batch(const SurfDev* d_surfs, int32_t n_surf, int32_t v_max, int6
                      const float* d_vdensity, bool full, float* d_enhn, float min_t, float dh,
                      int32_t labels_level, int32_t* d_labels, float* d_scaled, float* d_raw,
                      float* d_own, int32_t* d_status, hipStream_t stream, int out_mul = 1,
-                     int out_add = 0) {
+                     int out_add = 0, uint32_t batch_flags = 0) {
   if (P <= 0) return 0;
   if (sign_mask < 1 || sign_mask > 3) return set_error(TFCE_EINVAL, "sign_mask must be 1, 2 or 3");
   const int n_signs = (sign_mask == 3) ? 2 : 1;
@@ -244,7 +244,8 @@ static int run_batch(const SurfDev* d_surfs, int32_t n_surf, int32_t v_max, int6
   const size_t prep_bytes = (size_t)n_items * kPrepWords * sizeof(int32_t);
   // raw_max is mandatory for the kernel: park it in scratch when not wanted
   const size_t raw_bytes = (size_t)n_items * sizeof(float);
-  int rc = workspace_get(stream, slots_bytes + prep_bytes + raw_bytes + 256, &ws);
+  const size_t t0_bytes = (batch_flags & TFCE_BATCH_SHARED_THRESHOLDS) ? (size_t)n_items * sizeof(float) : 0;
+  int rc = workspace_get(stream, slots_bytes + prep_bytes + raw_bytes + t0_bytes + 512, &ws);
   if (rc) return rc;
   char* wbase = reinterpret_cast<char*>(ws->ptr);
   TfceBatchArgs a{};
@@ -275,6 +276,12 @@ static int run_batch(const SurfDev* d_surfs, int32_t n_surf, int32_t v_max, int6
   a.n_items = n_items;
   a.prep = reinterpret_cast<int32_t*>(wbase + slots_bytes);
   a.raw_max = d_raw ? d_raw : reinterpret_cast<float*>(wbase + slots_bytes + prep_bytes);
+  if (batch_flags & TFCE_BATCH_SHARED_THRESHOLDS) {
+    float* T0 = reinterpret_cast<float*>(wbase + slots_bytes + prep_bytes + ((raw_bytes + 255) / 256) * 256);
+    hipError_t e0 = launch_shared_T0(a, T0, (batch_flags & TFCE_BATCH_ZERO_IN_IMAGE) ? 1 : 0, stream);
+    if (e0 != hipSuccess) return set_error(-(int)e0, "shared threshold launch");
+    a.ext_T0 = T0;
+  }
   hipError_t e = launch_tfce_batch(a, full, grid, stream);
   if (e != hipSuccess) return set_error(-(int)e, std::string("tfce launch: ") + hipGetErrorString(e));
   return 0;
@@ -356,6 +363,17 @@ int tfce_surfset_destroy(tfce_surfset_t* s) {
   (void)hipFree(s->d_surfs);
   delete s;
   return 0;
+}
+
+int tfce_batch_ex(const tfce_surfset_t* s, const float* d_maps, int64_t map_stride, int P, int sign_mask,
+                  uint32_t flags, const float* d_vdensity, float* d_scaled, float* d_raw, float* d_own,
+                  int32_t* d_status, hipStream_t stream) {
+  if (!s || !d_maps || !d_scaled) return set_error(TFCE_EINVAL, "NULL argument");
+  if (flags & ~(uint32_t)(TFCE_BATCH_SHARED_THRESHOLDS | TFCE_BATCH_ZERO_IN_IMAGE))
+    return set_error(TFCE_EINVAL, "unknown batch flag");
+  return run_batch(s->d_surfs, s->n, s->v_max, s->e_max, d_maps, map_stride, P, sign_mask, nullptr, d_vdensity,
+                   d_vdensity != nullptr, nullptr, 0.0f, 0.0f, -1, nullptr, d_scaled, d_raw, d_own, d_status,
+                   stream, 1, 0, flags);
 }
 
 int tfce_surfset_order(const tfce_surfset_t* s, int32_t* user_index) {

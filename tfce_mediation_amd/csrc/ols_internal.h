@@ -94,6 +94,29 @@ struct OlsGemmArgs {
   int32_t n_mtiles, n_ntiles;
 };
 
+constexpr int kOlsDirectMaxN = 2048;  // subjects held in LDS by the FP64 kernel
+
+struct OlsDirectArgs {
+  const float* Y;        // [n][ldy]
+  int32_t n, k;
+  int64_t V, ldy;
+  const double* X;       // [n][k] (device)
+  const double* Ainv;    // [k][k] inverse used for the coefficients
+  const double* Sinv;    // [k][k] inverse whose diagonal scales se
+  double* coef;          // [k][V] or null
+  float* se;             // [k][V] or null
+  double* t;             // [k][V] or null
+  double* resid;         // [n][V] or null
+  double* sse;           // [V] or null
+  double* sumy;          // [V] or null
+  double* sumyy;         // [V] or null
+};
+
+hipError_t launch_ols_direct(const OlsDirectArgs& a, hipStream_t s);
+hipError_t launch_se_of_slope(const double* sigma2, const double* diag, int k, int64_t V, float* se,
+                              hipStream_t s);
+hipError_t launch_sobel(const float* ta, const float* tb, int64_t n, int alg, float ta_const, float* z,
+                        hipStream_t s);
 hipError_t launch_ols_prep(const OlsPrepArgs& a, hipStream_t s);
 hipError_t launch_ols_fixed(const OlsFixedArgs& a, hipStream_t s);
 hipError_t launch_ols_factors(const OlsFactorArgs& a, hipStream_t s);

@@ -439,3 +439,111 @@ hipError_t launch_ols_gemm(const OlsGemmArgs& a, int precision, hipStream_t s) {
 }
 
 }  // namespace tfcehip
+
+// ---------------------------------------------------------------------------
+// FP64 single-design OLS, one thread per element (the reference's numpy path
+// for one design: cynumstats.pyx:28-112).  a = invXX_a X'y, residuals, SSE,
+// se_j = fl32(sqrt(SSE/(n-k) invXX_se[j][j])), t = a / se.
+namespace tfcehip {
+namespace {
+
+__global__ __launch_bounds__(256) void ols_direct_kernel(OlsDirectArgs a) {
+  __shared__ double sX[kOlsDirectMaxN * kOlsMaxK];
+  for (int i = threadIdx.x; i < a.n * a.k; i += blockDim.x) sX[i] = a.X[i];
+  __syncthreads();
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= a.V) return;
+  const int n = a.n, k = a.k;
+  double xty[kOlsMaxK], coef[kOlsMaxK];
+#pragma unroll
+  for (int j = 0; j < kOlsMaxK; ++j) xty[j] = 0.0;
+  double sy = 0.0, syy = 0.0;
+  for (int i = 0; i < n; ++i) {
+    const double y = (double)a.Y[(int64_t)i * a.ldy + v];
+    sy += y;
+    syy += y * y;
+#pragma unroll
+    for (int j = 0; j < kOlsMaxK; ++j)
+      if (j < k) xty[j] = fma(sX[i * k + j], y, xty[j]);
+  }
+#pragma unroll
+  for (int j = 0; j < kOlsMaxK; ++j) {
+    double s = 0.0;
+    if (j < k)
+      for (int q = 0; q < k; ++q) s = fma(a.Ainv[j * k + q], xty[q], s);
+    coef[j] = s;
+  }
+  double sse = 0.0;
+  for (int i = 0; i < n; ++i) {
+    const double y = (double)a.Y[(int64_t)i * a.ldy + v];
+    double fit = 0.0;
+#pragma unroll
+    for (int j = 0; j < kOlsMaxK; ++j)
+      if (j < k) fit = fma(sX[i * k + j], coef[j], fit);
+    const double r = y - fit;
+    sse = fma(r, r, sse);
+    if (a.resid) a.resid[(int64_t)i * a.V + v] = r;
+  }
+  const double s2 = sse / (double)(n - k);
+#pragma unroll
+  for (int j = 0; j < kOlsMaxK; ++j) {
+    if (j < k) {
+      const float se = (float)sqrt(s2 * a.Sinv[j * k + j]);
+      if (a.coef) a.coef[(int64_t)j * a.V + v] = coef[j];
+      if (a.se) a.se[(int64_t)j * a.V + v] = se;
+      if (a.t) a.t[(int64_t)j * a.V + v] = coef[j] / (double)se;
+    }
+  }
+  if (a.sse) a.sse[v] = sse;
+  if (a.sumy) a.sumy[v] = sy;
+  if (a.sumyy) a.sumyy[v] = syy;
+}
+
+// Sobel-type z of the indirect effect from the two path t-values
+// (pyfunc.py:150-161, calc_indirect :2678-2709): 0 aroian, 1 sobel, 2 goodman.
+__global__ void sobel_kernel(const float* ta, const float* tb, int64_t n, int alg, float ta_const, float* z) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double a = ta ? (double)ta[i] : (double)ta_const;
+  const double b = (double)tb[i];
+  const double ia = 1.0 / (a * a), ib = 1.0 / (b * b), iab = 1.0 / (a * a * b * b);
+  double d;
+  if (alg == 0) d = ib + ia + iab;
+  else if (alg == 1) d = ib + ia;
+  else d = ib + ia - iab;
+  z[i] = (float)(1.0 / sqrt(d));
+}
+
+}  // namespace
+
+hipError_t launch_ols_direct(const OlsDirectArgs& a, hipStream_t s) {
+  const int64_t blocks = (a.V + 255) / 256;
+  hipLaunchKernelGGL(ols_direct_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_sobel(const float* ta, const float* tb, int64_t n, int alg, float ta_const, float* z,
+                        hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(sobel_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ta, tb, n, alg, ta_const, z);
+  return hipGetLastError();
+}
+
+}  // namespace tfcehip
+
+namespace tfcehip {
+namespace {
+// se[j][v] = fl32(sqrt(sigma2[v] * invXX[j][j]))  (cynumstats.pyx:47-52)
+__global__ void se_of_slope_kernel(const double* sigma2, const double* diag, int k, int64_t V, float* se) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= V) return;
+  for (int j = 0; j < k; ++j) se[(int64_t)j * V + v] = (float)sqrt(sigma2[v] * diag[j]);
+}
+}  // namespace
+hipError_t launch_se_of_slope(const double* sigma2, const double* diag, int k, int64_t V, float* se,
+                              hipStream_t s) {
+  if (V <= 0) return hipSuccess;
+  hipLaunchKernelGGL(se_of_slope_kernel, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, s, sigma2, diag, k, V, se);
+  return hipGetLastError();
+}
+}  // namespace tfcehip

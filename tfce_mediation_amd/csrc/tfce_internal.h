@@ -71,6 +71,7 @@ constexpr int kPrepWords = kPrepT + 2 * kMaxLevels;
 
 hipError_t launch_tfce_batch(const TfceBatchArgs& a, bool full, int grid, hipStream_t stream);
 hipError_t launch_sizepow(double* out, int32_t n, float E, hipStream_t stream);
+hipError_t launch_shared_T0(const TfceBatchArgs& a, float* T0, int zero_in_image, hipStream_t stream);
 hipError_t launch_fill_f32(float* p, int64_t n, float v, hipStream_t stream);
 
 // Slot layout: 8 arrays of the largest surface's element count.

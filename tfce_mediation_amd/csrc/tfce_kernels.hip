@@ -484,6 +484,26 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
   }
 }
 
+// Shared first threshold per (map, sign): the maximum over every surface of the
+// set, as when the surfaces are TFCE'd as one merged image (tm_func.py:54-123,
+// fast_tfce.hpp:34-35 then sees the global maximum).  One workgroup per
+// (map, sign); writes the value to every surface's item.
+__global__ __launch_bounds__(kTfceThreads) void shared_T0_kernel(TfceBatchArgs a, float* T0, int zero_in_image) {
+  __shared__ float s_red[kTfceThreads / 64];
+  const int p = blockIdx.x / a.n_signs, sg = blockIdx.x % a.n_signs;
+  const float sign = (a.n_signs == 2) ? (sg ? -1.0f : 1.0f) : a.sign0;
+  float m = -INFINITY;
+  for (int s = 0; s < a.n_surf; ++s) {
+    const SurfDev S = a.surfs[s];
+    const float* x = a.maps + (int64_t)p * a.map_stride + S.offset;
+    for (int v = threadIdx.x; v < S.V; v += kTfceThreads) m = fmaxf(m, sign * x[v]);
+  }
+  m = block_max(m, s_red);
+  if (zero_in_image) m = fmaxf(m, 0.0f);
+  for (int s = threadIdx.x; s < a.n_surf; s += kTfceThreads)
+    T0[((int64_t)s * a.n_maps + p) * a.n_signs + sg] = m;
+}
+
 __global__ void sizepow_kernel(double* out, int32_t n, double E) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i > n) return;
@@ -508,6 +528,12 @@ hipError_t launch_tfce_batch(const TfceBatchArgs& a, bool full, int grid, hipStr
     hipLaunchKernelGGL(tfce_batch_kernel<true>, dim3(grid), dim3(kTfceThreads), 0, stream, a);
   else
     hipLaunchKernelGGL(tfce_batch_kernel<false>, dim3(grid), dim3(kTfceThreads), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_shared_T0(const TfceBatchArgs& a, float* T0, int zero_in_image, hipStream_t stream) {
+  hipLaunchKernelGGL(shared_T0_kernel, dim3((unsigned)(a.n_maps * a.n_signs)), dim3(kTfceThreads), 0, stream, a, T0,
+                     zero_in_image);
   return hipGetLastError();
 }
 

@@ -1,0 +1,128 @@
+"""Drop-in for the permutation-path subset of ``tfce_mediation.pyfunc``.
+
+* ``create_adjac_vertex`` / ``create_adjac_voxel``   pyfunc.py:37-76
+* ``write_perm_maxTFCE_vertex`` / ``_voxel``          pyfunc.py:107-126
+* ``calc_sobelz`` / ``calc_indirect``                 pyfunc.py:130-162, 2678-2709
+* ``find_nearest``                                    pyfunc.py:254-261
+* ``check_blocks`` / ``rand_blocks``                  pyfunc.py:2711-2757
+The TFCE and statistics run on the GPU; the null writers append to the same
+files in the same text format as the reference, with Python file I/O instead
+of one ``os.system('echo ...')`` per value.
+"""
+import math
+
+import numpy as np
+
+from . import _lib
+from .adjacency import create_adjac_vertex, create_adjac_voxel  # noqa: F401
+from .cynumstats import calc_beta_se
+
+
+def _tfce_max_scaled(calc, stat_full, vdensity):
+    """max over elements of TFCE(stat) * (max(stat)/100) * vdensity (one map)."""
+    import torch
+    adj = calc.adjacency
+    d = torch.as_tensor(np.ascontiguousarray(stat_full, dtype=np.float32)).cuda()[None, :]
+    vd = None
+    if not np.isscalar(vdensity) and np.asarray(vdensity).ndim > 0:
+        vd = torch.as_tensor(np.ascontiguousarray(vdensity, dtype=np.float32)).cuda()
+    out = adj.batch_max(d, 1, vdensity=vd).cpu().numpy()[0, 0]
+    if vd is None and float(vdensity) != 1.0:
+        out = np.float32(out) * vdensity
+    return out
+
+
+def write_perm_maxTFCE_vertex(statname, vertStat, num_vertex, bin_mask_lh, bin_mask_rh, calcTFCE_lh,
+                              calcTFCE_rh, density_corr_lh=1, density_corr_rh=1):
+    vertStat = np.asarray(vertStat)
+    lh = np.zeros(bin_mask_lh.shape[0], dtype=np.float32)
+    rh = np.zeros(bin_mask_rh.shape[0], dtype=np.float32)
+    lh[bin_mask_lh] = vertStat[:num_vertex]
+    rh[bin_mask_rh] = vertStat[num_vertex:]
+    m_lh = _tfce_max_scaled(calcTFCE_lh, lh, density_corr_lh)
+    m_rh = _tfce_max_scaled(calcTFCE_rh, rh, density_corr_rh)
+    maxTFCE = np.array([m_lh, m_rh]).max()
+    with open("perm_%s_TFCE_maxVertex.csv" % statname, "a") as f:
+        f.write("%.4f\n" % maxTFCE)
+    return maxTFCE
+
+
+def write_perm_maxTFCE_voxel(statname, voxelStat, TFCEfunc):
+    v = np.asarray(voxelStat).astype(np.float32, order="C")
+    maxval = _tfce_max_scaled(TFCEfunc, v, 1)
+    with open("perm_%s_TFCE_maxVoxel.csv" % statname, "a") as f:
+        f.write("%1.4f\n" % maxval)
+    return maxval
+
+
+def calc_indirect(ta, tb, alg="aroian"):
+    """Sobel-type z from path t-values, on the GPU (float32 result)."""
+    import torch
+    code = {"aroian": 0, "sobel": 1, "goodman": 2}.get(alg)
+    if code is None:
+        raise ValueError("Unknown indirect test algorithm")
+    tb_t = torch.as_tensor(np.ascontiguousarray(np.atleast_1d(tb), dtype=np.float32)).cuda()
+    ta_arr = np.atleast_1d(ta)
+    ta_t = None
+    ta_c = 0.0
+    if ta_arr.size == 1 and tb_t.numel() != 1:
+        ta_c = float(ta_arr[0])
+    else:
+        ta_t = torch.as_tensor(np.ascontiguousarray(ta_arr, dtype=np.float32)).cuda()
+    z = torch.empty_like(tb_t)
+    _lib.check(_lib.lib().tfce_sobel_z(_lib.ptr(ta_t), _lib.ptr(tb_t), tb_t.numel(), code, ta_c, _lib.ptr(z),
+                                       _lib.stream_ptr()))
+    return z.cpu().numpy()
+
+
+def calc_sobelz(medtype, pred_x, depend_y, merge_y, n, num_vertex, alg="aroian"):
+    if medtype == "I":
+        a_b, a_se = calc_beta_se(pred_x, merge_y, n, num_vertex)
+        b_b, b_se = calc_beta_se(np.column_stack([pred_x, depend_y]), merge_y, n, num_vertex)
+        ta, tb = a_b / a_se[1], b_b / b_se[1]
+    elif medtype == "M":
+        a_b, a_se = calc_beta_se(pred_x, merge_y, n, num_vertex)
+        b_b, b_se = calc_beta_se(np.column_stack([depend_y, pred_x]), merge_y, n, num_vertex)
+        ta, tb = a_b / a_se[1], b_b / b_se[1]
+    elif medtype == "Y":
+        from scipy.stats import linregress
+        lr = linregress(pred_x, depend_y)
+        ta = lr.slope / lr.stderr
+        b_b, b_se = calc_beta_se(np.column_stack([depend_y, pred_x]), merge_y, n, num_vertex)
+        tb = b_b / b_se[1]
+    else:
+        raise ValueError("Invalid mediation type")
+    return calc_indirect(ta, tb, alg).astype(np.float64)
+
+
+def find_nearest(array, value, p_array):
+    idx = np.searchsorted(array, value, side="left")
+    if idx == len(p_array):
+        return p_array[idx - 1]
+    elif math.fabs(value - array[idx - 1]) < math.fabs(value - array[idx]):
+        return p_array[idx - 1]
+    else:
+        return p_array[idx]
+
+
+def check_blocks(block_list):
+    unique_blocks = np.unique(block_list)
+    sizes = [len(block_list[block_list == b]) for b in unique_blocks]
+    equal = all(x == sizes[0] for x in sizes)
+    if not equal:
+        print("Warning: blocks are not equal. Swaping with only occur within blocks, but not among blocks.")
+    return equal
+
+
+def rand_blocks(block_list, is_equal_sizes):
+    """Permutation index array within (and, for equal sizes, among) blocks,
+    drawn from numpy's global RNG exactly as pyfunc.py:2738-2757."""
+    indexer = np.array(range(len(block_list)))
+    randindex = []
+    if is_equal_sizes is True:
+        for block in np.random.permutation(list(np.unique(block_list))):
+            randindex.append(np.random.permutation(indexer[block_list == block]))
+    else:
+        for block in np.unique(block_list):
+            randindex.append(np.random.permutation(indexer[block_list == block]))
+    return np.concatenate(randindex)

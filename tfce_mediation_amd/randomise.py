@@ -1,0 +1,245 @@
+"""Permutation drivers: the step-2 randomise loops of the reference, batched.
+
+Each driver produces the same max-TFCE null the reference's per-permutation
+loop appends to its CSV files, for a whole range of permutations at once on
+the GPU (permuted t on MFMA, one persistent TFCE launch per batch):
+
+* ``vertex_regress``   vertex_tfce_multiple_regression_randomise.py:56-118
+  (two hemispheres, bin masks, vdensity, every contrast j = 1..k-1, +t then -t,
+  ``perm_tstat_con<j>_TFCE_maxVertex.csv`` in "%.4f");
+* ``voxel_regress``    voxel_tfce_multiple_regression_randomise.py:51-120
+  (``perm_tstat_con<j>_TFCE_maxVoxel.csv`` in "%1.4f");
+* ``vertex_mediation`` vertex_tfce_mediation_randomise.py:48-92 (Sobel z,
+  positive pass only, ``perm_Zstat_<type>_TFCE_maxVertex.csv``);
+* ``mmr``              tm_func.calculate_tfce in a loop (the mmr randomise of
+  tm_multimodality_multisurface_regression.py:523-572): per-surface nulls
+  ``perm_maxTFCE_surf<s>_tcon<c>.csv`` in "%f"; ``merged=True`` shares the
+  threshold sequence over the surfaces of a settings group as the merged
+  adjacency does, ``merged=False`` gives mmr-lr's per-surface thresholds.
+
+Permutation indices: ``permutation_indices`` draws them reproducibly from
+(seed, permutation number) with numpy's legacy RNG, the scheme of
+tm_func.py:147-149 (the reference's step-2 scripts seed from the clock,
+vertex_tfce_multiple_regression_randomise.py:91, which cannot be reproduced).
+Exchange blocks (``-e``) use pyfunc.rand_blocks' rule.
+"""
+import os
+
+import numpy as np
+
+from . import synth
+from .adjacency import induced_subgraph, to_csr
+from .ols import PermutedOLS
+from .tfce import Adjacency, SurfaceSet, TFCE_ADJ_IMPLICIT_ZERO, TFCE_ADJ_ZERO_IN_SCALE
+
+
+def permutation_indices(n, perm_numbers, seed=0, blocks=None):
+    """int32 [len(perm_numbers), n]: np.random.seed(p + seed) then a
+    permutation (or the exchange-block permutation of pyfunc.py:2738-2757)."""
+    from .pyfunc import check_blocks, rand_blocks
+    out = np.zeros((len(perm_numbers), n), dtype=np.int32)
+    equal = check_blocks(np.asarray(blocks)) if blocks is not None else None
+    for i, p in enumerate(perm_numbers):
+        np.random.seed(int(p) + int(seed))
+        if blocks is None:
+            out[i] = np.random.permutation(list(range(n)))
+        else:
+            out[i] = rand_blocks(np.asarray(blocks), equal)
+    return out
+
+
+def _hemi_surface(adjacency, bin_mask, H, E):
+    rp, col = to_csr(adjacency)
+    bin_mask = np.asarray(bin_mask, dtype=bool)
+    srp, scol = induced_subgraph(rp, col, bin_mask)
+    flags = (TFCE_ADJ_IMPLICIT_ZERO | TFCE_ADJ_ZERO_IN_SCALE) if not bin_mask.all() else 0
+    return Adjacency((srp, scol), H, E, flags)
+
+
+def _batches(perms, batch):
+    import torch
+    for a in range(0, len(perms), batch):
+        yield a, torch.as_tensor(np.ascontiguousarray(perms[a:a + batch], dtype=np.int32)).cuda()
+
+
+def _write(path, values, fmt):
+    with open(path, "a") as f:
+        for v in np.asarray(values, dtype=np.float64).ravel():
+            f.write((fmt % v) + "\n")
+
+
+def vertex_regress(ny, pred_x, adjac_lh, adjac_rh, bin_mask_lh, bin_mask_rh, optstfce, perms,
+                   vdensity_lh=1, vdensity_rh=1, specifyvars=None, out_dir=None, batch=256,
+                   precision="bf16x3"):
+    """Null rows [n_perm, 2] (+t, -t) per contrast, dict {tnum: array}; written
+    to perm_tstat_con<tnum>_TFCE_maxVertex.csv when out_dir is given."""
+    import torch
+    n = ny.shape[0]
+    X = np.column_stack([np.ones(n), pred_x])
+    k = X.shape[1]
+    if specifyvars is not None:
+        start, stop = specifyvars[0], specifyvars[1] + 1
+        perm_cols = list(range(start, stop))
+        contrasts = [j + 1 for j in range(stop - start)]  # tnum = j + 1 (randomise.py:110-113)
+    else:
+        perm_cols = None
+        contrasts = list(range(1, k))
+    H, E = float(optstfce[0]), float(optstfce[1])
+    lh = _hemi_surface(adjac_lh, bin_mask_lh, H, E)
+    rh = _hemi_surface(adjac_rh, bin_mask_rh, H, E)
+    nv_lh = int(np.asarray(bin_mask_lh, bool).sum())
+    surfs = SurfaceSet([lh, rh], [0, nv_lh])
+    vd = None
+    if not np.isscalar(vdensity_lh) or not np.isscalar(vdensity_rh):
+        vl = np.broadcast_to(np.asarray(vdensity_lh, np.float32), (len(bin_mask_lh),))[np.asarray(bin_mask_lh, bool)]
+        vr = np.broadcast_to(np.asarray(vdensity_rh, np.float32), (len(bin_mask_rh),))[np.asarray(bin_mask_rh, bool)]
+        vd = torch.as_tensor(np.ascontiguousarray(np.concatenate([vl, vr]))).cuda()
+    model = PermutedOLS(ny, X, contrasts=contrasts, perm_columns=perm_cols, precision=precision)
+    nulls = {c: [] for c in contrasts}
+    for _, pb in _batches(perms, batch):
+        t = model.t(pb)
+        for ci, c in enumerate(contrasts):
+            nulls[c].append(surfs.batch(t[ci], 3, vdensity=vd).max(0).values.cpu().numpy())
+    nulls = {c: np.concatenate(v, 0) for c, v in nulls.items()}
+    if out_dir is not None:
+        for c, v in nulls.items():
+            _write(os.path.join(out_dir, "perm_tstat_con%d_TFCE_maxVertex.csv" % c), v, "%.4f")
+    return nulls
+
+
+def voxel_regress(ny, pred_x, adjac, optstfce, perms, specifyvars=None, out_dir=None, batch=256,
+                  precision="bf16x3"):
+    """Voxel randomise: null rows [n_perm, 2] per contrast."""
+    n = ny.shape[0]
+    X = np.column_stack([np.ones(n), pred_x])
+    k = X.shape[1]
+    if specifyvars is not None:
+        start, stop = specifyvars[0], specifyvars[1] + 1
+        perm_cols, contrasts = list(range(start, stop)), [j + 1 for j in range(stop - start)]
+    else:
+        perm_cols, contrasts = None, list(range(1, k))
+    adj = Adjacency(adjac, float(optstfce[0]), float(optstfce[1]))
+    surfs = SurfaceSet([adj], [0])
+    model = PermutedOLS(ny, X, contrasts=contrasts, perm_columns=perm_cols, precision=precision)
+    nulls = {c: [] for c in contrasts}
+    for _, pb in _batches(perms, batch):
+        t = model.t(pb)
+        t = t.nan_to_num(nan=0.0)  # perm_tvalues[np.isnan(perm_tvalues)] = 0 (randomise.py:109)
+        for ci, c in enumerate(contrasts):
+            nulls[c].append(surfs.batch(t[ci], 3)[0].cpu().numpy())
+    nulls = {c: np.concatenate(v, 0) for c, v in nulls.items()}
+    if out_dir is not None:
+        for c, v in nulls.items():
+            _write(os.path.join(out_dir, "perm_tstat_con%d_TFCE_maxVoxel.csv" % c), v, "%1.4f")
+    return nulls
+
+
+class MediationModel:
+    """Batched permuted Sobel z (calc_sobelz, pyfunc.py:130-162) for one image
+    set: path A is y_img ~ [1, x]; path B is y_img ~ [1, y, x] (type M, the
+    mediator t of y), [1, x, y] (type I, t of x) or type Y with path A the
+    scalar x -> y regression.  Types M and I permute x only, type Y permutes
+    x and y together (vertex_tfce_mediation_randomise.py:82-90)."""
+
+    def __init__(self, M, pred_x, depend_y, medtype="M", alg="aroian", precision="bf16x3"):
+        import torch
+        x = np.asarray(pred_x, dtype=np.float64)
+        y = np.asarray(depend_y, dtype=np.float64)
+        n = x.shape[0]
+        self.medtype, self.alg = medtype, {"aroian": 0, "sobel": 1, "goodman": 2}[alg]
+        one = np.ones(n)
+        self.ta_const = None
+        if medtype in ("M", "I"):
+            self.A = PermutedOLS(M, np.column_stack([one, x]), contrasts=[1], perm_columns=[1], precision=precision)
+            if medtype == "M":
+                XB, cb, pc = np.column_stack([one, y, x]), 1, [2]
+            else:
+                XB, cb, pc = np.column_stack([one, x, y]), 1, [1]
+            self.B = PermutedOLS(M, XB, contrasts=[cb], perm_columns=pc, precision=precision)
+        elif medtype == "Y":
+            from .cynumstats import tval_int
+            Xa = np.column_stack([one, x])
+            self.ta_const = float(tval_int(Xa, np.linalg.inv(Xa.T @ Xa), y[:, None].astype(np.float32), n, 2, 1)[1, 0])
+            self.A = None
+            self.B = PermutedOLS(M, np.column_stack([one, y, x]), contrasts=[1], perm_columns=[1, 2],
+                                 precision=precision)
+        else:
+            raise ValueError("Invalid mediation type")
+        self.V = self.B.V
+
+    def z(self, perms=None):
+        import torch
+        from . import _lib
+        tb = self.B.t(perms)[0]
+        ta = self.A.t(perms)[0] if self.A is not None else None
+        z = torch.empty_like(tb)
+        _lib.check(_lib.lib().tfce_sobel_z(_lib.ptr(ta), _lib.ptr(tb), tb.numel(), self.alg,
+                                           float(self.ta_const or 0.0), _lib.ptr(z), _lib.stream_ptr()))
+        return z
+
+
+def vertex_mediation(y, pred_x, depend_y, medtype, adjac_lh, adjac_rh, bin_mask_lh, bin_mask_rh, optstfce, perms,
+                     vdensity_lh=1, vdensity_rh=1, out_dir=None, batch=256, precision="bf16x3"):
+    """Mediation randomise: null [n_perm] of the positive Sobel-z TFCE maxima."""
+    import torch
+    H, E = float(optstfce[0]), float(optstfce[1])
+    lh = _hemi_surface(adjac_lh, bin_mask_lh, H, E)
+    rh = _hemi_surface(adjac_rh, bin_mask_rh, H, E)
+    nv_lh = int(np.asarray(bin_mask_lh, bool).sum())
+    surfs = SurfaceSet([lh, rh], [0, nv_lh])
+    vd = None
+    if not np.isscalar(vdensity_lh) or not np.isscalar(vdensity_rh):
+        vl = np.broadcast_to(np.asarray(vdensity_lh, np.float32), (len(bin_mask_lh),))[np.asarray(bin_mask_lh, bool)]
+        vr = np.broadcast_to(np.asarray(vdensity_rh, np.float32), (len(bin_mask_rh),))[np.asarray(bin_mask_rh, bool)]
+        vd = torch.as_tensor(np.ascontiguousarray(np.concatenate([vl, vr]))).cuda()
+    med = MediationModel(y, pred_x, depend_y, medtype, precision=precision)
+    out = []
+    for _, pb in _batches(perms, batch):
+        z = med.z(pb)
+        out.append(surfs.batch(z, 1, vdensity=vd).max(0).values[:, 0].cpu().numpy())
+    null = np.concatenate(out)
+    if out_dir is not None:
+        _write(os.path.join(out_dir, "perm_Zstat_%s_TFCE_maxVertex.csv" % medtype), null, "%.4f")
+    return null
+
+
+def mmr(merge_y, pred_x, surfaces, offsets, settings, perms, groups=None, merged=True, zero_in_image=None,
+        vdensity=None, out_dir=None, batch=256, precision="bf16x3"):
+    """Multi-surface randomise.  surfaces: CSR per surface (in-mask elements),
+    offsets: their first column in merge_y; settings: (H, E) per surface;
+    groups: settings-group id per surface (default: one per distinct setting).
+    Returns {(surface, tcon): [n_perm, 2]} in the order the reference appends
+    (+ then -) and writes perm_maxTFCE_surf<s>_tcon<c>.csv when out_dir is set."""
+    import torch
+    n = merge_y.shape[0]
+    X = np.column_stack([np.ones(n), pred_x])
+    k = X.shape[1]
+    contrasts = list(range(1, k))
+    model = PermutedOLS(merge_y, X, contrasts=contrasts, precision=precision)
+    if groups is None:
+        keys = sorted(set(map(tuple, settings)))
+        groups = [keys.index(tuple(s)) for s in settings]
+    sets = {}
+    for g in sorted(set(groups)):
+        idx = [i for i, gg in enumerate(groups) if gg == g]
+        adjs = [Adjacency(surfaces[i], settings[i][0], settings[i][1]) for i in idx]
+        sets[g] = (idx, SurfaceSet(adjs, [offsets[i] for i in idx]))
+    vd = None
+    if vdensity is not None and not np.isscalar(vdensity):
+        vd = torch.as_tensor(np.ascontiguousarray(vdensity, dtype=np.float32)).cuda()
+    zi = zero_in_image or {}
+    res = {}
+    for _, pb in _batches(perms, batch):
+        t = model.t(pb)
+        for ci, c in enumerate(contrasts):
+            for g, (idx, ss) in sets.items():
+                per = ss.batch(t[ci], 3, vdensity=vd, shared_thresholds=merged,
+                               zero_in_image=bool(zi.get(g, False)) if merged else False)
+                per = per.cpu().numpy()
+                for j, s in enumerate(idx):
+                    res.setdefault((s, c), []).append(per[j])
+    res = {key: np.concatenate(v, 0) for key, v in res.items()}
+    if out_dir is not None:
+        for (s, c), v in sorted(res.items()):
+            _write(os.path.join(out_dir, "perm_maxTFCE_surf%d_tcon%d.csv" % (s, c)), v, "%f")
+    return res

@@ -142,7 +142,7 @@ class SurfaceSet:
             self.handle = None
 
     def batch(self, maps, sign_mask=3, T0=None, vdensity=None, want_raw=False, want_own=False,
-              want_status=False, stream=None):
+              want_status=False, stream=None, shared_thresholds=False, zero_in_image=False):
         import torch
         assert maps.is_cuda and maps.dtype == torch.float32 and maps.stride(-1) == 1
         P = maps.shape[0]
@@ -159,9 +159,16 @@ class SurfaceSet:
             assert T0.is_cuda and T0.numel() == n_items
         if vdensity is not None:
             assert vdensity.is_cuda and vdensity.dtype == torch.float32 and vdensity.numel() >= self.width
-        _lib.check(_lib.lib().tfce_batch(self.handle, _lib.ptr(maps), stride, P, int(sign_mask),
-                                        _lib.ptr(T0), _lib.ptr(vdensity), _lib.ptr(scaled), _lib.ptr(raw),
-                                        _lib.ptr(own), _lib.ptr(st), _lib.stream_ptr(stream)))
+        if shared_thresholds:
+            assert T0 is None
+            flags = 1 | (2 if zero_in_image else 0)
+            _lib.check(_lib.lib().tfce_batch_ex(self.handle, _lib.ptr(maps), stride, P, int(sign_mask), flags,
+                                               _lib.ptr(vdensity), _lib.ptr(scaled), _lib.ptr(raw), _lib.ptr(own),
+                                               _lib.ptr(st), _lib.stream_ptr(stream)))
+        else:
+            _lib.check(_lib.lib().tfce_batch(self.handle, _lib.ptr(maps), stride, P, int(sign_mask),
+                                            _lib.ptr(T0), _lib.ptr(vdensity), _lib.ptr(scaled), _lib.ptr(raw),
+                                            _lib.ptr(own), _lib.ptr(st), _lib.stream_ptr(stream)))
         inv = torch.as_tensor(self._inv, device=dev, dtype=torch.long)
 
         def user(x):

@@ -192,6 +192,22 @@ int tfce_batch_ex(const tfce_surfset_t *set, const float *d_maps, int64_t map_st
 /* Number of resident workgroups per CU used by the TFCE sweep (autotuner knob). */
 int tfce_set_occupancy(int32_t wgs_per_cu);
 
+/* ---------------------------------------------------- learned early stop */
+/* No counterpart in the reference (nearest: the parametric null fits of
+ * bin/tm_maths:532-552).  Tail model 32 -> 64 -> 64 -> 2, tanh, FP32, mapping
+ * 32 standardised quantiles of the first null values to the standardised
+ * 95% / 99% critical values (tools/plot_permutations.py:81-85 definition). */
+int tfce_mlp_param_count(void);
+int tfce_mlp_forward(const float *d_params, const float *d_x, int32_t n, float *d_y, hipStream_t stream);
+/* `steps` Adam steps (beta1 0.9, beta2 0.999, eps 1e-8, step count from t0+1)
+ * on MSE over minibatches d_batch_idx[steps][batch] (batch <= 128) of the
+ * examples d_x[n][32] -> d_y[n][2]; params, adam_m, adam_v are updated in
+ * place.  Optional d_loss[steps] (minibatch loss before each update) and
+ * d_grad0 (gradient of the first step).  One workgroup; deterministic. */
+int tfce_mlp_train(float *d_params, float *d_adam_m, float *d_adam_v, const float *d_x, const float *d_y,
+                   const int32_t *d_batch_idx, int32_t steps, int32_t batch, int32_t t0, float lr,
+                   float weight_decay, float *d_loss, float *d_grad0, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,6 +11,7 @@
 
 #include "tfcehip.h"
 #include "capi_common.h"
+#include "mlp_internal.h"
 #include "ols_internal.h"
 
 using namespace tfcehip;
@@ -544,6 +545,44 @@ int tfce_sobel_z(const float* d_ta, const float* d_tb, int64_t n, int32_t alg, f
   if (alg < 0 || alg > 2) return set_error(TFCE_EINVAL, "alg must be 0 (aroian), 1 (sobel) or 2 (goodman)");
   hipError_t e = launch_sobel(d_ta, d_tb, n, alg, ta_const, d_z, stream);
   if (e != hipSuccess) return set_error(-(int)e, "sobel launch");
+  return 0;
+}
+
+int tfce_mlp_param_count(void) { return kMlpParams; }
+
+int tfce_mlp_forward(const float* d_params, const float* d_x, int32_t n, float* d_y, hipStream_t stream) {
+  if (!d_params || !d_x || !d_y) return set_error(TFCE_EINVAL, "NULL argument");
+  hipError_t e = launch_mlp_forward(d_params, d_x, n, d_y, stream);
+  if (e != hipSuccess) return set_error(-(int)e, "mlp forward launch");
+  return 0;
+}
+
+int tfce_mlp_train(float* d_params, float* d_adam_m, float* d_adam_v, const float* d_x, const float* d_y,
+                   const int32_t* d_batch_idx, int32_t steps, int32_t batch, int32_t t0, float lr,
+                   float weight_decay, float* d_loss, float* d_grad0, hipStream_t stream) {
+  if (!d_params || !d_adam_m || !d_adam_v || !d_x || !d_y || !d_batch_idx)
+    return set_error(TFCE_EINVAL, "NULL argument");
+  if (batch < 1 || batch > kMlpBatch) return set_error(TFCE_EINVAL, "batch must be in [1, 128]");
+  if (steps < 1) return set_error(TFCE_EINVAL, "steps must be >= 1");
+  MlpTrainArgs a{};
+  a.params = d_params;
+  a.adam_m = d_adam_m;
+  a.adam_v = d_adam_v;
+  a.X = d_x;
+  a.Y = d_y;
+  a.batch_idx = d_batch_idx;
+  a.steps = steps;
+  a.batch = batch;
+  a.t0 = t0;
+  a.lr = lr;
+  a.beta1 = 0.9f;
+  a.beta2 = 0.999f;
+  a.eps = 1e-8f;
+  a.weight_decay = weight_decay;
+  a.loss = d_loss;
+  a.grad0 = d_grad0;
+  hipError_t e = launch_mlp_train(a, stream);
+  if (e != hipSuccess) return set_error(-(int)e, "mlp train launch");
   return 0;
 }
 

@@ -140,6 +140,9 @@ def main():
     ap.add_argument("--cpu-perms-per-worker", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="copy a buffer of this many MiB once before the timed region, so that a "
+                         "counter run (FETCH_SIZE / WRITE_SIZE) holds a plain copy of known size")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -169,6 +172,15 @@ def main():
         eng.null_batch(perm_batches[s])
     torch.cuda.synchronize()
     log("[bench] warmup done")
+
+    if args.copy_calibration:
+        src = torch.ones(args.copy_calibration << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] calibration copy: {args.copy_calibration} MiB read, {args.copy_calibration} MiB written")
+        del src, dst
 
     # ---- timed region -------------------------------------------------------
     stream = torch.cuda.current_stream()

@@ -203,9 +203,11 @@ int tfce_mlp_forward(const float *d_params, const float *d_x, int32_t n, float *
  * on MSE over minibatches d_batch_idx[steps][batch] (batch <= 128) of the
  * examples d_x[n][32] -> d_y[n][2]; params, adam_m, adam_v are updated in
  * place.  Optional d_loss[steps] (minibatch loss before each update) and
- * d_grad0 (gradient of the first step).  One workgroup; deterministic. */
+ * d_grad0 (gradient of the first step).  One workgroup; deterministic.  A
+ * minibatch holding an index outside [0, n) is skipped (no update) and its
+ * d_loss entry is NaN. */
 int tfce_mlp_train(float *d_params, float *d_adam_m, float *d_adam_v, const float *d_x, const float *d_y,
-                   const int32_t *d_batch_idx, int32_t steps, int32_t batch, int32_t t0, float lr,
+                   int32_t n, const int32_t *d_batch_idx, int32_t steps, int32_t batch, int32_t t0, float lr,
                    float weight_decay, float *d_loss, float *d_grad0, hipStream_t stream);
 
 #ifdef __cplusplus

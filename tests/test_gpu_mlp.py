@@ -98,3 +98,22 @@ def test_learned_prediction_beats_short_empirical(torch):
     for src, e in res:
         if src == "learned":
             assert np.all(e < 0.05)
+
+
+def test_out_of_range_minibatch_is_skipped(torch):
+    """The kernel's own index check: a minibatch naming an example >= n leaves
+    the parameters untouched and reports a NaN loss for that step."""
+    from tfce_mediation_amd import _lib
+    from tfce_mediation_amd.learned_stop import TailMLP
+    say("[gpu] mlp minibatch index check")
+    model = TailMLP(seed=4)
+    before = model.numpy_params()
+    X = torch.zeros((10, 32), device="cuda")
+    Y = torch.zeros((10, 2), device="cuda")
+    idx = torch.tensor([[0, 1, 2, 10]], dtype=torch.int32, device="cuda")
+    loss = torch.zeros(1, device="cuda")
+    _lib.check(_lib.lib().tfce_mlp_train(_lib.ptr(model.params), _lib.ptr(model.m), _lib.ptr(model.v),
+                                         _lib.ptr(X), _lib.ptr(Y), 10, _lib.ptr(idx), 1, 4, 0, 1e-2, 0.0,
+                                         _lib.ptr(loss), None, _lib.stream_ptr()))
+    assert np.isnan(loss.cpu().numpy()[0])
+    assert np.array_equal(model.numpy_params(), before)

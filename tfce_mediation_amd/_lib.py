@@ -48,7 +48,7 @@ SIGNATURES = {
     "tfce_se_of_slope": (c_int, [c_p, c_p, c_i32, c_i64, c_p, c_p]),
     "tfce_mlp_param_count": (c_int, []),
     "tfce_mlp_forward": (c_int, [c_p, c_p, c_i32, c_p, c_p]),
-    "tfce_mlp_train": (c_int, [c_p, c_p, c_p, c_p, c_p, c_p, c_i32, c_i32, c_i32, c_f32, c_f32, c_p, c_p, c_p]),
+    "tfce_mlp_train": (c_int, [c_p, c_p, c_p, c_p, c_p, c_i32, c_p, c_i32, c_i32, c_i32, c_f32, c_f32, c_p, c_p, c_p]),
     "tfce_sobel_z": (c_int, [c_p, c_p, c_i64, c_i32, c_f32, c_p, c_p]),
 }
 

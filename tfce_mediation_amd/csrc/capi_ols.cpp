@@ -558,12 +558,13 @@ int tfce_mlp_forward(const float* d_params, const float* d_x, int32_t n, float* 
 }
 
 int tfce_mlp_train(float* d_params, float* d_adam_m, float* d_adam_v, const float* d_x, const float* d_y,
-                   const int32_t* d_batch_idx, int32_t steps, int32_t batch, int32_t t0, float lr,
+                   int32_t n, const int32_t* d_batch_idx, int32_t steps, int32_t batch, int32_t t0, float lr,
                    float weight_decay, float* d_loss, float* d_grad0, hipStream_t stream) {
   if (!d_params || !d_adam_m || !d_adam_v || !d_x || !d_y || !d_batch_idx)
     return set_error(TFCE_EINVAL, "NULL argument");
   if (batch < 1 || batch > kMlpBatch) return set_error(TFCE_EINVAL, "batch must be in [1, 128]");
   if (steps < 1) return set_error(TFCE_EINVAL, "steps must be >= 1");
+  if (n < 1) return set_error(TFCE_EINVAL, "n must be >= 1");
   MlpTrainArgs a{};
   a.params = d_params;
   a.adam_m = d_adam_m;
@@ -571,6 +572,7 @@ int tfce_mlp_train(float* d_params, float* d_adam_m, float* d_adam_v, const floa
   a.X = d_x;
   a.Y = d_y;
   a.batch_idx = d_batch_idx;
+  a.n = n;
   a.steps = steps;
   a.batch = batch;
   a.t0 = t0;

@@ -19,7 +19,7 @@ struct MlpTrainArgs {
   const float* X;         // [n][32]
   const float* Y;         // [n][2]
   const int32_t* batch_idx;  // [steps][batch] example indices
-  int32_t steps, batch, t0;
+  int32_t n, steps, batch, t0;
   float lr, beta1, beta2, eps, weight_decay;
   float* loss;            // [steps] minibatch loss before the update, or null
   float* grad0;           // [kMlpParams] gradient of the first step, or null

@@ -64,12 +64,23 @@ __global__ __launch_bounds__(kMlpThreads) void mlp_train_kernel(MlpTrainArgs a) 
   __shared__ float sD[kMlpBatch * kMlpOut];
   __shared__ float sG[kMlpParams];         // gradient of the current step
   __shared__ float sLoss[kMlpThreads];
+  __shared__ int sBad;
   const int tid = threadIdx.x;
   const int B = a.batch;
   for (int i = tid; i < kMlpParams; i += kMlpThreads) sP[i] = a.params[i];
   __syncthreads();
   for (int step = 0; step < a.steps; ++step) {
     const int32_t* idx = a.batch_idx + (int64_t)step * B;
+    if (tid == 0) sBad = 0;
+    __syncthreads();
+    if (tid < B && (idx[tid] < 0 || idx[tid] >= a.n)) sBad = 1;
+    __syncthreads();
+    const int bad = sBad;
+    __syncthreads();  // every thread has read the flag before it is cleared again
+    if (bad) {  // uniform across the workgroup: skip this minibatch
+      if (tid == 0 && a.loss) a.loss[step] = __builtin_nanf("");
+      continue;
+    }
     // forward, one example per thread
     float loss = 0.0f;
     if (tid < B) {

@@ -271,16 +271,74 @@ __device__ __forceinline__ void ols_epilogue(const OlsGemmArgs& a, int p, int64_
   }
 }
 
-template <int PREC>
-__global__ __launch_bounds__(256) void ols_gemm_kernel(OlsGemmArgs a) {
-  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, h = l >> 5, r = l & 31;
+// Epilogue through LDS.  The accumulators of a wave cover, for tile (rt, ct),
+// rows row0 + 32 rt + 8 (i / 4) + 4 h + (i % 4) (i = accumulator index 0..15)
+// of column 32 ct + r.  The workgroup writes one 32-column slice of every
+// wave's tiles into an LDS block ep[256 rows][ldc], then each thread owns one
+// column of the block and walks permutations: it reads the permutation's
+// rpp rows (X_p'y for the permuted columns) and runs the per-element
+// epilogue.  Unrolled register code stays small (only the LDS write is
+// unrolled), so the accumulators never leave registers for scratch.
+template <int RT, int CT>
+__device__ __forceinline__ void epi_put(float* ep, int ldc, const f32x16 (&acc)[RT][CT], int sel, int row0,
+                                        int c0, int h, int r) {
+#pragma unroll
+  for (int ct = 0; ct < CT; ++ct) {
+    if (ct != sel) continue;
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+      for (int i = 0; i < 16; ++i)
+        ep[(row0 + 32 * rt + 8 * (i >> 2) + 4 * h + (i & 3)) * ldc + c0 + r] = acc[rt][ct][i];
+  }
+}
+
+// Thread (cl, slot): column cl of the block (global element `col`),
+// permutations slot, slot + step, ... of the workgroup's row tile.
+__device__ __forceinline__ void epi_run(const OlsGemmArgs& a, const float* ep, int ldc, int cl, int64_t col,
+                                        int64_t mrow0, int slot, int step) {
+  const bool cvalid = col < a.V;
+  if (!cvalid) return;
+  float gF[kOlsMaxK], hF[kOlsMaxK];
+#pragma unroll
+  for (int q = 0; q < kOlsMaxK; ++q) {
+    gF[q] = q < a.f ? a.gF[(int64_t)q * a.V + col] : 0.0f;
+    hF[q] = q < a.f ? a.hF[(int64_t)q * a.V + col] : 0.0f;
+  }
+  const double r0 = a.r0[col], mu = a.mu[col];
+  const int ppt = kGemmRows / a.rpp;
+  const int p0 = (int)(mrow0 / a.rpp);
+  for (int pp = slot; pp < ppt; pp += step) {
+    const int p = p0 + pp;
+    if (p >= a.P) break;
+    float gz[kOlsMaxK];
+#pragma unroll
+    for (int q = 0; q < kOlsMaxK; ++q) {
+      gz[q] = q < a.m ? ep[(pp * a.rpp + q) * ldc + cl] : 0.0f;
+      if (!a.centered && q < a.m) gz[q] += (float)(a.colsumZ[q] * mu);
+    }
+    ols_epilogue(a, p, col, gz, gF, hF, r0, mu);
+  }
+}
+
+// Block -> tile mapping shared by both contraction kernels: the n_mtiles row
+// tiles of one column tile are consecutive blocks of one XCD (blockIdx mod 8),
+// so that XCD's L2 holds the column tile of Y while all of them read it.
+__device__ __forceinline__ bool gemm_tile(const OlsGemmArgs& a, int& mt, int& nt) {
   const int b = blockIdx.x, xcd = b & 7, jb = b >> 3;
-  const int mt = jb % a.n_mtiles;
-  const int nt = (jb / a.n_mtiles) * 8 + xcd;
-  if (nt >= a.n_ntiles) return;
+  mt = jb % a.n_mtiles;
+  nt = (jb / a.n_mtiles) * 8 + xcd;
+  return nt < a.n_ntiles;
+}
+
+// Plain FP32 contraction (v_mfma_f32_32x32x2f32): each wave holds 64 rows x
+// 128 columns and reads its fragments straight from global memory.
+__global__ __launch_bounds__(256) void ols_gemm_fp32_kernel(OlsGemmArgs a) {
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, h = l >> 5, r = l & 31;
+  int mt, nt;
+  if (!gemm_tile(a, mt, nt)) return;
   const int64_t m0 = (int64_t)mt * kGemmRows + w * 64;
   const int64_t n0 = (int64_t)nt * kGemmCols;
-
   f32x16 acc[2][4];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
@@ -288,116 +346,151 @@ __global__ __launch_bounds__(256) void ols_gemm_kernel(OlsGemmArgs a) {
     for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int q = 0; q < 16; ++q) acc[i][j][q] = 0.0f;
-
   const int nkg = a.Kp / 8;
   for (int kg0 = 0; kg0 < nkg; kg0 += 2) {
     const int kg = kg0 + h;
-    if constexpr (PREC == kPrecFP32) {
-      float A[2][8], B[4][8];
+    float A[2][8], B[4][8];
 #pragma unroll
-      for (int rt = 0; rt < 2; ++rt) {
-        const float4* s = reinterpret_cast<const float4*>(a.a32 + ((int64_t)kg * a.Mpad + m0 + 32 * rt + r) * 8);
-        float4 x0 = s[0], x1 = s[1];
-        A[rt][0] = x0.x; A[rt][1] = x0.y; A[rt][2] = x0.z; A[rt][3] = x0.w;
-        A[rt][4] = x1.x; A[rt][5] = x1.y; A[rt][6] = x1.z; A[rt][7] = x1.w;
-      }
+    for (int rt = 0; rt < 2; ++rt) {
+      const float4* s = reinterpret_cast<const float4*>(a.a32 + ((int64_t)kg * a.Mpad + m0 + 32 * rt + r) * 8);
+      float4 x0 = s[0], x1 = s[1];
+      A[rt][0] = x0.x; A[rt][1] = x0.y; A[rt][2] = x0.z; A[rt][3] = x0.w;
+      A[rt][4] = x1.x; A[rt][5] = x1.y; A[rt][6] = x1.z; A[rt][7] = x1.w;
+    }
 #pragma unroll
-      for (int ct = 0; ct < 4; ++ct) {
-        const float4* s = reinterpret_cast<const float4*>(a.b32 + ((int64_t)kg * a.Npad + n0 + 32 * ct + r) * 8);
-        float4 x0 = s[0], x1 = s[1];
-        B[ct][0] = x0.x; B[ct][1] = x0.y; B[ct][2] = x0.z; B[ct][3] = x0.w;
-        B[ct][4] = x1.x; B[ct][5] = x1.y; B[ct][6] = x1.z; B[ct][7] = x1.w;
-      }
+    for (int ct = 0; ct < 4; ++ct) {
+      const float4* s = reinterpret_cast<const float4*>(a.b32 + ((int64_t)kg * a.Npad + n0 + 32 * ct + r) * 8);
+      float4 x0 = s[0], x1 = s[1];
+      B[ct][0] = x0.x; B[ct][1] = x0.y; B[ct][2] = x0.z; B[ct][3] = x0.w;
+      B[ct][4] = x1.x; B[ct][5] = x1.y; B[ct][6] = x1.z; B[ct][7] = x1.w;
+    }
 #pragma unroll
-      for (int jj = 0; jj < 8; ++jj)
-#pragma unroll
-        for (int rt = 0; rt < 2; ++rt)
-#pragma unroll
-          for (int ct = 0; ct < 4; ++ct)
-            acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x2f32(A[rt][jj], B[ct][jj], acc[rt][ct], 0, 0, 0);
-    } else {
-      constexpr int NP = (PREC == kPrecBF16x6) ? 3 : 2;
-      bf16x8 A[NP][2], B[NP][4];
-#pragma unroll
-      for (int q = 0; q < NP; ++q) {
-#pragma unroll
-        for (int rt = 0; rt < 2; ++rt)
-          A[q][rt] = ld_bf16x8(a.abf[q] + ((int64_t)kg * a.Mpad + m0 + 32 * rt + r) * 8);
-#pragma unroll
-        for (int ct = 0; ct < 4; ++ct)
-          B[q][ct] = ld_bf16x8(a.bbf[q] + ((int64_t)kg * a.Npad + n0 + 32 * ct + r) * 8);
-      }
+    for (int jj = 0; jj < 8; ++jj)
 #pragma unroll
       for (int rt = 0; rt < 2; ++rt)
 #pragma unroll
-        for (int ct = 0; ct < 4; ++ct) {
-          f32x16 c = acc[rt][ct];
-          if constexpr (PREC == kPrecBF16x6) {
-            c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[2][rt], B[0][ct], c, 0, 0, 0);
-            c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[1][rt], B[1][ct], c, 0, 0, 0);
-            c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[0][rt], B[2][ct], c, 0, 0, 0);
-          }
-          c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[1][rt], B[0][ct], c, 0, 0, 0);
-          c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[0][rt], B[1][ct], c, 0, 0, 0);
-          c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[0][rt], B[0][ct], c, 0, 0, 0);
-          acc[rt][ct] = c;
-        }
-    }
+        for (int ct = 0; ct < 4; ++ct)
+          acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x2f32(A[rt][jj], B[ct][jj], acc[rt][ct], 0, 0, 0);
   }
+  // epilogue: four passes of 32 columns through a [256][32] LDS block
+  __shared__ float ep[kGemmRows * 32];
+#pragma unroll
+  for (int pass = 0; pass < 4; ++pass) {
+    __syncthreads();
+    epi_put<2, 4>(ep, 32, acc, pass, w * 64, 0, h, r);
+    __syncthreads();
+    epi_run(a, ep, 32, tid & 31, n0 + 32 * pass + (tid & 31), (int64_t)mt * kGemmRows, tid >> 5, 8);
+  }
+}
 
-  // ---- epilogue: lane holds rows 32rt + 8(i/4) + 4h + (i%4) of column r ----
+// Split-bf16 contraction staged through LDS.  Workgroup tile 256 rows x 128
+// columns; the four waves form a 2 x 2 grid of 128 x 64 wave tiles (4 x 2
+// MFMA 32x32x16 tiles).  One k step is a pair of k-groups (16 k); its A
+// fragments ([2][256 rows][8] bf16 per plane) and B fragments
+// ([2][128 columns][8]) are copied global -> LDS by the LDS DMA
+// (global_load_lds_dwordx4: each wave instruction moves 64 consecutive
+// 16-byte fragments, landing at consecutive LDS addresses), so every A and B
+// element crosses from L2 once per workgroup instead of once per wave.  Two
+// LDS stages: step s reads its fragments into registers, then issues the DMA
+// of step s+1 into the other stage, then runs its MFMAs while that DMA is in
+// flight; one barrier per step separates the DMA of a stage from the reads
+// of the step before.
+template <int NP>
+struct GemmStage {
+  static constexpr int kA = 2 * kGemmRows * 16;   // bytes of A per plane
+  static constexpr int kB = 2 * kGemmCols * 16;   // bytes of B per plane
+  static constexpr int kPlane = kA + kB;
+  static constexpr int kBytes = NP * kPlane;
+  static_assert(kA == 4 * 2 * 1024 && kB == 4 * 1024, "four waves x (2 A + 1 B) 1 KiB copies per plane");
+};
+
+// Wave w copies, per plane, the A rows 64w..64w+63 of both k-groups and the B
+// columns 64 (w & 1)..+63 of k-group w >> 1: three 1 KiB copies per plane.
+template <int NP>
+__device__ __forceinline__ void gemm_stage_copy(const OlsGemmArgs& a, uint8_t* stage, int kg0, int64_t mrow0,
+                                                int64_t ncol0, int w, int l) {
+  using S = GemmStage<NP>;
+  typedef __attribute__((address_space(3))) void* lds_ptr;
+  const int jb = w >> 1, nc = (w & 1) * 64;
 #pragma unroll
-  for (int ct = 0; ct < 4; ++ct) {
-    const int64_t col = n0 + 32 * ct + r;
-    const bool cvalid = col < a.V;
-    float gF[kOlsMaxK], hF[kOlsMaxK];
-    double r0 = 0.0, mu = 0.0;
+  for (int q = 0; q < NP; ++q) {
 #pragma unroll
-    for (int q = 0; q < kOlsMaxK; ++q) {
-      gF[q] = (cvalid && q < a.f) ? a.gF[(int64_t)q * a.V + col] : 0.0f;
-      hF[q] = (cvalid && q < a.f) ? a.hF[(int64_t)q * a.V + col] : 0.0f;
+    for (int j = 0; j < 2; ++j) {
+      const uint16_t* src = a.abf[q] + ((int64_t)(kg0 + j) * a.Mpad + mrow0 + 64 * w + l) * 8;
+      __builtin_amdgcn_global_load_lds((const void*)src,
+                                       (lds_ptr)(stage + q * S::kPlane + (j * kGemmRows + 64 * w) * 16), 16, 0, 0);
     }
-    if (cvalid) {
-      r0 = a.r0[col];
-      mu = a.mu[col];
+    const uint16_t* src = a.bbf[q] + ((int64_t)(kg0 + jb) * a.Npad + ncol0 + nc + l) * 8;
+    __builtin_amdgcn_global_load_lds((const void*)src,
+                                     (lds_ptr)(stage + q * S::kPlane + S::kA + (jb * kGemmCols + nc) * 16), 16, 0,
+                                     0);
+  }
+}
+
+template <int NP>
+__global__ __launch_bounds__(256) void ols_gemm_bf16_kernel(OlsGemmArgs a) {
+  using S = GemmStage<NP>;
+  constexpr int kSmem = 2 * S::kBytes > kGemmRows * 64 * 4 ? 2 * S::kBytes : kGemmRows * 64 * 4;
+  __shared__ __attribute__((aligned(16))) uint8_t smem[kSmem];
+  const int tid = threadIdx.x, l = tid & 63, h = l >> 5, r = l & 31;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wr = w >> 1, wc = w & 1;
+  int mt, nt;
+  if (!gemm_tile(a, mt, nt)) return;
+  const int64_t mrow0 = (int64_t)mt * kGemmRows, ncol0 = (int64_t)nt * kGemmCols;
+  f32x16 acc[4][2];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int q = 0; q < 16; ++q) acc[i][j][q] = 0.0f;
+
+  const int nkg = a.Kp / 8;
+  gemm_stage_copy<NP>(a, smem, 0, mrow0, ncol0, w, l);
+  for (int kg0 = 0, st = 0; kg0 < nkg; kg0 += 2, st ^= 1) {
+    __builtin_amdgcn_s_waitcnt(0xF70);  // vmcnt(0): this wave's copies of step kg0 have landed
+    __syncthreads();                    // ... and every other wave's
+    const uint8_t* cur = smem + st * S::kBytes;
+    bf16x8 A[NP][4], B[NP][2];
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+#pragma unroll
+      for (int rt = 0; rt < 4; ++rt)
+        A[q][rt] = *reinterpret_cast<const bf16x8*>(cur + q * S::kPlane +
+                                                     (h * kGemmRows + wr * 128 + 32 * rt + r) * 16);
+#pragma unroll
+      for (int ct = 0; ct < 2; ++ct)
+        B[q][ct] = *reinterpret_cast<const bf16x8*>(cur + q * S::kPlane + S::kA +
+                                                     (h * kGemmCols + wc * 64 + 32 * ct + r) * 16);
     }
+    if (kg0 + 2 < nkg) gemm_stage_copy<NP>(a, smem + (st ^ 1) * S::kBytes, kg0 + 2, mrow0, ncol0, w, l);
 #pragma unroll
-    for (int rt = 0; rt < 2; ++rt) {
+    for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
-      for (int a4 = 0; a4 < 4; ++a4) {
-        float G[4], O[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) G[q] = acc[rt][ct][4 * a4 + q];
-        if (a.rpp == 4) {
-          const int p = (int)((m0 + 32 * rt + 8 * a4 + 4 * h) >> 2);
-          float gz[kOlsMaxK];
-#pragma unroll
-          for (int q = 0; q < kOlsMaxK; ++q) gz[q] = q < 4 ? G[q & 3] : 0.0f;
-          if (!a.centered) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-              if (q < a.m) gz[q] += (float)(a.colsumZ[q] * mu);
-          }
-          if (cvalid && p < a.P) ols_epilogue(a, p, col, gz, gF, hF, r0, mu);
-        } else {
-#pragma unroll
-          for (int q = 0; q < 4; ++q) O[q] = __shfl_xor(G[q], 32, 64);
-          const int p = (int)((m0 + 32 * rt + 8 * a4) >> 3);
-          float gz[kOlsMaxK];
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            gz[q] = h ? O[q] : G[q];
-            gz[4 + q] = h ? G[q] : O[q];
-          }
-          if (!a.centered) {
-#pragma unroll
-            for (int q = 0; q < kOlsMaxK; ++q)
-              if (q < a.m) gz[q] += (float)(a.colsumZ[q] * mu);
-          }
-          if (cvalid && p < a.P && (a4 & 1) == h) ols_epilogue(a, p, col, gz, gF, hF, r0, mu);
+      for (int ct = 0; ct < 2; ++ct) {
+        f32x16 c = acc[rt][ct];
+        if constexpr (NP == 3) {
+          c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[2][rt], B[0][ct], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[1][rt], B[1][ct], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[0][rt], B[2][ct], c, 0, 0, 0);
         }
+        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[1][rt], B[0][ct], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[0][rt], B[1][ct], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[0][rt], B[0][ct], c, 0, 0, 0);
+        acc[rt][ct] = c;
       }
-    }
+  }
+  // epilogue: two passes of 64 columns (32 from each wave column) through a
+  // [256][64] LDS block that reuses the staging buffer
+  float* ep = reinterpret_cast<float*>(smem);
+#pragma unroll
+  for (int pass = 0; pass < 2; ++pass) {
+    __syncthreads();
+    epi_put<4, 2>(ep, 64, acc, pass, wr * 128, wc * 32, h, r);
+    __syncthreads();
+    const int cl = tid & 63;
+    epi_run(a, ep, 64, cl, ncol0 + (cl >> 5) * 64 + 32 * pass + (cl & 31), mrow0, tid >> 6, 4);
   }
 }
 
@@ -430,11 +523,11 @@ hipError_t launch_ols_gemm(const OlsGemmArgs& a, int precision, hipStream_t s) {
   const int64_t ntp = (a.n_ntiles + 7) / 8 * 8;
   const unsigned blocks = (unsigned)(ntp * a.n_mtiles);
   if (precision == kPrecFP32)
-    hipLaunchKernelGGL(ols_gemm_kernel<kPrecFP32>, dim3(blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(ols_gemm_fp32_kernel, dim3(blocks), dim3(256), 0, s, a);
   else if (precision == kPrecBF16x3)
-    hipLaunchKernelGGL(ols_gemm_kernel<kPrecBF16x3>, dim3(blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(ols_gemm_bf16_kernel<2>, dim3(blocks), dim3(256), 0, s, a);
   else
-    hipLaunchKernelGGL(ols_gemm_kernel<kPrecBF16x6>, dim3(blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(ols_gemm_bf16_kernel<3>, dim3(blocks), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

@@ -110,7 +110,7 @@ class TailMLP:
         loss = torch.empty(steps, dtype=torch.float32, device="cuda")
         g0 = torch.empty(self.n_params, dtype=torch.float32, device="cuda") if want_grad0 else None
         _lib.check(_lib.lib().tfce_mlp_train(_lib.ptr(self.params), _lib.ptr(self.m), _lib.ptr(self.v),
-                                             _lib.ptr(Xd), _lib.ptr(Yd), _lib.ptr(idx_d), steps, b, self.t,
+                                             _lib.ptr(Xd), _lib.ptr(Yd), n, _lib.ptr(idx_d), steps, b, self.t,
                                              float(lr), float(weight_decay), _lib.ptr(loss), _lib.ptr(g0),
                                              _lib.stream_ptr()))
         self.t += steps
@@ -184,3 +184,172 @@ def generate_nulls(make_config, seeds, n_values, batch=256, precision="bf16x3"):
         out.append(null)
         del eng
     return out
+
+
+# ----------------------------------------------------------------------------
+# Workloads for training and evaluating the tail model
+# ----------------------------------------------------------------------------
+class WorkloadNull:
+    """One permutation workload whose max-TFCE null can be drawn in batches.
+
+    kind "glm": t of the last design column, +t and -t, two values per
+    permutation (the order vertex/voxel randomise appends them);
+    kind "med": type-M Sobel z of a mediation model, positive only, one value
+    per permutation (vertex_tfce_mediation_randomise.py)."""
+
+    def __init__(self, kind, Y, X=None, surfaces=None, offsets=None, settings=None, x=None, y=None,
+                 precision="bf16x3"):
+        from .engine import PermutationEngine
+        self.kind = kind
+        if kind == "glm":
+            self.n = X.shape[0]
+            self.engine = PermutationEngine(Y, X, surfaces, offsets, contrast=X.shape[1] - 1,
+                                            precision=precision, settings=settings)
+        else:
+            from .randomise import MediationModel
+            from .tfce import Adjacency, SurfaceSet
+            self.n = len(x)
+            self.model = MediationModel(Y, x, y, "M", precision=precision)
+            self.surfaces = SurfaceSet([Adjacency(c, s[0], s[1]) for c, s in zip(surfaces, settings)], offsets)
+
+    def values(self, n_perm, seed=0, batch=256):
+        from . import synth
+        out, done = [], 0
+        while done < n_perm:
+            P = min(batch, n_perm - done)
+            pb = synth.permutations(self.n, P, seed + done, "cuda")
+            if self.kind == "glm":
+                out.append(self.engine.null_batch(pb).cpu().numpy().ravel())
+            else:
+                z = self.model.z(pb)
+                out.append(self.surfaces.batch(z, 1).max(0).values[:, 0].cpu().numpy())
+            done += P
+        return np.concatenate(out).astype(np.float64)
+
+
+def random_workload(rng, kind=None, precision="bf16x3"):
+    """A small random workload: vertex GLM (icosphere order 4-6, one or two
+    hemispheres), voxel GLM (ellipsoid mask, 26-connectivity, E = 0.5) or
+    vertex mediation; n, smoothness and covariates drawn from rng."""
+    import torch
+    from . import synth
+    from .adjacency import voxel_csr
+    if kind is None:
+        kind = rng.choice(["vertex", "voxel", "med"], p=[0.45, 0.3, 0.25])
+    seed = int(rng.integers(1 << 30))
+    g = synth._gen(seed, "cuda")
+    n = int(rng.integers(40, 301))
+    if kind in ("vertex", "med"):
+        order = int(rng.choice([4, 5, 6], p=[0.2, 0.5, 0.3]))
+        csr = synth.icosphere_csr(order)
+        Vh = csr[0].shape[0] - 1
+        hemis = int(rng.choice([1, 2]))
+        S = synth.smoothing_operator(*csr, device="cuda")
+        passes = int(rng.integers(1, 9))
+        Y = torch.cat([synth.standardize(synth.mesh_smooth(torch.randn((n, Vh), generator=g, device="cuda"), S,
+                                                           passes)) for _ in range(hemis)], 1).contiguous()
+        surfaces, offsets, settings = [csr] * hemis, [h * Vh for h in range(hemis)], [(2.0, 1.0)] * hemis
+        if kind == "med":
+            x, y = rng.standard_normal(n), rng.standard_normal(n)
+            return WorkloadNull("med", Y, surfaces=surfaces, offsets=offsets, settings=settings, x=x, y=y,
+                                precision=precision)
+    else:
+        semi = tuple(float(s) for s in rng.uniform(7, 15, 3))
+        shape = tuple(int(2 * s + 5) for s in semi)
+        center = tuple((sh - 1) / 2.0 for sh in shape)
+        mask = synth.ellipsoid_mask(shape, center, semi)
+        vol = synth.gaussian_smooth3d(torch.randn((n,) + shape, generator=g, device="cuda"),
+                                      float(rng.uniform(1.5, 4.0)))
+        Y = synth.standardize(vol[:, torch.from_numpy(mask).cuda()]).contiguous()
+        surfaces, offsets, settings = [voxel_csr(mask, 26)], [0], [(2.0, 0.5)]
+    X = synth.design(rng, n, int(rng.integers(1, 5)), binary=bool(rng.random() < 0.5))
+    return WorkloadNull("glm", Y, X=X, surfaces=surfaces, offsets=offsets, settings=settings, precision=precision)
+
+
+def c3_workload(seed, precision="bf16x3"):
+    """Two order-7 icospheres (327,684 vertices), n = 500, k = 6."""
+    from . import synth
+    cfg = synth.config3(seed=seed, device="cuda")
+    return WorkloadNull("glm", cfg["Y"], X=cfg["X"], surfaces=cfg["surfaces"], offsets=cfg["offsets"],
+                        settings=[(cfg["H"], cfg["E"])] * len(cfg["surfaces"]), precision=precision)
+
+
+def c5_workload(seed, precision="bf16x3"):
+    """Mediation on two order-7 icospheres, n = 300, three covariates
+    regressed out of the mediator images first."""
+    import torch
+    from . import synth
+    cfg = synth.config5(seed=seed, device="cuda")
+    C = torch.as_tensor(np.column_stack([np.ones(cfg["n"]), cfg["covars"]]), dtype=torch.float64).cuda()
+    M = cfg["M"].double()
+    M = (M - C @ torch.linalg.lstsq(C, M).solution).float().contiguous()
+    return WorkloadNull("med", M, surfaces=cfg["surfaces"], offsets=cfg["offsets"],
+                        settings=[(cfg["H"], cfg["E"])] * len(cfg["surfaces"]), x=cfg["x"], y=cfg["y"],
+                        precision=precision)
+
+
+def study(n_train=64, n_train_perm=10000, n_eval=4, n_eval_perm=10000, m0=500, delta=0.01, steps=4000, seed=0,
+          log=print):
+    """Train on random small workloads, then evaluate on the full-size
+    two-hemisphere GLM and mediation workloads against 10,000-permutation
+    plain runs.  Returns a dict of per-null results."""
+    import time
+    rng = np.random.default_rng(seed)
+    t0 = time.time()
+    train = []
+    for i in range(n_train):
+        wl = random_workload(rng)
+        vals = wl.values(n_train_perm, seed=int(rng.integers(1 << 30)))
+        train.append(vals)
+        del wl
+        if (i + 1) % 8 == 0:
+            log(f"[study] {i + 1}/{n_train} training nulls ({time.time() - t0:.0f}s)")
+    X, Y = make_examples(train, m0=m0, per_null=128, seed=seed + 1)
+    model = TailMLP(seed=seed + 2)
+    losses = model.train(X, Y, steps=steps, lr=2e-3)
+    log(f"[study] trained on {len(X)} examples: loss {losses[:50].mean():.4f} -> {losses[-200:].mean():.4f}")
+    rule = LearnedStop(model, m0=m0, n_max=None, delta=delta)
+    out = {"train_nulls": n_train, "train_perm": n_train_perm, "m0": m0, "delta": delta, "eval": []}
+    for name, make in (("c3_glm_327k", c3_workload), ("c5_mediation_327k", c5_workload)):
+        for e in range(n_eval):
+            wl = make(20261300 + 17 * e)
+            ts = time.time()
+            vals = wl.values(n_eval_perm, seed=1000003 * (e + 1))
+            dt = time.time() - ts
+            del wl
+            per_perm = 2 if name.startswith("c3") else 1
+            truth = np.array([critical_value(vals, a) for a in ALPHAS])
+            pred = model.predict_critical(vals[:m0])
+            emp = np.array([critical_value(vals[:m0], a) for a in ALPHAS])
+            rule.n_max = len(vals)
+            used, thr, src = rule.simulate(vals)
+            rec = dict(workload=name, seed=e, values=len(vals), seconds=round(dt, 2),
+                       truth=truth.tolist(), pred_at_m0=pred.tolist(), emp_at_m0=emp.tolist(),
+                       rel_err_pred=(np.abs(pred - truth) / truth).tolist(),
+                       rel_err_emp=(np.abs(emp - truth) / truth).tolist(),
+                       stop_values=int(used), stop_perms=int(used // per_perm), stop_source=src,
+                       stop_rel_err=(np.abs(thr - truth) / truth).tolist())
+            out["eval"].append(rec)
+            log(f"[study] {name} #{e}: pred@{m0} err {np.round(rec['rel_err_pred'], 4)} emp@{m0} err "
+                f"{np.round(rec['rel_err_emp'], 4)}; stop after {rec['stop_perms']} perms ({src}), err "
+                f"{np.round(rec['stop_rel_err'], 4)}")
+    return out
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description="learned-stop study on synthetic workloads")
+    ap.add_argument("--train", type=int, default=64)
+    ap.add_argument("--train-perm", type=int, default=10000)
+    ap.add_argument("--eval", type=int, default=4)
+    ap.add_argument("--eval-perm", type=int, default=10000)
+    ap.add_argument("--delta", type=float, default=0.01)
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--out", default=None)
+    args = ap.parse_args()
+    res = study(args.train, args.train_perm, args.eval, args.eval_perm, delta=args.delta, steps=args.steps,
+                log=lambda *a: print(*a, flush=True))
+    if args.out:
+        with open(args.out, "w") as f:
+            json.dump(res, f, indent=1)

@@ -127,6 +127,22 @@ def cpu_baseline(cfg, perms_np, n_workers, per_worker):
                 aggregate_rate=n_done / max(worker_times), values=values, used_reference=res[0][2])
 
 
+def _measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or (None, None)."""
+    path = os.path.join(ROOT, "profiles", "%s_tfce_hbm_traffic.md" % workload)
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 4 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel \
+                        and parts[2] == workload:
+                    return int(parts[3]), os.path.relpath(path, ROOT)
+    except OSError:
+        pass
+    return None, None
+
+
 # ----------------------------------------------------------------------------
 def main():
     ap = argparse.ArgumentParser()
@@ -262,6 +278,7 @@ def main():
     bytes_per_sign = n_thr * (4 * (V + 1) + 4 * nnz + 8 * V) + 8 * V   # per permutation, both hemispheres
     units = P * 2
     achieved = bytes_per_sign * units / (tfce_ms * 1e-3) / 1e9
+    traffic, traffic_src = _measured_traffic("tfce_batch_kernel<false>", "c3")
     k = cfg["X"].shape[1]
     gemm_flops = 2.0 * k * n * V * P
     out = {
@@ -297,7 +314,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
             "algorithmic_bytes_per_unit": bytes_per_sign,
             "units_per_launch": units,
             "note": "level-sweep byte model (100 thresholds x (CSR + map + labels) per sign); the kernel "

@@ -298,12 +298,14 @@ def study(n_train=64, n_train_perm=10000, n_eval=4, n_eval_perm=10000, m0=500, d
     t0 = time.time()
     train = []
     for i in range(n_train):
+        ts = time.time()
         wl = random_workload(rng)
+        tb = time.time()
         vals = wl.values(n_train_perm, seed=int(rng.integers(1 << 30)))
         train.append(vals)
+        log(f"[study] training null {i + 1}/{n_train}: {wl.kind}, set-up {tb - ts:.1f}s, "
+            f"{n_train_perm} permutations {time.time() - tb:.1f}s")
         del wl
-        if (i + 1) % 8 == 0:
-            log(f"[study] {i + 1}/{n_train} training nulls ({time.time() - t0:.0f}s)")
     X, Y = make_examples(train, m0=m0, per_null=128, seed=seed + 1)
     model = TailMLP(seed=seed + 2)
     losses = model.train(X, Y, steps=steps, lr=2e-3)
@@ -312,7 +314,9 @@ def study(n_train=64, n_train_perm=10000, n_eval=4, n_eval_perm=10000, m0=500, d
     out = {"train_nulls": n_train, "train_perm": n_train_perm, "m0": m0, "delta": delta, "eval": []}
     for name, make in (("c3_glm_327k", c3_workload), ("c5_mediation_327k", c5_workload)):
         for e in range(n_eval):
+            ts = time.time()
             wl = make(20261300 + 17 * e)
+            log(f"[study] {name} #{e}: set-up {time.time() - ts:.1f}s")
             ts = time.time()
             vals = wl.values(n_eval_perm, seed=1000003 * (e + 1))
             dt = time.time() - ts

@@ -156,6 +156,10 @@ def main():
     ap.add_argument("--cpu-perms-per-worker", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default contraction tile and sweep occupancy")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="skip the hipGraph replay measurement of the same steps")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="copy a buffer of this many MiB once before the timed region, so that a "
                          "counter run (FETCH_SIZE / WRITE_SIZE) holds a plain copy of known size")
@@ -182,6 +186,12 @@ def main():
     torch.cuda.synchronize()
     n = cfg["n"]
     perm_batches = [synth.permutations(n, P, 1000 + s, "cuda") for s in range(args.warmup + args.steps)]
+
+    # ---- launch shapes picked on this workload (not cached: measured here) ----
+    tuned = None
+    if not args.no_autotune:
+        from tfce_mediation_amd import autotune
+        tuned = autotune.tune(eng, P, cache=None, log=log)
 
     # ---- warmup -------------------------------------------------------------
     for s in range(args.warmup):
@@ -223,12 +233,29 @@ def main():
     log(f"[bench] batched {args.precision}: {value:.1f} perms/s  ({elapsed / args.steps * 1e3:.2f} ms/step; "
         f"contraction {gemm_ms:.2f} ms, TFCE {tfce_ms:.2f} ms)")
 
+    # ---- the same steps replayed from a captured hipGraph ---------------------
+    graph_rate = None
+    if not args.no_graph:
+        run = eng.graph(P)
+        same = np.array_equal(run(perm_batches[args.warmup]).cpu().numpy(),
+                              eng.null_batch(perm_batches[args.warmup]).cpu().numpy())
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for s in range(args.steps):
+            run(perm_batches[args.warmup + s])
+        torch.cuda.synchronize()
+        graph_rate = args.steps * P / (time.perf_counter() - t0)
+        log(f"[bench] hipGraph replay: {graph_rate:.1f} perms/s (replay output equals eager: {same})")
+        if not same:
+            graph_rate = None
+        del run
+
     # ---- plain FP32 per-permutation path ------------------------------------
     plain = None
     if not args.no_plain:
         eng32 = PermutationEngine(cfg["Y"], cfg["X"], cfg["surfaces"], cfg["offsets"], cfg["H"], cfg["E"],
                                   contrast=cfg["contrast"], precision="fp32")
-        adjs = [Adjacency(csr, cfg["H"], cfg["E"]) for csr in cfg["surfaces"]]
+        adjs = [Adjacency(csr, cfg["H"], cfg["E"]) for csr in eng32.csrs]
         Vh = adjs[0].V
         enh = torch.empty(Vh, dtype=torch.float32, device="cuda")
 
@@ -302,6 +329,9 @@ def main():
             "precision": args.precision,
             "plain_fp32_perms_per_s": None if plain is None else round(plain, 3),
             "batched_over_plain": None if plain is None else round(value / plain, 2),
+            "hipgraph_replay_perms_per_s": None if graph_rate is None else round(graph_rate, 1),
+            "autotuned": None if tuned is None else {"gemm_tile": ["256x128", "512x64", "512x128"][tuned["gemm_tile"]],
+                                                     "tfce_wgs_per_cu": tuned["tfce_wgs_per_cu"]},
             "contraction_ms_per_step": round(gemm_ms, 3),
             "tfce_ms_per_step": round(tfce_ms, 3),
             "contraction_tflops": round(gemm_flops / (gemm_ms * 1e-3) / 1e12, 2),

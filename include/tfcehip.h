@@ -52,6 +52,14 @@ int tfce_adj_create(const int32_t *row_ptr, const int32_t *col, int32_t V, int64
 int tfce_adj_create_ex(const int32_t *row_ptr, const int32_t *col, int32_t V, int64_t nnz,
                        float H, float E, uint32_t flags, tfce_adj_t **out);
 int tfce_adj_destroy(tfce_adj_t *adj);
+
+/* Locality order of an adjacency, host only: new_of_old[v] = new index of
+ * element v (reverse Cuthill-McKee per component from a pseudo-peripheral
+ * vertex).  No counterpart in the reference, whose TFCE walks the caller's
+ * element order; relabelling a symmetric adjacency and its maps with this
+ * order changes no partition and no TFCE value, and puts neighbours close in
+ * memory, which cuts the random HBM traffic of the threshold sweep. */
+int tfce_locality_order(const int32_t *row_ptr, const int32_t *col, int32_t V, int32_t *new_of_old);
 int tfce_adj_info(const tfce_adj_t *adj, int32_t *V, int64_t *nnz, int32_t *symmetric);
 
 /* Replaces CreateAdjSet.run(image, enhn) (tfce.pyx:44-45 -> fast_tfce.hpp:11-95):
@@ -107,6 +115,8 @@ int tfce_batch(const tfce_surfset_t *set, const float *d_maps, int64_t map_strid
 #define TFCE_PREC_FP32 0    /* FP32 matrix cores */
 #define TFCE_PREC_BF16X3 1  /* hi/lo bf16 split, three products */
 #define TFCE_PREC_BF16X6 2  /* three-piece bf16 split, six products */
+#define TFCE_PREC_FP16X3 3  /* hi/lo fp16 split of operands scaled by powers of two
+                               (per element of Y, per permuted design column), three products */
 
 typedef struct tfce_ols_data tfce_ols_data_t;
 typedef struct tfce_ols_model tfce_ols_model_t;
@@ -138,6 +148,12 @@ int tfce_ols_info(const tfce_ols_model_t *model, int32_t *k, int32_t *n_fixed, i
 int tfce_ols_perm_t(const tfce_ols_model_t *model, const int32_t *d_perm, int32_t P, float *d_t,
                     int64_t ldt, hipStream_t stream);
 
+/* Workgroup tile of the split-precision contraction (process-wide; the
+ * autotuner's knob): 0 = 256 rows x 128 elements (default), 1 = 512 x 64,
+ * 2 = 512 x 128.  No counterpart in the reference. */
+int tfce_ols_set_tile(int32_t tile);
+int tfce_ols_tile(void);
+
 /* One design, FP64 throughout (the numpy path of cynumstats.pyx:28-112 for a
  * single X): coefficients with inv(X'X) computed here, residuals, SSE and
  * se_j = fl32(sqrt(SSE/(n-k) * S[j][j])) with S = invXX_se (the invXX the
@@ -160,6 +176,21 @@ int tfce_se_of_slope(const double *d_sigma2, const double *invXX, int32_t k, int
  * d_ta == NULL uses the scalar ta_const (mediation type Y, pyfunc.py:142). */
 int tfce_sobel_z(const float *d_ta, const float *d_tb, int64_t n, int32_t alg, float ta_const,
                  float *d_z, hipStream_t stream);
+
+/* One-call forms (blocking; they build and release the handles above).
+ * ols_perm_t: t of column `contrast` of X[perm_p] (X host, n x k row-major
+ * float64; every non-constant column moves, as the reference's full-row
+ * permutation X[perm] does, vertex_tfce_multiple_regression_randomise.py:105-107)
+ * for P permutations d_perm[P][n]; d_t[P][V].  The reference's tval_int takes
+ * invXX besides X; row permutation leaves X'X unchanged, so X alone fixes it.
+ * sobel_perm_z: per permutation of x, the Sobel-family z of calc_sobelz
+ * (pyfunc.py:130-162, vertex_tfce_mediation_randomise.py:82-91) for mediator
+ * images d_M[n][V]: medtype 0 = M, 1 = I, 2 = Y (type Y moves x and y
+ * together); alg 0 aroian, 1 sobel, 2 goodman; d_z[P][V]. */
+int ols_perm_t(const double *X, const float *d_Y, const int32_t *d_perm, int32_t n, int32_t k, int64_t V,
+               int32_t contrast, int32_t precision, int32_t P, float *d_t, hipStream_t stream);
+int sobel_perm_z(const float *d_M, int32_t n, int64_t V, const double *x, const double *y, int32_t medtype,
+                 int32_t alg, const int32_t *d_perm, int32_t P, int32_t precision, float *d_z, hipStream_t stream);
 
 /* ------------------------------------------------------------------ FWER */
 /* 1 - p(FWE) of each element against the sorted (ascending) null, FP64.

@@ -73,3 +73,49 @@ def test_vertex_density_formula():
     assert vd.dtype == np.float32
     assert np.allclose(vd[deg == 6], 1 - 1 + deg.mean() / 6)
     assert np.allclose(vd[deg == 5], 1 - 5 / 6 + deg.mean() / 6)
+
+
+def _bandwidth(rp, col):
+    src = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    return int(np.abs(src - np.asarray(col)).max()) if len(col) else 0
+
+
+def test_locality_order_is_a_band_reducing_permutation():
+    """tfce_locality_order (host code of the library): a permutation; a
+    shuffled path comes back with bandwidth 1; on an icosphere the bandwidth is
+    within 25% of scipy's reverse Cuthill-McKee; components stay contiguous."""
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import reverse_cuthill_mckee
+    from tfce_mediation_amd.adjacency import locality_order, relabel_csr, to_csr
+    from tfce_mediation_amd import synth
+    rng = np.random.default_rng(0)
+    n = 500
+    lab = rng.permutation(n)
+    edges = [(lab[i], lab[i + 1]) for i in range(n - 1)]
+    adj = [[] for _ in range(n)]
+    for a, b in edges:
+        adj[a].append(b)
+        adj[b].append(a)
+    rp, col = to_csr(adj)
+    order = locality_order(rp, col)
+    assert np.array_equal(np.sort(order), np.arange(n))
+    rp2, col2 = relabel_csr(rp, col, order)
+    assert _bandwidth(rp2, col2) == 1
+    # the relabelled graph is the same graph
+    src = np.repeat(np.arange(n), np.diff(rp))
+    e1 = sorted(zip(order[src].tolist(), order[np.asarray(col)].tolist()))
+    src2 = np.repeat(np.arange(n), np.diff(rp2))
+    assert e1 == sorted(zip(src2.tolist(), col2.tolist()))
+    rp, col = synth.icosphere_csr(5)
+    V = len(rp) - 1
+    rp2, col2 = relabel_csr(rp, col, locality_order(rp, col))
+    r = reverse_cuthill_mckee(csr_matrix((np.ones(len(col)), col, rp), shape=(V, V)), symmetric_mode=True)
+    inv = np.empty(V, np.int64)
+    inv[r] = np.arange(V)
+    rp3, col3 = relabel_csr(rp, col, inv)
+    assert _bandwidth(rp2, col2) <= 1.25 * _bandwidth(rp3, col3)
+    assert _bandwidth(rp2, col2) < _bandwidth(rp, col) / 10
+    # two components: each occupies a contiguous id range
+    adj = [[1], [0, 2], [1], [4], [3]]
+    o = locality_order(*to_csr(adj))
+    assert sorted(o[:3].tolist()) in ([0, 1, 2], [2, 3, 4])

@@ -244,3 +244,39 @@ def test_tm_func_calculate_tfce_observed(torch):
     for s in range(2):
         a, b = pos_arr[s], pos_arr[s + 1]
         np.testing.assert_array_equal(tp[0, a:b], enh[a:b] * (tv[0][a:b].max() / np.float32(100)))
+
+
+def test_one_call_entry_points_match_the_handle_path():
+    """ols_perm_t / sobel_perm_z (one-call C ABI) against PermutedOLS and
+    MediationModel on the same permutations: identical kernels, identical
+    numbers (type Y's scalar path-A t comes from a closed form here and from
+    numpy's inverse there: rel 1e-6)."""
+    import torch
+    from tfce_mediation_amd import _lib, synth
+    from tfce_mediation_amd.ols import PermutedOLS
+    from tfce_mediation_amd.randomise import MediationModel
+    print("[gpu] one-call ols_perm_t / sobel_perm_z", flush=True)
+    rng = np.random.default_rng(8)
+    n, V, P = 70, 3000, 24
+    Y = torch.randn((n, V), generator=torch.Generator(device="cuda").manual_seed(2), device="cuda")
+    X = synth.design(rng, n, 3)
+    perms = synth.permutations(n, P, 4, "cuda")
+    lib = _lib.lib()
+    for prec in ("fp32", "bf16x3"):
+        t = torch.empty((P, V), device="cuda")
+        Xc = np.ascontiguousarray(X, dtype=np.float64)
+        _lib.check(lib.ols_perm_t(Xc.ctypes.data, _lib.ptr(Y), _lib.ptr(perms), n, X.shape[1], V, X.shape[1] - 1,
+                                  _lib.PREC[prec], P, _lib.ptr(t), _lib.stream_ptr()))
+        ref = PermutedOLS(Y, X, contrasts=[X.shape[1] - 1], precision=prec).t(perms)[0]
+        np.testing.assert_array_equal(t.cpu().numpy(), ref.cpu().numpy())
+    x, yv = rng.standard_normal(n), rng.standard_normal(n)
+    for mt, code in (("M", 0), ("I", 1), ("Y", 2)):
+        z = torch.empty((P, V), device="cuda")
+        _lib.check(lib.sobel_perm_z(_lib.ptr(Y), n, V, np.ascontiguousarray(x).ctypes.data,
+                                    np.ascontiguousarray(yv).ctypes.data, code, 0, _lib.ptr(perms), P,
+                                    _lib.PREC["bf16x3"], _lib.ptr(z), _lib.stream_ptr()))
+        ref = MediationModel(Y, x, yv, mt, precision="bf16x3").z(perms).cpu().numpy()
+        if mt == "Y":
+            np.testing.assert_allclose(z.cpu().numpy(), ref, rtol=1e-6, atol=1e-7)
+        else:
+            np.testing.assert_array_equal(z.cpu().numpy(), ref)

@@ -1,0 +1,56 @@
+"""The batched engine's element layout: holding the surfaces in the library's
+locality order must not change a single null value (max-TFCE is invariant
+under relabelling a symmetric graph, and each element's t-value is computed
+by the same arithmetic wherever its column sits)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_locality_layout_leaves_the_null_unchanged():
+    import torch
+    from tfce_mediation_amd import synth
+    from tfce_mediation_amd.engine import PermutationEngine
+    print("[gpu] engine locality layout", flush=True)
+    for make in (lambda: synth.config1(seed=5, n=80, order=5, device="cuda"),
+                 lambda: synth.config2(seed=6, n=60, shape=(30, 34, 30), center=(14.5, 16.5, 14.5),
+                                       semi=(12, 14, 12), device="cuda")):
+        cfg = make()
+        perms = synth.permutations(cfg["n"], 64, 11, "cuda")
+        out = []
+        for locality in (False, True):
+            eng = PermutationEngine(cfg["Y"], cfg["X"], cfg["surfaces"], cfg["offsets"], cfg["H"], cfg["E"],
+                                    contrast=cfg["contrast"], precision="bf16x3", locality=locality)
+            out.append(eng.null_batch(perms).cpu().numpy())
+            if locality:
+                assert eng.cols is not None
+                t = eng.tvalues(perms[:4]).cpu().numpy()
+        np.testing.assert_array_equal(out[0], out[1])
+        # the t-values are the same numbers in a permuted column order
+        eng0 = PermutationEngine(cfg["Y"], cfg["X"], cfg["surfaces"], cfg["offsets"], cfg["H"], cfg["E"],
+                                 contrast=cfg["contrast"], precision="bf16x3", locality=False)
+        t0 = eng0.tvalues(perms[:4]).cpu().numpy()
+        np.testing.assert_array_equal(t0[:, eng.cols], t)
+        torch.cuda.synchronize()
+
+
+def test_graph_replay_matches_eager():
+    """The captured batch replayed several times, with eager batches and
+    synchronisations between the replays, gives the eager null every time."""
+    import torch
+    from tfce_mediation_amd import synth
+    from tfce_mediation_amd.engine import PermutationEngine
+    print("[gpu] engine hipGraph replay", flush=True)
+    cfg = synth.config1(seed=9, n=64, order=5, device="cuda")
+    eng = PermutationEngine(cfg["Y"], cfg["X"], cfg["surfaces"], cfg["offsets"], cfg["H"], cfg["E"],
+                            contrast=cfg["contrast"], precision="bf16x3")
+    P = 48
+    run = eng.graph(P)
+    for rep in range(4):
+        perms = synth.permutations(cfg["n"], P, 100 + rep, "cuda")
+        got = run(perms).clone()
+        torch.cuda.synchronize()
+        ref = eng.null_batch(perms)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(got.cpu().numpy(), ref.cpu().numpy())

@@ -43,10 +43,10 @@ def errors(t, ref):
     return norm, point
 
 
-BARS = {"fp32": (1e-5, 1e-5), "bf16x6": (1e-5, 1e-5), "bf16x3": (2e-5, None)}
+BARS = {"fp32": (1e-5, 1e-5), "bf16x6": (1e-5, 1e-5), "fp16x3": (1e-5, 1e-5), "bf16x3": (2e-5, None)}
 
 
-@pytest.mark.parametrize("prec", ["fp32", "bf16x3", "bf16x6"])
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3", "bf16x6", "fp16x3"])
 @pytest.mark.parametrize("n,k,V,P", [(100, 4, 3000, 37), (160, 6, 2500, 20), (75, 5, 1030, 9)])
 def test_perm_t_matches_fp64(torch, prec, n, k, V, P):
     from tfce_mediation_amd.ols import PermutedOLS
@@ -66,7 +66,7 @@ def test_perm_t_matches_fp64(torch, prec, n, k, V, P):
         assert point <= pb
 
 
-@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3", "fp16x3"])
 def test_partial_permutation_and_offset(torch, prec):
     """-v mode (only columns 2..3 move: per-permutation Gram factors) and a
     large per-element mean (centring)."""
@@ -105,3 +105,49 @@ def test_constant_element_gives_nan_like_reference(torch):
     t = model.t().cpu().numpy()
     assert np.all(np.isnan(t[:, 0, 5]))
     assert np.all(np.isfinite(np.delete(t[:, 0, :], 5, axis=1)))
+
+
+def test_fp16x3_scaling_covers_wide_ranges(torch):
+    """fp16 pieces of power-of-two scaled operands: data of magnitude 1e4 and
+    1e-4 and a covariate of magnitude 1e3 keep the FP32-class bar."""
+    from tfce_mediation_amd.ols import PermutedOLS
+    n, k, V, P = 120, 4, 2000, 16
+    say("[gpu] ols fp16x3 scaling")
+    X, Y = make(n, k, V, seed=21)
+    X[:, 1] *= 1000.0
+    Y[:, : V // 2] *= 1e4
+    Y[:, V // 2:] *= 1e-4
+    rng = np.random.default_rng(5)
+    perms = np.stack([rng.permutation(n) for _ in range(P)]).astype(np.int32)
+    con = [1, 2, 3]
+    t = PermutedOLS(torch.from_numpy(Y).cuda(), X, contrasts=con, precision="fp16x3").t(
+        torch.from_numpy(perms).cuda()).cpu().numpy()
+    ref = ools.perm_t(X, Y, perms, con)
+    norm, point = errors(t, ref)
+    say(f"    norm={norm:.2e} point={point:.2e}")
+    assert norm <= 1e-5 and point <= 1e-5
+
+
+@pytest.mark.parametrize("prec", ["bf16x3", "fp16x3", "bf16x6"])
+def test_every_tile_gives_identical_t(torch, prec):
+    """The contraction's workgroup tiles (autotuner choices) sum each output
+    in the same k order: identical t-values, including a P that leaves a
+    partial row tile."""
+    from tfce_mediation_amd import _lib
+    from tfce_mediation_amd.ols import PermutedOLS
+    n, k, V, P = 130, 6, 5000, 70
+    say(f"[gpu] ols tiles {prec}")
+    X, Y = make(n, k, V, seed=31)
+    rng = np.random.default_rng(9)
+    perms = torch.from_numpy(np.stack([rng.permutation(n) for _ in range(P)]).astype(np.int32)).cuda()
+    model = PermutedOLS(torch.from_numpy(Y).cuda(), X, contrasts=[1, 5], precision=prec)
+    lib = _lib.lib()
+    outs = []
+    try:
+        for tile in (0, 1, 2):
+            _lib.check(lib.tfce_ols_set_tile(tile))
+            outs.append(model.t(perms).cpu().numpy())
+    finally:
+        _lib.check(lib.tfce_ols_set_tile(0))
+    for o in outs[1:]:
+        np.testing.assert_array_equal(o, outs[0])

@@ -25,6 +25,7 @@ SIGNATURES = {
     "tfce_adj_create": (c_int, [c_p, c_p, c_i32, c_i64, c_f32, c_f32, c_p]),
     "tfce_adj_create_ex": (c_int, [c_p, c_p, c_i32, c_i64, c_f32, c_f32, c_u32, c_p]),
     "tfce_adj_destroy": (c_int, [c_p]),
+    "tfce_locality_order": (c_int, [c_p, c_p, c_i32, c_p]),
     "tfce_adj_info": (c_int, [c_p, c_p, c_p, c_p]),
     "tfce_run": (c_int, [c_p, c_p, c_p, c_f32, c_int, c_p]),
     "tfce_run_ex": (c_int, [c_p, c_p, c_p, c_f32, c_f32, c_int, c_i32, c_p, c_p, c_p]),
@@ -41,6 +42,8 @@ SIGNATURES = {
     "tfce_ols_model_destroy": (c_int, [c_p]),
     "tfce_ols_info": (c_int, [c_p, c_p, c_p, c_p]),
     "tfce_ols_perm_t": (c_int, [c_p, c_p, c_i32, c_p, c_i64, c_p]),
+    "tfce_ols_set_tile": (c_int, [c_i32]),
+    "tfce_ols_tile": (c_int, []),
     "tfce_fwe_p": (c_int, [c_p, c_i32, c_p, c_i64, c_i32, c_p, c_p]),
     "tfce_fwe_p64": (c_int, [c_p, c_i32, c_p, c_i64, c_i32, c_p, c_p]),
     "tfce_ols_direct": (c_int, [c_p, c_i32, c_i64, c_i64, c_p, c_i32, c_p, c_p, c_p, c_p, c_p, c_p, c_p,
@@ -49,10 +52,12 @@ SIGNATURES = {
     "tfce_mlp_param_count": (c_int, []),
     "tfce_mlp_forward": (c_int, [c_p, c_p, c_i32, c_p, c_p]),
     "tfce_mlp_train": (c_int, [c_p, c_p, c_p, c_p, c_p, c_i32, c_p, c_i32, c_i32, c_i32, c_f32, c_f32, c_p, c_p, c_p]),
+    "ols_perm_t": (c_int, [c_p, c_p, c_p, c_i32, c_i32, c_i64, c_i32, c_i32, c_i32, c_p, c_p]),
+    "sobel_perm_z": (c_int, [c_p, c_i32, c_i64, c_p, c_p, c_i32, c_i32, c_p, c_i32, c_i32, c_p, c_p]),
     "tfce_sobel_z": (c_int, [c_p, c_p, c_i64, c_i32, c_f32, c_p, c_p]),
 }
 
-PREC = {"fp32": 0, "bf16x3": 1, "bf16x6": 2}
+PREC = {"fp32": 0, "bf16x3": 1, "bf16x6": 2, "fp16x3": 3}
 
 _lock = threading.Lock()
 _lib = None

@@ -148,3 +148,71 @@ def vertex_density(row_ptr):
     """1 - deg/max(deg) + mean(deg)/max(deg), float32 (float64 arithmetic)."""
     deg = np.diff(np.asarray(row_ptr, dtype=np.int64)).astype(np.float64)
     return np.array((1 - (deg / deg.max()) + (deg.mean() / deg.max())), dtype=np.float32)
+
+
+def locality_order(row_ptr, col):
+    """new_of_old[v]: reverse Cuthill-McKee order of the adjacency, computed
+    by the library's host code (tfce_locality_order)."""
+    from . import _lib
+    row_ptr = np.ascontiguousarray(row_ptr, dtype=np.int32)
+    col = np.ascontiguousarray(col, dtype=np.int32)
+    V = row_ptr.shape[0] - 1
+    out = np.empty(V, dtype=np.int32)
+    lib = _lib.load_library()
+    rc = lib.tfce_locality_order(row_ptr.ctypes.data, col.ctypes.data if col.size else None, V, out.ctypes.data)
+    if rc != 0:
+        raise _lib.TfceHipError(f"tfce_locality_order: {lib.tfce_last_error().decode()}")
+    return out
+
+
+def relabel_csr(row_ptr, col, new_of_old):
+    """The same graph with element v renamed new_of_old[v]; each new list is
+    sorted."""
+    row_ptr = np.asarray(row_ptr, dtype=np.int64)
+    col = np.asarray(col, dtype=np.int64)
+    new_of_old = np.asarray(new_of_old, dtype=np.int64)
+    V = row_ptr.shape[0] - 1
+    src = new_of_old[np.repeat(np.arange(V), np.diff(row_ptr))]
+    dst = new_of_old[col]
+    o = np.lexsort((dst, src))
+    rp = np.zeros(V + 1, dtype=np.int32)
+    np.cumsum(np.bincount(src, minlength=V), out=rp[1:])
+    return rp, dst[o].astype(np.int32)
+
+
+def is_symmetric(row_ptr, col):
+    """Every listed pair (v, u), u != v, is also listed as (u, v)."""
+    row_ptr = np.asarray(row_ptr, dtype=np.int64)
+    col = np.asarray(col, dtype=np.int64)
+    V = row_ptr.shape[0] - 1
+    src = np.repeat(np.arange(V), np.diff(row_ptr))
+    keep = src != col
+    fwd = np.unique(src[keep] * V + col[keep])
+    bwd = np.unique(col[keep] * V + src[keep])
+    return fwd.shape == bwd.shape and bool(np.array_equal(fwd, bwd))
+
+
+def locality_layout(surfaces, offsets, width):
+    """Element layout for the batched sweep: every symmetric surface is
+    relabelled with its locality order (neighbours close in memory); others
+    keep their order (their tie rule depends on the caller's indices).
+    Returns (relabelled CSRs, cols) with cols[new column] = old map column, so
+    maps[:, cols] are the maps in the new layout.  Relabelling a symmetric
+    graph changes no partition, no TFCE value and no maximum."""
+    cols = np.arange(width, dtype=np.int64)
+    out, cache = [], {}
+    for (rp, col), off in zip(surfaces, offsets):
+        key = (id(rp), id(col))
+        if key not in cache:
+            V = len(rp) - 1
+            if is_symmetric(rp, col):
+                new_of_old = locality_order(rp, col)
+                old_of_new = np.empty(V, dtype=np.int64)
+                old_of_new[new_of_old] = np.arange(V)
+                cache[key] = (relabel_csr(rp, col, new_of_old), old_of_new)
+            else:
+                cache[key] = ((rp, col), np.arange(V, dtype=np.int64))
+        csr, old_of_new = cache[key]
+        out.append(csr)
+        cols[off:off + len(old_of_new)] = off + old_of_new
+    return out, cols

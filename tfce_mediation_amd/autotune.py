@@ -1,0 +1,110 @@
+"""Autotuner for the two launch-shape knobs of the batched path.
+
+* the workgroup tile of the split-precision contraction
+  (``tfce_ols_set_tile``: 256 x 128, 512 x 64 or 512 x 128 rows x elements);
+* the number of resident sweep workgroups per compute unit
+  (``tfce_set_occupancy``): more hide more latency, fewer keep more of each
+  slot's scratch in cache.
+
+Each candidate is timed with HIP events on the real workload (one batch of P
+permutations of an engine), the contraction and the sweep separately since
+each knob touches one of them.  Every contraction tile must reproduce the
+default tile's t-values bit for bit (the k order of the sums does not depend
+on the tile); a tile that does not is dropped.  Results are cached in a JSON
+file keyed by device and workload shape.
+"""
+import json
+import os
+
+import numpy as np
+
+from . import _lib
+
+TILES = (0, 1, 2)
+OCCUPANCIES = (2, 3, 4, 5, 6, 8)
+DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "tfce_mediation_amd", "autotune.json")
+
+
+def _time(fn, reps):
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ts.append(e0.elapsed_time(e1))
+    return float(np.median(ts))
+
+
+def workload_key(engine, P):
+    import torch
+    surf = ",".join(str(a.V) for a in engine.surfaces.adjs) if hasattr(engine.surfaces, "adjs") else "?"
+    return "|".join([torch.cuda.get_device_name(0), f"n={engine.n}", f"k={engine.k}", f"V={engine.V}",
+                     f"P={P}", f"prec={engine.ols.precision}", f"surfaces={surf}"])
+
+
+def apply(cfg):
+    lib = _lib.lib()
+    _lib.check(lib.tfce_ols_set_tile(int(cfg["gemm_tile"])))
+    _lib.check(lib.tfce_set_occupancy(int(cfg["tfce_wgs_per_cu"])))
+
+
+def tune(engine, P=256, reps=3, tiles=TILES, occupancies=OCCUPANCIES, cache=DEFAULT_CACHE, seed=0, log=None):
+    """Pick and apply the fastest tile and occupancy for engine's batches of P
+    permutations; returns {"gemm_tile", "tfce_wgs_per_cu", "timings_ms"}."""
+    from . import synth
+    key = workload_key(engine, P)
+    if cache and os.path.exists(cache):
+        try:
+            with open(cache) as f:
+                hit = json.load(f).get(key)
+            if hit:
+                apply(hit)
+                return dict(hit, cached=True)
+        except (OSError, ValueError):
+            pass
+    lib = _lib.lib()
+    perms = synth.permutations(engine.n, P, seed, "cuda")
+    timings = {"gemm": {}, "tfce": {}}
+    ref = None
+    for tile in tiles:
+        _lib.check(lib.tfce_ols_set_tile(tile))
+        t = engine.tvalues(perms)
+        out = t.cpu().numpy()
+        if ref is None:
+            ref = out
+        elif not np.array_equal(out, ref, equal_nan=True):
+            if log:
+                log(f"[autotune] tile {tile} changes the t-values; dropped")
+            continue
+        timings["gemm"][tile] = _time(lambda: engine.tvalues(perms), reps)
+    best_tile = min(timings["gemm"], key=timings["gemm"].get)
+    _lib.check(lib.tfce_ols_set_tile(best_tile))
+    t = engine.tvalues(perms)
+    for occ in occupancies:
+        _lib.check(lib.tfce_set_occupancy(occ))
+        timings["tfce"][occ] = _time(lambda: engine.surfaces.batch(t, 3, vdensity=engine.vdensity), reps)
+    best_occ = min(timings["tfce"], key=timings["tfce"].get)
+    cfg = {"gemm_tile": int(best_tile), "tfce_wgs_per_cu": int(best_occ),
+           "timings_ms": {k: {str(a): round(b, 3) for a, b in v.items()} for k, v in timings.items()}}
+    apply(cfg)
+    if log:
+        log(f"[autotune] {key}: tile {best_tile}, {best_occ} sweep workgroups per CU; {cfg['timings_ms']}")
+    if cache:
+        try:
+            os.makedirs(os.path.dirname(cache), exist_ok=True)
+            data = {}
+            if os.path.exists(cache):
+                with open(cache) as f:
+                    data = json.load(f)
+            data[key] = cfg
+            with open(cache, "w") as f:
+                json.dump(data, f, indent=1)
+        except (OSError, ValueError):
+            pass
+    return dict(cfg, cached=False)

@@ -2,6 +2,7 @@
 // batched permuted t-values.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <map>
 #include <mutex>
@@ -21,6 +22,7 @@ struct tfce_ols_data {
   int64_t V = 0, Npad = 0;
   double* mu = nullptr;
   double* yy = nullptr;
+  float* yinv = nullptr;  // fp16 mode: per-element inverse scale
   float* plane32 = nullptr;
   uint16_t* bf[3] = {nullptr, nullptr, nullptr};
 };
@@ -42,6 +44,7 @@ struct tfce_ols_model {
   double* r0 = nullptr;
   float* Linv1 = nullptr;  // shared factors
   double* dc1 = nullptr;
+  float zscale[kOlsMaxK] = {1, 1, 1, 1, 1, 1, 1, 1};  // fp16 mode: per permuted column
 };
 
 namespace {
@@ -157,7 +160,7 @@ int tfce_ols_data_create(const float* d_Y, int32_t n, int64_t V, int64_t ldy, in
   if (!out || !d_Y) return set_error(TFCE_EINVAL, "NULL argument");
   *out = nullptr;
   if (n < 2 || V < 1 || ldy < V) return set_error(TFCE_EINVAL, "need n >= 2, V >= 1, ldy >= V");
-  if (precision < kPrecFP32 || precision > kPrecBF16x6) return set_error(TFCE_EINVAL, "unknown precision");
+  if (precision < kPrecFP32 || precision > kPrecFP16x3) return set_error(TFCE_EINVAL, "unknown precision");
   tfce_ols_data* d = new tfce_ols_data();
   d->n = n;
   d->Kp = (n + 15) / 16 * 16;
@@ -170,7 +173,8 @@ int tfce_ols_data_create(const float* d_Y, int32_t n, int64_t V, int64_t ldy, in
   const size_t plane = (size_t)d->Kp * d->Npad;
   int rc = 0;
   if ((rc = dalloc(&d->mu, V, "mu")) || (rc = dalloc(&d->yy, V, "yy")) ||
-      (rc = dalloc(&d->plane32, plane, "plane32"))) {
+      (rc = dalloc(&d->plane32, plane, "plane32")) ||
+      (precision == kPrecFP16x3 && (rc = dalloc(&d->yinv, V, "yinv")))) {
     tfce_ols_data_destroy(d);
     return rc;
   }
@@ -188,6 +192,8 @@ int tfce_ols_data_create(const float* d_Y, int32_t n, int64_t V, int64_t ldy, in
   a.Npad = d->Npad;
   a.center = d->center;
   a.pieces = d->pieces;
+  a.f16 = precision == kPrecFP16x3 ? 1 : 0;
+  a.yinv = d->yinv;
   a.mu = d->mu;
   a.yy = d->yy;
   a.plane32 = d->plane32;
@@ -203,7 +209,8 @@ int tfce_ols_data_create(const float* d_Y, int32_t n, int64_t V, int64_t ldy, in
 
 int tfce_ols_data_destroy(tfce_ols_data_t* d) {
   if (!d) return 0;
-  for (void* p : {(void*)d->mu, (void*)d->yy, (void*)d->plane32, (void*)d->bf[0], (void*)d->bf[1], (void*)d->bf[2]})
+  for (void* p : {(void*)d->mu, (void*)d->yy, (void*)d->yinv, (void*)d->plane32, (void*)d->bf[0], (void*)d->bf[1],
+                  (void*)d->bf[2]})
     if (p) (void)hipFree(p);
   delete d;
   return 0;
@@ -269,6 +276,12 @@ int tfce_ols_model_create(const tfce_ols_data_t* data, const double* X, int32_t 
       Z32[i * m + q] = (float)Z[i * m + q];
       colsum[q] += (double)Z32[i * m + q];
     }
+  }
+  // fp16 mode: power of two per permuted column with max|z| * s in [2^13, 2^14)
+  for (int q = 0; q < m; ++q) {
+    float mx = 0.0f;
+    for (int i = 0; i < n; ++i) mx = std::max(mx, std::fabs(Z32[i * m + q]));
+    md->zscale[q] = (mx > 0.0f && std::isfinite(mx)) ? std::ldexp(1.0f, 13 - std::ilogb(mx)) : 1.0f;
   }
   for (int a = 0; a < f; ++a)
     for (int b = 0; b < f; ++b) {
@@ -368,6 +381,16 @@ int tfce_ols_model_destroy(tfce_ols_model_t* m) {
   return 0;
 }
 
+static int g_gemm_tile = 0;
+
+int tfce_ols_set_tile(int32_t tile) {
+  if (tile < 0 || tile >= kGemmTiles) return set_error(TFCE_EINVAL, "tile must be 0, 1 or 2");
+  g_gemm_tile = tile;
+  return 0;
+}
+
+int tfce_ols_tile(void) { return g_gemm_tile; }
+
 int tfce_ols_perm_t(const tfce_ols_model_t* md, const int32_t* d_perm, int32_t P, float* d_t, int64_t ldt,
                     hipStream_t stream) {
   if (!md || !d_t) return set_error(TFCE_EINVAL, "NULL argument");
@@ -376,7 +399,10 @@ int tfce_ols_perm_t(const tfce_ols_model_t* md, const int32_t* d_perm, int32_t P
   if (P < 1) return set_error(TFCE_EINVAL, "P must be >= 1");
   if (ldt < d->V) return set_error(TFCE_EINVAL, "ldt must be >= V");
   const int rpp = md->m <= 4 ? 4 : 8;
-  const int per_tile = kGemmRows / rpp;
+  const int tile = d->precision == kPrecFP32 ? 0 : g_gemm_tile;
+  const int tile_rows = d->precision == kPrecFP32 ? kGemmRows : gemm_tile_rows(tile);
+  const int tile_cols = d->precision == kPrecFP32 ? kGemmCols : gemm_tile_cols(tile);
+  const int per_tile = tile_rows / rpp;
   const int64_t Ppad = ((int64_t)P + per_tile - 1) / per_tile * per_tile;
   const int64_t Mpad = Ppad * rpp;
   const int pieces = d->pieces;
@@ -396,6 +422,8 @@ int tfce_ols_perm_t(const tfce_ols_model_t* md, const int32_t* d_perm, int32_t P
   ba.rpp = rpp;
   ba.Mpad = Mpad;
   ba.pieces = pieces;
+  ba.f16 = d->precision == kPrecFP16x3 ? 1 : 0;
+  for (int q = 0; q < kOlsMaxK; ++q) ba.zscale[q] = md->zscale[q];
   const size_t plane_a = (size_t)d->Kp * Mpad;
   if (pieces == 0) {
     ba.a32 = reinterpret_cast<float*>(base);
@@ -462,9 +490,11 @@ int tfce_ols_perm_t(const tfce_ols_model_t* md, const int32_t* d_perm, int32_t P
   ga.colsumZ = md->colsumZ;
   ga.t = d_t;
   ga.ldt = ldt;
-  ga.n_mtiles = (int32_t)(Mpad / kGemmRows);
-  ga.n_ntiles = (int32_t)(d->Npad / kGemmCols);
-  e = launch_ols_gemm(ga, d->precision, stream);
+  ga.yinv = d->yinv;
+  for (int q = 0; q < kOlsMaxK; ++q) ga.zinv[q] = 1.0f / md->zscale[q];
+  ga.n_mtiles = (int32_t)(Mpad / tile_rows);
+  ga.n_ntiles = (int32_t)(d->Npad / tile_cols);
+  e = launch_ols_gemm(ga, d->precision, tile, stream);
   if (e != hipSuccess) return set_error(-(int)e, "ols gemm launch");
   return 0;
 }

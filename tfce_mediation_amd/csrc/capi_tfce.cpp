@@ -204,6 +204,79 @@ int tfce_adj_create_ex(const int32_t* row_ptr, const int32_t* col, int32_t V, in
   return 0;
 }
 
+// Reverse Cuthill-McKee order, component by component.  Each component starts
+// at a pseudo-peripheral vertex (repeated BFS from a smallest-degree vertex of
+// the last BFS level while the eccentricity grows); BFS visits unvisited
+// neighbours in increasing degree, ties by index.  The graph is taken as
+// given (out-lists), self loops ignored.
+int tfce_locality_order(const int32_t* row_ptr, const int32_t* col, int32_t V, int32_t* new_of_old) {
+  if (V <= 0 || !row_ptr || !new_of_old) return set_error(TFCE_EINVAL, "bad arguments");
+  if (row_ptr[0] != 0) return set_error(TFCE_EADJ, "row_ptr[0] must be 0");
+  for (int32_t v = 0; v < V; ++v)
+    if (row_ptr[v + 1] < row_ptr[v]) return set_error(TFCE_EADJ, "row_ptr must be non-decreasing");
+  if (row_ptr[V] > 0 && !col) return set_error(TFCE_EINVAL, "col is NULL");
+  for (int64_t e = 0; e < row_ptr[V]; ++e)
+    if (col[e] < 0 || col[e] >= V) return set_error(TFCE_EADJ, "adjacency index out of range");
+  std::vector<int32_t> deg(V), order, mark(V, -1), level(V);
+  for (int32_t v = 0; v < V; ++v) deg[v] = row_ptr[v + 1] - row_ptr[v];
+  order.reserve(V);
+  std::vector<char> done(V, 0);
+  std::vector<int32_t> queue(V), nb;
+  int stamp = 0;
+  // BFS over the not-yet-ordered vertices from s; returns the vertices in visit
+  // order (in queue[0, n)) and their BFS levels; sorted=true visits neighbours
+  // by increasing degree
+  auto bfs = [&](int32_t s, bool sorted) {
+    ++stamp;
+    int32_t head = 0, tail = 0;
+    queue[tail++] = s;
+    mark[s] = stamp;
+    level[s] = 0;
+    while (head < tail) {
+      const int32_t v = queue[head++];
+      nb.clear();
+      for (int32_t e = row_ptr[v]; e < row_ptr[v + 1]; ++e) {
+        const int32_t u = col[e];
+        if (u != v && !done[u] && mark[u] != stamp) {
+          mark[u] = stamp;
+          level[u] = level[v] + 1;
+          nb.push_back(u);
+        }
+      }
+      if (sorted)
+        std::sort(nb.begin(), nb.end(), [&](int32_t x, int32_t y) { return deg[x] != deg[y] ? deg[x] < deg[y] : x < y; });
+      for (int32_t u : nb) queue[tail++] = u;
+    }
+    return tail;
+  };
+  for (int32_t s0 = 0; s0 < V; ++s0) {
+    if (done[s0]) continue;
+    // pseudo-peripheral start in s0's component
+    int32_t s = s0;
+    int32_t n = bfs(s, false);
+    for (int32_t v = 0; v < n; ++v)
+      if (deg[queue[v]] < deg[s] || (deg[queue[v]] == deg[s] && queue[v] < s)) s = queue[v];
+    int ecc = -1;
+    for (int iter = 0; iter < 8; ++iter) {
+      n = bfs(s, false);
+      const int e = level[queue[n - 1]];
+      if (e <= ecc) break;
+      ecc = e;
+      int32_t best = -1;
+      for (int32_t i = n - 1; i >= 0 && level[queue[i]] == e; --i)
+        if (best < 0 || deg[queue[i]] < deg[best] || (deg[queue[i]] == deg[best] && queue[i] < best)) best = queue[i];
+      s = best;
+    }
+    n = bfs(s, true);
+    for (int32_t i = 0; i < n; ++i) {
+      done[queue[i]] = 1;
+      order.push_back(queue[i]);
+    }
+  }
+  for (int32_t i = 0; i < V; ++i) new_of_old[order[i]] = V - 1 - i;
+  return 0;
+}
+
 int tfce_adj_create(const int32_t* row_ptr, const int32_t* col, int32_t V, int64_t nnz, float H, float E,
                     tfce_adj_t** out) {
   return tfce_adj_create_ex(row_ptr, col, V, nnz, H, E, 0u, out);

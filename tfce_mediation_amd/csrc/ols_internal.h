@@ -9,6 +9,7 @@ namespace tfcehip {
 constexpr int kPrecFP32 = 0;    // FP32 MFMA (v_mfma_f32_32x32x2_f32)
 constexpr int kPrecBF16x3 = 1;  // hi/lo bf16 split, hi*hi + hi*lo + lo*hi, FP32 accumulate
 constexpr int kPrecBF16x6 = 2;  // three-piece bf16 split, every product of order <= 2
+constexpr int kPrecFP16x3 = 3;  // hi/lo fp16 split of power-of-two scaled operands, three products
 
 constexpr int kOlsMaxK = 8;         // design columns
 constexpr int kOlsMaxContrasts = 8;
@@ -16,7 +17,9 @@ constexpr int kGemmRows = 256;      // rows (permutation x coefficient) per work
 constexpr int kGemmCols = 128;      // elements per workgroup
 constexpr int kColAlign = 1024;     // element padding: 8 XCDs x 128 columns
 
-inline int ols_pieces(int precision) { return precision == kPrecBF16x6 ? 3 : (precision == kPrecBF16x3 ? 2 : 1); }
+inline int ols_pieces(int precision) {
+  return precision == kPrecBF16x6 ? 3 : ((precision == kPrecBF16x3 || precision == kPrecFP16x3) ? 2 : 1);
+}
 
 struct OlsPrepArgs {
   const float* Y;   // [n][ldy]
@@ -25,6 +28,8 @@ struct OlsPrepArgs {
   int64_t V, ldy, Npad;
   int32_t center;
   int32_t pieces;   // bf16 pieces to write (0 = none)
+  int32_t f16;      // pieces are fp16 of y * yscale[v] (power of two per element)
+  float* yinv;      // [V] 1 / yscale (fp16 mode)
   double* mu;       // [V]
   double* yy;       // [V]
   float* plane32;   // [Kp/8][Npad][8] centred float32 (always written)
@@ -63,6 +68,8 @@ struct OlsBuildAArgs {
   int32_t n, m, P, Kp, rpp;
   int64_t Mpad;
   int32_t pieces;
+  int32_t f16;           // pieces are fp16 of z * zscale[j]
+  float zscale[8];       // power of two per permuted column (fp16 mode)
   float* a32;            // [Kp/8][Mpad][8] (FP32 mode)
   uint16_t* abf[3];      // [Kp/8][Mpad][8] bf16 pieces
 };
@@ -92,6 +99,8 @@ struct OlsGemmArgs {
   float* t;              // [n_con][P][ldt]
   int64_t ldt;
   int32_t n_mtiles, n_ntiles;
+  const float* yinv;     // [V] per-element inverse data scale, or null
+  float zinv[8];         // per permuted column inverse design scale (fp16 mode)
 };
 
 constexpr int kOlsDirectMaxN = 2048;  // subjects held in LDS by the FP64 kernel
@@ -121,6 +130,11 @@ hipError_t launch_ols_prep(const OlsPrepArgs& a, hipStream_t s);
 hipError_t launch_ols_fixed(const OlsFixedArgs& a, hipStream_t s);
 hipError_t launch_ols_factors(const OlsFactorArgs& a, hipStream_t s);
 hipError_t launch_ols_build_a(const OlsBuildAArgs& a, hipStream_t s);
-hipError_t launch_ols_gemm(const OlsGemmArgs& a, int precision, hipStream_t s);
+// Workgroup tile of the split contraction: 0 = 256 x 128 (2 x 2 waves),
+// 1 = 512 x 64 (4 x 1 waves), 2 = 512 x 128 (4 x 2 waves).  FP32 uses 256 x 128.
+constexpr int kGemmTiles = 3;
+inline int gemm_tile_rows(int tile) { return tile == 0 ? 256 : 512; }
+inline int gemm_tile_cols(int tile) { return tile == 1 ? 64 : 128; }
+hipError_t launch_ols_gemm(const OlsGemmArgs& a, int precision, int tile, hipStream_t s);
 
 }  // namespace tfcehip

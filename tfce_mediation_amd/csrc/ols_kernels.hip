@@ -16,6 +16,8 @@
 // Y is centred per element first (exact for every coefficient once the model
 // holds an intercept; the intercept's own coefficient gets the mean back), so
 // the bf16 split loses precision relative to the spread of y, not its offset.
+#include <type_traits>
+
 #include "ols_internal.h"
 
 namespace tfcehip {
@@ -23,6 +25,7 @@ namespace tfcehip {
 namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 __device__ __forceinline__ uint16_t bf16_rne(float x) {
@@ -45,6 +48,23 @@ __device__ __forceinline__ void split_bf16(float x, int pieces, uint16_t out[3])
   }
 }
 
+// Split a (scaled) value into fp16 hi and lo: hi = fl16(x), lo = fl16(x - hi);
+// x - hi is exact in float, so hi + lo carries about 22 significant bits.
+__device__ __forceinline__ void split_f16(float x, uint16_t out[3]) {
+  const _Float16 hi = (_Float16)x;
+  const _Float16 lo = (_Float16)(x - (float)hi);
+  out[0] = __builtin_bit_cast(uint16_t, hi);
+  out[1] = __builtin_bit_cast(uint16_t, lo);
+  out[2] = 0;
+}
+
+// Power of two s with max_abs * s in [2^13, 2^14): fp16 operands well inside
+// the normal range; exact to undo.
+__device__ __forceinline__ float f16_scale(float max_abs) {
+  if (!(max_abs > 0.0f) || !(max_abs < INFINITY)) return 1.0f;
+  return ldexpf(1.0f, 13 - ilogbf(max_abs));
+}
+
 __global__ __launch_bounds__(256) void ols_prep_kernel(OlsPrepArgs a) {
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= a.Npad) return;
@@ -53,6 +73,14 @@ __global__ __launch_bounds__(256) void ols_prep_kernel(OlsPrepArgs a) {
   if (valid && a.center) {
     for (int i = 0; i < a.n; ++i) mu += (double)a.Y[(int64_t)i * a.ldy + v];
     mu /= (double)a.n;
+  }
+  float ys = 1.0f;
+  if (a.f16) {
+    float mx = 0.0f;
+    if (valid)
+      for (int i = 0; i < a.n; ++i) mx = fmaxf(mx, fabsf((float)((double)a.Y[(int64_t)i * a.ldy + v] - mu)));
+    ys = f16_scale(mx);
+    if (valid) a.yinv[v] = 1.0f / ys;
   }
   double yy = 0.0;
   const int nkg = a.Kp / 8;
@@ -67,7 +95,10 @@ __global__ __launch_bounds__(256) void ols_prep_kernel(OlsPrepArgs a) {
       vals[j] = y;
       yy += (double)y * (double)y;
       uint16_t s[3];
-      split_bf16(y, a.pieces, s);
+      if (a.f16)
+        split_f16(y * ys, s);
+      else
+        split_bf16(y, a.pieces, s);
       pc[0][j] = s[0];
       pc[1][j] = s[1];
       pc[2][j] = s[2];
@@ -199,7 +230,10 @@ __global__ __launch_bounds__(256) void ols_build_a_kernel(OlsBuildAArgs a) {
     }
     vals[jj] = z;
     uint16_t s[3];
-    split_bf16(z, a.pieces, s);
+    if (a.f16)
+      split_f16(j < 8 ? z * a.zscale[j] : 0.0f, s);
+    else
+      split_bf16(z, a.pieces, s);
     pc[0][jj] = s[0];
     pc[1][jj] = s[1];
     pc[2][jj] = s[2];
@@ -296,7 +330,7 @@ __device__ __forceinline__ void epi_put(float* ep, int ldc, const f32x16 (&acc)[
 // Thread (cl, slot): column cl of the block (global element `col`),
 // permutations slot, slot + step, ... of the workgroup's row tile.
 __device__ __forceinline__ void epi_run(const OlsGemmArgs& a, const float* ep, int ldc, int cl, int64_t col,
-                                        int64_t mrow0, int slot, int step) {
+                                        int64_t mrow0, int rows, int slot, int step) {
   const bool cvalid = col < a.V;
   if (!cvalid) return;
   float gF[kOlsMaxK], hF[kOlsMaxK];
@@ -306,7 +340,8 @@ __device__ __forceinline__ void epi_run(const OlsGemmArgs& a, const float* ep, i
     hF[q] = q < a.f ? a.hF[(int64_t)q * a.V + col] : 0.0f;
   }
   const double r0 = a.r0[col], mu = a.mu[col];
-  const int ppt = kGemmRows / a.rpp;
+  const float yinv = a.yinv ? a.yinv[col] : 1.0f;
+  const int ppt = rows / a.rpp;
   const int p0 = (int)(mrow0 / a.rpp);
   for (int pp = slot; pp < ppt; pp += step) {
     const int p = p0 + pp;
@@ -315,6 +350,7 @@ __device__ __forceinline__ void epi_run(const OlsGemmArgs& a, const float* ep, i
 #pragma unroll
     for (int q = 0; q < kOlsMaxK; ++q) {
       gz[q] = q < a.m ? ep[(pp * a.rpp + q) * ldc + cl] : 0.0f;
+      if (a.yinv) gz[q] = gz[q] * a.zinv[q] * yinv;  // powers of two: exact
       if (!a.centered && q < a.m) gz[q] += (float)(a.colsumZ[q] * mu);
     }
     ols_epilogue(a, p, col, gz, gF, hF, r0, mu);
@@ -379,15 +415,16 @@ __global__ __launch_bounds__(256) void ols_gemm_fp32_kernel(OlsGemmArgs a) {
     __syncthreads();
     epi_put<2, 4>(ep, 32, acc, pass, w * 64, 0, h, r);
     __syncthreads();
-    epi_run(a, ep, 32, tid & 31, n0 + 32 * pass + (tid & 31), (int64_t)mt * kGemmRows, tid >> 5, 8);
+    epi_run(a, ep, 32, tid & 31, n0 + 32 * pass + (tid & 31), (int64_t)mt * kGemmRows, kGemmRows, tid >> 5, 8);
   }
 }
 
-// Split-bf16 contraction staged through LDS.  Workgroup tile 256 rows x 128
-// columns; the four waves form a 2 x 2 grid of 128 x 64 wave tiles (4 x 2
-// MFMA 32x32x16 tiles).  One k step is a pair of k-groups (16 k); its A
-// fragments ([2][256 rows][8] bf16 per plane) and B fragments
-// ([2][128 columns][8]) are copied global -> LDS by the LDS DMA
+// Split-bf16 / split-fp16 contraction staged through LDS.  The waves of a
+// workgroup form a WR x WC grid of 128 x 64 wave tiles (4 x 2 MFMA 32x32x16
+// tiles each), so the workgroup tile is (128 WR) rows x (64 WC) columns; the
+// shape is a launch-time choice (the autotuner's).  One k step is a pair of
+// k-groups (16 k); its A fragments ([2][rows][8] per plane) and B fragments
+// ([2][columns][8]) are copied global -> LDS by the LDS DMA
 // (global_load_lds_dwordx4: each wave instruction moves 64 consecutive
 // 16-byte fragments, landing at consecutive LDS addresses), so every A and B
 // element crosses from L2 once per workgroup instead of once per wave.  Two
@@ -395,49 +432,62 @@ __global__ __launch_bounds__(256) void ols_gemm_fp32_kernel(OlsGemmArgs a) {
 // of step s+1 into the other stage, then runs its MFMAs while that DMA is in
 // flight; one barrier per step separates the DMA of a stage from the reads
 // of the step before.
-template <int NP>
+template <int NP, int WR, int WC>
 struct GemmStage {
-  static constexpr int kA = 2 * kGemmRows * 16;   // bytes of A per plane
-  static constexpr int kB = 2 * kGemmCols * 16;   // bytes of B per plane
+  static constexpr int kRows = 128 * WR, kCols = 64 * WC, kThreads = 64 * WR * WC;
+  static constexpr int kA = 2 * kRows * 16;   // bytes of A per plane
+  static constexpr int kB = 2 * kCols * 16;   // bytes of B per plane
   static constexpr int kPlane = kA + kB;
   static constexpr int kBytes = NP * kPlane;
-  static_assert(kA == 4 * 2 * 1024 && kB == 4 * 1024, "four waves x (2 A + 1 B) 1 KiB copies per plane");
+  static constexpr int kEpi = kRows * 32 * WC * 4;  // epilogue block: rows x (32 WC) floats
+  static constexpr int kSmem = 2 * kBytes > kEpi ? 2 * kBytes : kEpi;
 };
 
-// Wave w copies, per plane, the A rows 64w..64w+63 of both k-groups and the B
-// columns 64 (w & 1)..+63 of k-group w >> 1: three 1 KiB copies per plane.
-template <int NP>
+// The 1 KiB copies of one stage (64 rows or columns of one k-group of one
+// plane each) go round-robin over the waves.
+template <int NP, int WR, int WC>
 __device__ __forceinline__ void gemm_stage_copy(const OlsGemmArgs& a, uint8_t* stage, int kg0, int64_t mrow0,
-                                                int64_t ncol0, int w, int l) {
-  using S = GemmStage<NP>;
+                                                int64_t ncol0, int w) {
+  using S = GemmStage<NP, WR, WC>;
   typedef __attribute__((address_space(3))) void* lds_ptr;
-  const int jb = w >> 1, nc = (w & 1) * 64;
+  constexpr int W = WR * WC, nA = 2 * S::kRows / 64, nB = 2 * S::kCols / 64;
+  const int l = threadIdx.x & 63;
 #pragma unroll
   for (int q = 0; q < NP; ++q) {
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const uint16_t* src = a.abf[q] + ((int64_t)(kg0 + j) * a.Mpad + mrow0 + 64 * w + l) * 8;
-      __builtin_amdgcn_global_load_lds((const void*)src,
-                                       (lds_ptr)(stage + q * S::kPlane + (j * kGemmRows + 64 * w) * 16), 16, 0, 0);
+    for (int c0 = 0; c0 < nA; c0 += W) {
+      const int c = c0 + w;
+      if (c < nA) {
+        const int j = c / (S::kRows / 64), rc = (c % (S::kRows / 64)) * 64;
+        const uint16_t* src = a.abf[q] + ((int64_t)(kg0 + j) * a.Mpad + mrow0 + rc + l) * 8;
+        __builtin_amdgcn_global_load_lds((const void*)src,
+                                         (lds_ptr)(stage + q * S::kPlane + (j * S::kRows + rc) * 16), 16, 0, 0);
+      }
     }
-    const uint16_t* src = a.bbf[q] + ((int64_t)(kg0 + jb) * a.Npad + ncol0 + nc + l) * 8;
-    __builtin_amdgcn_global_load_lds((const void*)src,
-                                     (lds_ptr)(stage + q * S::kPlane + S::kA + (jb * kGemmCols + nc) * 16), 16, 0,
-                                     0);
+#pragma unroll
+    for (int c0 = 0; c0 < nB; c0 += W) {
+      const int c = c0 + w;
+      if (c < nB) {
+        const int j = c / (S::kCols / 64), nc = (c % (S::kCols / 64)) * 64;
+        const uint16_t* src = a.bbf[q] + ((int64_t)(kg0 + j) * a.Npad + ncol0 + nc + l) * 8;
+        __builtin_amdgcn_global_load_lds((const void*)src,
+                                         (lds_ptr)(stage + q * S::kPlane + S::kA + (j * S::kCols + nc) * 16), 16,
+                                         0, 0);
+      }
+    }
   }
 }
 
-template <int NP>
-__global__ __launch_bounds__(256) void ols_gemm_bf16_kernel(OlsGemmArgs a) {
-  using S = GemmStage<NP>;
-  constexpr int kSmem = 2 * S::kBytes > kGemmRows * 64 * 4 ? 2 * S::kBytes : kGemmRows * 64 * 4;
-  __shared__ __attribute__((aligned(16))) uint8_t smem[kSmem];
+template <int NP, bool F16, int WR, int WC>
+__global__ __launch_bounds__(64 * WR * WC) void ols_gemm_split_kernel(OlsGemmArgs a) {
+  using S = GemmStage<NP, WR, WC>;
+  __shared__ __attribute__((aligned(16))) uint8_t smem[S::kSmem];
   const int tid = threadIdx.x, l = tid & 63, h = l >> 5, r = l & 31;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int wr = w >> 1, wc = w & 1;
+  const int wr = w / WC, wc = w % WC;
   int mt, nt;
   if (!gemm_tile(a, mt, nt)) return;
-  const int64_t mrow0 = (int64_t)mt * kGemmRows, ncol0 = (int64_t)nt * kGemmCols;
+  const int64_t mrow0 = (int64_t)mt * S::kRows, ncol0 = (int64_t)nt * S::kCols;
   f32x16 acc[4][2];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
@@ -447,50 +497,59 @@ __global__ __launch_bounds__(256) void ols_gemm_bf16_kernel(OlsGemmArgs a) {
       for (int q = 0; q < 16; ++q) acc[i][j][q] = 0.0f;
 
   const int nkg = a.Kp / 8;
-  gemm_stage_copy<NP>(a, smem, 0, mrow0, ncol0, w, l);
+  gemm_stage_copy<NP, WR, WC>(a, smem, 0, mrow0, ncol0, w);
   for (int kg0 = 0, st = 0; kg0 < nkg; kg0 += 2, st ^= 1) {
     __builtin_amdgcn_s_waitcnt(0xF70);  // vmcnt(0): this wave's copies of step kg0 have landed
     __syncthreads();                    // ... and every other wave's
     const uint8_t* cur = smem + st * S::kBytes;
-    bf16x8 A[NP][4], B[NP][2];
+    typedef typename std::conditional<F16, f16x8, bf16x8>::type op8;
+    op8 A[NP][4], B[NP][2];
 #pragma unroll
     for (int q = 0; q < NP; ++q) {
 #pragma unroll
       for (int rt = 0; rt < 4; ++rt)
-        A[q][rt] = *reinterpret_cast<const bf16x8*>(cur + q * S::kPlane +
-                                                     (h * kGemmRows + wr * 128 + 32 * rt + r) * 16);
+        A[q][rt] = *reinterpret_cast<const op8*>(cur + q * S::kPlane +
+                                                  (h * S::kRows + wr * 128 + 32 * rt + r) * 16);
 #pragma unroll
       for (int ct = 0; ct < 2; ++ct)
-        B[q][ct] = *reinterpret_cast<const bf16x8*>(cur + q * S::kPlane + S::kA +
-                                                     (h * kGemmCols + wc * 64 + 32 * ct + r) * 16);
+        B[q][ct] = *reinterpret_cast<const op8*>(cur + q * S::kPlane + S::kA +
+                                                  (h * S::kCols + wc * 64 + 32 * ct + r) * 16);
     }
-    if (kg0 + 2 < nkg) gemm_stage_copy<NP>(a, smem + (st ^ 1) * S::kBytes, kg0 + 2, mrow0, ncol0, w, l);
+    if (kg0 + 2 < nkg) gemm_stage_copy<NP, WR, WC>(a, smem + (st ^ 1) * S::kBytes, kg0 + 2, mrow0, ncol0, w);
 #pragma unroll
     for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
       for (int ct = 0; ct < 2; ++ct) {
         f32x16 c = acc[rt][ct];
-        if constexpr (NP == 3) {
-          c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[2][rt], B[0][ct], c, 0, 0, 0);
-          c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[1][rt], B[1][ct], c, 0, 0, 0);
-          c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[0][rt], B[2][ct], c, 0, 0, 0);
+        if constexpr (F16) {
+          c = __builtin_amdgcn_mfma_f32_32x32x16_f16(A[1][rt], B[0][ct], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_32x32x16_f16(A[0][rt], B[1][ct], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_32x32x16_f16(A[0][rt], B[0][ct], c, 0, 0, 0);
+        } else {
+          if constexpr (NP == 3) {
+            c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[2][rt], B[0][ct], c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[1][rt], B[1][ct], c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[0][rt], B[2][ct], c, 0, 0, 0);
+          }
+          c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[1][rt], B[0][ct], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[0][rt], B[1][ct], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[0][rt], B[0][ct], c, 0, 0, 0);
         }
-        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[1][rt], B[0][ct], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[0][rt], B[1][ct], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[0][rt], B[0][ct], c, 0, 0, 0);
         acc[rt][ct] = c;
       }
   }
-  // epilogue: two passes of 64 columns (32 from each wave column) through a
-  // [256][64] LDS block that reuses the staging buffer
+  // epilogue: two passes (column tile ct of every wave) through an LDS block
+  // [rows][32 WC] that reuses the staging buffer
   float* ep = reinterpret_cast<float*>(smem);
+  constexpr int ldc = 32 * WC;
 #pragma unroll
   for (int pass = 0; pass < 2; ++pass) {
     __syncthreads();
-    epi_put<4, 2>(ep, 64, acc, pass, wr * 128, wc * 32, h, r);
+    epi_put<4, 2>(ep, ldc, acc, pass, wr * 128, wc * 32, h, r);
     __syncthreads();
-    const int cl = tid & 63;
-    epi_run(a, ep, 64, cl, ncol0 + (cl >> 5) * 64 + 32 * pass + (cl & 31), mrow0, tid >> 6, 4);
+    const int cl = tid % ldc;
+    epi_run(a, ep, ldc, cl, ncol0 + (cl >> 5) * 64 + 32 * pass + (cl & 31), mrow0, S::kRows, tid / ldc,
+            S::kThreads / ldc);
   }
 }
 
@@ -519,15 +578,28 @@ hipError_t launch_ols_build_a(const OlsBuildAArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_ols_gemm(const OlsGemmArgs& a, int precision, hipStream_t s) {
+template <int WR, int WC>
+static void launch_split(const OlsGemmArgs& a, int precision, unsigned blocks, hipStream_t s) {
+  const dim3 grid(blocks), block(64 * WR * WC);
+  if (precision == kPrecBF16x3)
+    hipLaunchKernelGGL((ols_gemm_split_kernel<2, false, WR, WC>), grid, block, 0, s, a);
+  else if (precision == kPrecFP16x3)
+    hipLaunchKernelGGL((ols_gemm_split_kernel<2, true, WR, WC>), grid, block, 0, s, a);
+  else
+    hipLaunchKernelGGL((ols_gemm_split_kernel<3, false, WR, WC>), grid, block, 0, s, a);
+}
+
+hipError_t launch_ols_gemm(const OlsGemmArgs& a, int precision, int tile, hipStream_t s) {
   const int64_t ntp = (a.n_ntiles + 7) / 8 * 8;
   const unsigned blocks = (unsigned)(ntp * a.n_mtiles);
   if (precision == kPrecFP32)
     hipLaunchKernelGGL(ols_gemm_fp32_kernel, dim3(blocks), dim3(256), 0, s, a);
-  else if (precision == kPrecBF16x3)
-    hipLaunchKernelGGL(ols_gemm_bf16_kernel<2>, dim3(blocks), dim3(256), 0, s, a);
+  else if (tile == 1)
+    launch_split<4, 1>(a, precision, blocks, s);
+  else if (tile == 2)
+    launch_split<4, 2>(a, precision, blocks, s);
   else
-    hipLaunchKernelGGL(ols_gemm_bf16_kernel<3>, dim3(blocks), dim3(256), 0, s, a);
+    launch_split<2, 2>(a, precision, blocks, s);
   return hipGetLastError();
 }
 

@@ -15,16 +15,32 @@ import numpy as np
 
 from . import synth
 from .ols import PermutedOLS
+from .adjacency import locality_layout
 from .tfce import Adjacency, SurfaceSet, TFCE_ADJ_IMPLICIT_ZERO, TFCE_ADJ_ZERO_IN_SCALE
 
 
 class PermutationEngine:
     def __init__(self, Y, X, surfaces, offsets, H=2.0, E=1.0, contrast=None, precision="bf16x3",
-                 perm_columns=None, vdensity=None, implicit_zero=None, settings=None):
+                 perm_columns=None, vdensity=None, implicit_zero=None, settings=None, locality=True):
         """surfaces: list of CSR (row_ptr, col); offsets: first element of each
-        surface in a map row; settings: optional per-surface (H, E)."""
+        surface in a map row; settings: optional per-surface (H, E).
+        locality: hold the elements of every symmetric surface in the
+        library's locality order (Y's columns are permuted once here); the
+        null maxima do not depend on the element order."""
         import torch
         X = np.asarray(X, dtype=np.float64)
+        self.cols = None
+        if locality:
+            width = int(Y.shape[1])
+            surfaces, cols = locality_layout(surfaces, offsets, width)
+            if not np.array_equal(cols, np.arange(width)):
+                self.cols = cols
+                if isinstance(Y, torch.Tensor):
+                    Y = Y.index_select(1, torch.as_tensor(cols, device=Y.device))
+                else:
+                    Y = np.asarray(Y)[:, cols]
+                if vdensity is not None:
+                    vdensity = np.asarray(vdensity, dtype=np.float32)[cols]
         self.n, self.k = X.shape
         if contrast is None:
             contrast = self.k - 1
@@ -39,12 +55,15 @@ class PermutationEngine:
             h, e = settings[i] if settings is not None else (H, E)
             flags = (TFCE_ADJ_IMPLICIT_ZERO | TFCE_ADJ_ZERO_IN_SCALE) if implicit_zero[i] else 0
             adjs.append(Adjacency(csr, h, e, flags))
+        self.csrs = surfaces  # in the engine's element layout
         self.surfaces = SurfaceSet(adjs, offsets)
         self.vdensity = None
         if vdensity is not None:
             self.vdensity = torch.as_tensor(np.asarray(vdensity, dtype=np.float32)).cuda()
 
     def tvalues(self, perms):
+        """[P, width] t-values in the engine's element layout (column j is
+        map column cols[j] when a locality layout is in use)."""
         return self.ols.t(perms)[0]
 
     def null_batch(self, perms, per_surface=False):
@@ -52,6 +71,33 @@ class PermutationEngine:
         t = self.tvalues(perms)
         per = self.surfaces.batch(t, 3, vdensity=self.vdensity)
         return per if per_surface else per.max(0).values
+
+    def graph(self, P):
+        """One batch of P permutations captured as a HIP graph.  Returns
+        run(perms) -> [P, 2] device tensor (the graph's own output buffer,
+        overwritten by the next replay).  The library's scratch is grown on
+        the capture stream by two eager warm-up batches first, so nothing is
+        allocated while capturing; every launch of the batch is a kernel (no
+        memset nodes)."""
+        import torch
+        s = torch.cuda.Stream()
+        static_perm = torch.arange(self.n, dtype=torch.int32, device="cuda").repeat(P, 1).contiguous()
+        torch.cuda.synchronize()
+        with torch.cuda.stream(s):
+            for _ in range(2):
+                self.null_batch(static_perm)
+        s.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            static_out = self.null_batch(static_perm)
+        torch.cuda.synchronize()
+
+        def run(perms):
+            static_perm.copy_(perms)
+            g.replay()
+            return static_out
+        run.graph = g
+        return run
 
     def run(self, n_perm, batch=256, seed=0, perms=None):
         """Null of n_perm permutations: [n_perm, 2] (host numpy)."""

@@ -206,16 +206,22 @@ class WorkloadNull:
             self.engine = PermutationEngine(Y, X, surfaces, offsets, contrast=X.shape[1] - 1,
                                             precision=precision, settings=settings)
         else:
-            from .randomise import MediationModel
+            from .randomise import MediationModel, _in_layout
             from .tfce import Adjacency, SurfaceSet
             self.n = len(x)
+            surfaces, Y, _ = _in_layout(list(surfaces), offsets, Y)
             self.model = MediationModel(Y, x, y, "M", precision=precision)
             self.surfaces = SurfaceSet([Adjacency(c, s[0], s[1]) for c, s in zip(surfaces, settings)], offsets)
 
-    def values(self, n_perm, seed=0, batch=256):
+    def values(self, n_perm, seed=0, batch=256, log=None):
+        import time
         from . import synth
         out, done = [], 0
+        t0 = tl = time.time()
         while done < n_perm:
+            if log is not None and time.time() - tl > 20:
+                tl = time.time()
+                log(f"[study]   {done}/{n_perm} permutations ({tl - t0:.0f}s)")
             P = min(batch, n_perm - done)
             pb = synth.permutations(self.n, P, seed + done, "cuda")
             if self.kind == "glm":
@@ -227,7 +233,7 @@ class WorkloadNull:
         return np.concatenate(out).astype(np.float64)
 
 
-def random_workload(rng, kind=None, precision="bf16x3"):
+def random_workload(rng, kind=None, precision="bf16x3", log=None):
     """A small random workload: vertex GLM (icosphere order 4-6, one or two
     hemispheres), voxel GLM (ellipsoid mask, 26-connectivity, E = 0.5) or
     vertex mediation; n, smoothness and covariates drawn from rng."""
@@ -235,10 +241,12 @@ def random_workload(rng, kind=None, precision="bf16x3"):
     from . import synth
     from .adjacency import voxel_csr
     if kind is None:
-        kind = rng.choice(["vertex", "voxel", "med"], p=[0.45, 0.3, 0.25])
+        kind = str(rng.choice(["vertex", "voxel", "med"], p=[0.45, 0.3, 0.25]))
     seed = int(rng.integers(1 << 30))
     g = synth._gen(seed, "cuda")
     n = int(rng.integers(40, 301))
+    if log is not None:
+        log(f"[study]   workload {kind}, n={n}")
     if kind in ("vertex", "med"):
         order = int(rng.choice([4, 5, 6], p=[0.2, 0.5, 0.3]))
         csr = synth.icosphere_csr(order)
@@ -299,9 +307,9 @@ def study(n_train=64, n_train_perm=10000, n_eval=4, n_eval_perm=10000, m0=500, d
     train = []
     for i in range(n_train):
         ts = time.time()
-        wl = random_workload(rng)
+        wl = random_workload(rng, log=log)
         tb = time.time()
-        vals = wl.values(n_train_perm, seed=int(rng.integers(1 << 30)))
+        vals = wl.values(n_train_perm, seed=int(rng.integers(1 << 30)), log=log)
         train.append(vals)
         log(f"[study] training null {i + 1}/{n_train}: {wl.kind}, set-up {tb - ts:.1f}s, "
             f"{n_train_perm} permutations {time.time() - tb:.1f}s")
@@ -318,7 +326,7 @@ def study(n_train=64, n_train_perm=10000, n_eval=4, n_eval_perm=10000, m0=500, d
             wl = make(20261300 + 17 * e)
             log(f"[study] {name} #{e}: set-up {time.time() - ts:.1f}s")
             ts = time.time()
-            vals = wl.values(n_eval_perm, seed=1000003 * (e + 1))
+            vals = wl.values(n_eval_perm, seed=1000003 * (e + 1), log=log)
             dt = time.time() - ts
             del wl
             per_perm = 2 if name.startswith("c3") else 1

@@ -28,7 +28,7 @@ import os
 import numpy as np
 
 from . import synth
-from .adjacency import induced_subgraph, to_csr
+from .adjacency import induced_subgraph, locality_layout, to_csr
 from .ols import PermutedOLS
 from .tfce import Adjacency, SurfaceSet, TFCE_ADJ_IMPLICIT_ZERO, TFCE_ADJ_ZERO_IN_SCALE
 
@@ -48,12 +48,34 @@ def permutation_indices(n, perm_numbers, seed=0, blocks=None):
     return out
 
 
-def _hemi_surface(adjacency, bin_mask, H, E):
+def _hemi_csr(adjacency, bin_mask):
+    """(CSR induced on the mask, adjacency flags) of one hemisphere."""
     rp, col = to_csr(adjacency)
     bin_mask = np.asarray(bin_mask, dtype=bool)
-    srp, scol = induced_subgraph(rp, col, bin_mask)
     flags = (TFCE_ADJ_IMPLICIT_ZERO | TFCE_ADJ_ZERO_IN_SCALE) if not bin_mask.all() else 0
-    return Adjacency((srp, scol), H, E, flags)
+    return induced_subgraph(rp, col, bin_mask), flags
+
+
+def _hemi_surface(adjacency, bin_mask, H, E):
+    csr, flags = _hemi_csr(adjacency, bin_mask)
+    return Adjacency(csr, H, E, flags)
+
+
+def _in_layout(csrs, offsets, Y, vd=None):
+    """The surfaces, data columns and vdensity in the locality layout
+    (adjacency.locality_layout): only the null maxima leave these drivers, and
+    they do not depend on the element order."""
+    import torch
+    csrs2, cols = locality_layout(csrs, offsets, int(Y.shape[1]))
+    if np.array_equal(cols, np.arange(len(cols))):
+        return csrs, Y, vd
+    if isinstance(Y, torch.Tensor):
+        Y = Y.index_select(1, torch.as_tensor(cols, device=Y.device))
+    else:
+        Y = np.asarray(Y)[:, cols]
+    if vd is not None:
+        vd = vd[torch.as_tensor(cols, device=vd.device)]
+    return csrs2, Y, vd
 
 
 def _batches(perms, batch):
@@ -85,15 +107,15 @@ def vertex_regress(ny, pred_x, adjac_lh, adjac_rh, bin_mask_lh, bin_mask_rh, opt
         perm_cols = None
         contrasts = list(range(1, k))
     H, E = float(optstfce[0]), float(optstfce[1])
-    lh = _hemi_surface(adjac_lh, bin_mask_lh, H, E)
-    rh = _hemi_surface(adjac_rh, bin_mask_rh, H, E)
+    (lh, fl), (rh, fr) = _hemi_csr(adjac_lh, bin_mask_lh), _hemi_csr(adjac_rh, bin_mask_rh)
     nv_lh = int(np.asarray(bin_mask_lh, bool).sum())
-    surfs = SurfaceSet([lh, rh], [0, nv_lh])
     vd = None
     if not np.isscalar(vdensity_lh) or not np.isscalar(vdensity_rh):
         vl = np.broadcast_to(np.asarray(vdensity_lh, np.float32), (len(bin_mask_lh),))[np.asarray(bin_mask_lh, bool)]
         vr = np.broadcast_to(np.asarray(vdensity_rh, np.float32), (len(bin_mask_rh),))[np.asarray(bin_mask_rh, bool)]
         vd = torch.as_tensor(np.ascontiguousarray(np.concatenate([vl, vr]))).cuda()
+    (lh, rh), ny, vd = _in_layout([lh, rh], [0, nv_lh], ny, vd)
+    surfs = SurfaceSet([Adjacency(lh, H, E, fl), Adjacency(rh, H, E, fr)], [0, nv_lh])
     model = PermutedOLS(ny, X, contrasts=contrasts, perm_columns=perm_cols, precision=precision)
     nulls = {c: [] for c in contrasts}
     for _, pb in _batches(perms, batch):
@@ -118,8 +140,8 @@ def voxel_regress(ny, pred_x, adjac, optstfce, perms, specifyvars=None, out_dir=
         perm_cols, contrasts = list(range(start, stop)), [j + 1 for j in range(stop - start)]
     else:
         perm_cols, contrasts = None, list(range(1, k))
-    adj = Adjacency(adjac, float(optstfce[0]), float(optstfce[1]))
-    surfs = SurfaceSet([adj], [0])
+    (csr,), ny, _ = _in_layout([to_csr(adjac)], [0], ny)
+    surfs = SurfaceSet([Adjacency(csr, float(optstfce[0]), float(optstfce[1]))], [0])
     model = PermutedOLS(ny, X, contrasts=contrasts, perm_columns=perm_cols, precision=precision)
     nulls = {c: [] for c in contrasts}
     for _, pb in _batches(perms, batch):
@@ -183,15 +205,15 @@ def vertex_mediation(y, pred_x, depend_y, medtype, adjac_lh, adjac_rh, bin_mask_
     """Mediation randomise: null [n_perm] of the positive Sobel-z TFCE maxima."""
     import torch
     H, E = float(optstfce[0]), float(optstfce[1])
-    lh = _hemi_surface(adjac_lh, bin_mask_lh, H, E)
-    rh = _hemi_surface(adjac_rh, bin_mask_rh, H, E)
+    (lh, fl), (rh, fr) = _hemi_csr(adjac_lh, bin_mask_lh), _hemi_csr(adjac_rh, bin_mask_rh)
     nv_lh = int(np.asarray(bin_mask_lh, bool).sum())
-    surfs = SurfaceSet([lh, rh], [0, nv_lh])
     vd = None
     if not np.isscalar(vdensity_lh) or not np.isscalar(vdensity_rh):
         vl = np.broadcast_to(np.asarray(vdensity_lh, np.float32), (len(bin_mask_lh),))[np.asarray(bin_mask_lh, bool)]
         vr = np.broadcast_to(np.asarray(vdensity_rh, np.float32), (len(bin_mask_rh),))[np.asarray(bin_mask_rh, bool)]
         vd = torch.as_tensor(np.ascontiguousarray(np.concatenate([vl, vr]))).cuda()
+    (lh, rh), y, vd = _in_layout([lh, rh], [0, nv_lh], y, vd)
+    surfs = SurfaceSet([Adjacency(lh, H, E, fl), Adjacency(rh, H, E, fr)], [0, nv_lh])
     med = MediationModel(y, pred_x, depend_y, medtype, precision=precision)
     out = []
     for _, pb in _batches(perms, batch):
@@ -215,6 +237,10 @@ def mmr(merge_y, pred_x, surfaces, offsets, settings, perms, groups=None, merged
     X = np.column_stack([np.ones(n), pred_x])
     k = X.shape[1]
     contrasts = list(range(1, k))
+    vd = None
+    if vdensity is not None and not np.isscalar(vdensity):
+        vd = torch.as_tensor(np.ascontiguousarray(vdensity, dtype=np.float32)).cuda()
+    surfaces, merge_y, vd = _in_layout([to_csr(c) for c in surfaces], offsets, merge_y, vd)
     model = PermutedOLS(merge_y, X, contrasts=contrasts, precision=precision)
     if groups is None:
         keys = sorted(set(map(tuple, settings)))
@@ -224,9 +250,6 @@ def mmr(merge_y, pred_x, surfaces, offsets, settings, perms, groups=None, merged
         idx = [i for i, gg in enumerate(groups) if gg == g]
         adjs = [Adjacency(surfaces[i], settings[i][0], settings[i][1]) for i in idx]
         sets[g] = (idx, SurfaceSet(adjs, [offsets[i] for i in idx]))
-    vd = None
-    if vdensity is not None and not np.isscalar(vdensity):
-        vd = torch.as_tensor(np.ascontiguousarray(vdensity, dtype=np.float32)).cuda()
     zi = zero_in_image or {}
     res = {}
     for _, pb in _batches(perms, batch):

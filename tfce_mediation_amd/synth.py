@@ -106,22 +106,26 @@ def mesh_smooth(Y, S, passes):
 
 
 def gaussian_smooth3d(vol, fwhm):
-    """vol [n, X, Y, Z] torch float32, separable Gaussian of the given FWHM (voxels)."""
+    """vol [n, X, Y, Z] torch float32, separable Gaussian of the given FWHM
+    (voxels), zero padded.  Written as shifted weighted sums (elementwise
+    kernels only): a library convolution would compile a kernel per new
+    volume shape on first use, which takes tens of seconds."""
     torch = _torch()
     sigma = fwhm / (2.0 * np.sqrt(2.0 * np.log(2.0)))
     r = int(np.ceil(3 * sigma))
     t = np.arange(-r, r + 1, dtype=np.float64)
     k = np.exp(-0.5 * (t / sigma) ** 2)
-    k = torch.tensor(k / k.sum(), dtype=torch.float32, device=vol.device)
+    k = k / k.sum()
     out = vol
     for dim in (1, 2, 3):
-        out = out.movedim(dim, -1)
-        shp = out.shape
-        flat = out.reshape(-1, 1, shp[-1])
-        flat = torch.nn.functional.conv1d(flat, k.view(1, 1, -1), padding=r)
-        out = flat.reshape(shp).movedim(-1, dim)
+        x = out.movedim(dim, -1)
+        xp = torch.nn.functional.pad(x, (r, r))
+        acc = torch.zeros_like(x)
+        L = x.shape[-1]
+        for j in range(2 * r + 1):
+            acc.add_(xp[..., j:j + L], alpha=float(k[j]))
+        out = acc.movedim(-1, dim)
     return out.contiguous()
-
 
 def standardize(Y):
     """Per-element zero mean, unit variance over subjects (torch, dim 0)."""

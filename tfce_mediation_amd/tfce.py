@@ -132,6 +132,7 @@ class SurfaceSet:
         _lib.check(lib.tfce_surfset_order(self.handle, order.ctypes.data))
         self.launch_order = order  # launch position -> caller's index
         self._inv = np.argsort(order)
+        self._inv_dev = {}  # device -> index tensor (made once, outside any graph capture)
         self.n = n
         self.width = int(max(o + a.V for o, a in zip(self.offsets, self.adjs)))
 
@@ -169,7 +170,9 @@ class SurfaceSet:
             _lib.check(_lib.lib().tfce_batch(self.handle, _lib.ptr(maps), stride, P, int(sign_mask),
                                             _lib.ptr(T0), _lib.ptr(vdensity), _lib.ptr(scaled), _lib.ptr(raw),
                                             _lib.ptr(own), _lib.ptr(st), _lib.stream_ptr(stream)))
-        inv = torch.as_tensor(self._inv, device=dev, dtype=torch.long)
+        inv = self._inv_dev.get(str(dev))
+        if inv is None:
+            inv = self._inv_dev[str(dev)] = torch.as_tensor(self._inv, device=dev, dtype=torch.long)
 
         def user(x):
             return None if x is None else x.view(self.n, P, ns).index_select(0, inv)

@@ -1,23 +1,26 @@
 #!/usr/bin/env python
 """Permutation throughput of the TFCE_mediation hot path on one MI355X.
 
-Workload (default): two order-7 icospheres (2 x 163,842 = 327,684 vertices),
-n = 500 subjects, design [1, 4 N(0,1) covariates, binary group] (k = 6),
-contrast on the group column, t two-tailed (+t and -t), TFCE H=2 E=1 with the
-reference step max/100, max over hemispheres per permutation.  Synthetic
-smoothed data with five effect patches, random-init, generated on the GPU.
+Workload (default, c3): two order-7 icospheres (2 x 163,842 = 327,684
+vertices), n = 500 subjects, design [1, 4 N(0,1) covariates, binary group]
+(k = 6), contrast on the group column, t two-tailed (+t and -t), TFCE H=2 E=1
+with the reference step max/100, max over hemispheres per permutation.
+Synthetic smoothed data with five effect patches, generated on the GPU.
+`--config c1|c2|c4|c5` measures the other workloads of the project's table the
+same way (c5: mediation, +z only).
 
 A step is one batch of P = 256 permutations through the whole path:
 permuted-design contraction on MFMA (split-bf16, 3 products) with the fused
-t epilogue, then one persistent TFCE launch over every (hemisphere,
-permutation, sign) item, then the max over hemispheres.  Inputs are resident
-in HBM before the timed region.
+t epilogue, then one persistent TFCE launch over every (surface, permutation,
+sign) item, then the max over surfaces.  Inputs are resident in HBM before the
+timed region.
 
-Printed: one JSON line (rank 0) with the metric, the plain FP32 per-permutation
-GPU path measured the same way, a roofline entry for the dominant kernel
-(TFCE sweep) from HIP events inside the timed region, and the CPU baseline
-(reference TFCE build + the reference's numpy t-value code path) timed on
-this host's cores over a bounded sample.
+Printed: one JSON line with the metric, the plain FP32 per-permutation GPU path
+measured the same way, the same steps replayed from a captured hipGraph, the
+autotuner's launch shapes, a roofline entry for the dominant kernel (TFCE
+sweep) from HIP events inside the timed region, and the CPU baseline
+(reference TFCE build + the reference's numpy t-value code path) timed on this
+host's cores over a bounded sample.
 """
 import argparse
 import json
@@ -32,6 +35,22 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
+N_THR = 100
+
+WORKLOADS = {
+    "c1": ("c1: order-5 icosphere (10,242 vertices), n=100, k=4, contrast=group, TFCE H=2 E=1 dh=max/100, "
+           "+/- t", "synthetic (smoothed Gaussian, one 300-vertex effect patch; random permutations)"),
+    "c2": ("c2: 91x109x91 grid, ellipsoid mask, 26-connectivity, n=200, k=5, contrast=last, TFCE H=2 E=0.5, "
+           "+/- t", "synthetic (Gaussian noise, FWHM 3 voxels, two r=4 effect spheres; random permutations)"),
+    "c3": ("c3: 2 x order-7 icosphere (327,684 vertices), n=500, k=6, contrast=group, TFCE H=2 E=1 "
+           "dh=max/100, +/- t, max over hemispheres",
+           "synthetic (smoothed Gaussian on order-7 icospheres, 5 effect patches; random permutations)"),
+    "c4": ("c4: 40 icospheres of orders 4-7 + 10 ellipsoid voxel masks, n=350, k=4, mesh H=2 E=1, voxel "
+           "H=2 E=0.5, +/- t, max over surfaces", "synthetic (N(0,1) data; random permutations)"),
+    "c5": ("c5: mediation on 2 x order-7 icosphere (327,684 vertices), n=300, type M Aroian z, 3 covariates "
+           "regressed out, TFCE H=2 E=1, +z only, max over hemispheres",
+           "synthetic (smoothed Gaussian mediator with a 1,500-vertex a-path patch; random permutations)"),
+}
 
 
 def log(*a):
@@ -42,7 +61,7 @@ def log(*a):
 # CPU baseline worker (spawned process, single-threaded BLAS)
 # ----------------------------------------------------------------------------
 def _cpu_worker(args):
-    ypath, xpath, ppath, csr_path, perm_ids, H, E = args
+    ypath, xpath, ppath, csr_path, perm_ids, settings, offsets, contrast = args
     import numpy as np
     from oracle import ols as ools
     from oracle import tfce as otfce
@@ -50,8 +69,7 @@ def _cpu_worker(args):
     X = np.load(xpath)
     perms = np.load(ppath)
     z = np.load(csr_path)
-    row_ptr, col = z["row_ptr"], z["col"]
-    Vh = len(row_ptr) - 1
+    csrs = [(z["rp%d" % i], z["col%d" % i]) for i in range(len(settings))]
     use_ref = otfce.have_reference()
     n, k = X.shape
     out = []
@@ -66,16 +84,17 @@ def _cpu_worker(args):
         se = np.zeros((k, Y.shape[1]), dtype=np.float32)
         for j in range(Y.shape[1]):
             se[:, j] = np.sqrt(np.diag(sigma2[j] * invXX))
-        t = (a / se)[k - 1]
+        t = (a / se)[contrast]
         vals = []
         for sgn in (1.0, -1.0):
             best = None
-            for h in range(2):
-                img = np.ascontiguousarray(sgn * t[h * Vh:(h + 1) * Vh], dtype=np.float32)
+            for (rp, col), (H, E), off in zip(csrs, settings, offsets):
+                V = len(rp) - 1
+                img = np.ascontiguousarray(sgn * t[off:off + V], dtype=np.float32)
                 if use_ref:
-                    enh = otfce.ref_tfce(row_ptr, col, img, H, E)
+                    enh = otfce.ref_tfce(rp, col, img, H, E)
                 else:
-                    enh, _, _ = otfce.tfce(row_ptr, col, img, H, E)
+                    enh, _, _ = otfce.tfce(rp, col, img, H, E)
                 m = enh.max() * (img.max() / np.float32(100))
                 best = m if best is None else max(best, m)
             vals.append(float(best))
@@ -83,7 +102,7 @@ def _cpu_worker(args):
     return time.time() - t0, out, use_ref
 
 
-def cpu_baseline(cfg, perms_np, n_workers, per_worker):
+def cpu_baseline(cfg, perms_np, n_workers, per_worker, settings):
     import numpy as np
     tmp = tempfile.mkdtemp(prefix="tfce_cpu_")
     ypath = os.path.join(tmp, "Y.npy")
@@ -93,9 +112,12 @@ def cpu_baseline(cfg, perms_np, n_workers, per_worker):
     ppath = os.path.join(tmp, "perms.npy")
     np.save(ppath, perms_np)
     csr_path = os.path.join(tmp, "csr.npz")
-    np.savez(csr_path, row_ptr=cfg["surfaces"][0][0], col=cfg["surfaces"][0][1])
-    jobs = [(ypath, xpath, ppath, csr_path, list(range(w * per_worker, (w + 1) * per_worker)), cfg["H"], cfg["E"])
-            for w in range(n_workers)]
+    arrays = {}
+    for i, (rp, col) in enumerate(cfg["surfaces"]):
+        arrays["rp%d" % i], arrays["col%d" % i] = rp, col
+    np.savez(csr_path, **arrays)
+    jobs = [(ypath, xpath, ppath, csr_path, list(range(w * per_worker, (w + 1) * per_worker)), settings,
+             list(cfg["offsets"]), cfg["contrast"]) for w in range(n_workers)]
     env_keep = {k: os.environ.get(k) for k in ("OPENBLAS_NUM_THREADS", "OMP_NUM_THREADS", "MKL_NUM_THREADS")}
     for k in env_keep:
         os.environ[k] = "1"
@@ -143,10 +165,42 @@ def _measured_traffic(kernel, workload):
     return None, None
 
 
+def make_workload(name, precision):
+    """(cfg, engine, sign_mask, per-surface (H, E), credited contraction flops per permutation)."""
+    import numpy as np
+    import torch
+    from tfce_mediation_amd import synth
+    from tfce_mediation_amd.engine import MediationEngine, PermutationEngine
+    cfg = getattr(synth, "config%d" % int(name[1]))(device="cuda")
+    n_surf = len(cfg["surfaces"])
+    settings = cfg.get("settings") or [(cfg["H"], cfg["E"])] * n_surf
+    if name == "c5":
+        C = torch.as_tensor(np.column_stack([np.ones(cfg["n"]), cfg["covars"]]), dtype=torch.float64).cuda()
+        M = cfg["M"].double()
+        M = (M - C @ torch.linalg.lstsq(C, M).solution).float().contiguous()
+        cfg["Y"] = M
+        eng = MediationEngine(M, cfg["x"], cfg["y"], cfg["surfaces"], cfg["offsets"], cfg["H"], cfg["E"],
+                              precision=precision)
+        return cfg, eng, 1, settings, 2.0 * (2 + 3) * cfg["n"] * cfg["V"]
+    eng = PermutationEngine(cfg["Y"], cfg["X"], cfg["surfaces"], cfg["offsets"], cfg.get("H", 2.0),
+                            cfg.get("E", 1.0), contrast=cfg["contrast"], precision=precision, settings=settings)
+    return cfg, eng, 3, settings, 2.0 * cfg["X"].shape[1] * cfg["n"] * cfg["V"]
+
+
+def plain_engine(name, cfg, settings, precision="fp32"):
+    from tfce_mediation_amd.engine import MediationEngine, PermutationEngine
+    if name == "c5":
+        return MediationEngine(cfg["Y"], cfg["x"], cfg["y"], cfg["surfaces"], cfg["offsets"], cfg["H"], cfg["E"],
+                               precision=precision)
+    return PermutationEngine(cfg["Y"], cfg["X"], cfg["surfaces"], cfg["offsets"], contrast=cfg["contrast"],
+                             precision=precision, settings=settings)
+
+
 # ----------------------------------------------------------------------------
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--config", default="c3", choices=sorted(WORKLOADS))
     ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=256)
@@ -172,18 +226,17 @@ def main():
     import numpy as np
     import torch
     from tfce_mediation_amd import synth
-    from tfce_mediation_amd.engine import PermutationEngine
     from tfce_mediation_amd.tfce import Adjacency
 
     torch.cuda.set_device(0)
     t_setup = time.time()
-    cfg = synth.config3(device="cuda")
+    name = args.config
+    cfg, eng, sign_mask, settings, flops_per_perm = make_workload(name, args.precision)
+    n_signs = 2 if sign_mask == 3 else 1
     torch.cuda.synchronize()
-    log(f"[bench] data generated in {time.time() - t_setup:.1f}s: V={cfg['V']} n={cfg['n']}")
+    log(f"[bench] {name}: data and engine ready in {time.time() - t_setup:.1f}s: V={cfg['V']} n={cfg['n']} "
+        f"surfaces={len(cfg['surfaces'])}")
     P = args.batch
-    eng = PermutationEngine(cfg["Y"], cfg["X"], cfg["surfaces"], cfg["offsets"], cfg["H"], cfg["E"],
-                            contrast=cfg["contrast"], precision=args.precision)
-    torch.cuda.synchronize()
     n = cfg["n"]
     perm_batches = [synth.permutations(n, P, 1000 + s, "cuda") for s in range(args.warmup + args.steps)]
 
@@ -221,7 +274,7 @@ def main():
         e0.record(stream)
         t = eng.tvalues(perms)
         e1.record(stream)
-        per = eng.surfaces.batch(t, 3)
+        per = eng.surfaces.batch(t, sign_mask, vdensity=eng.vdensity)
         e2.record(stream)
         per.max(0).values
         ev.append((e0, e1, e2))
@@ -251,23 +304,26 @@ def main():
         del run
 
     # ---- plain FP32 per-permutation path ------------------------------------
+    # the reference's call structure: one t map per permutation, then one
+    # CreateAdjSet.run per (sign, surface) and the write_perm_maxTFCE rescale
     plain = None
     if not args.no_plain:
-        eng32 = PermutationEngine(cfg["Y"], cfg["X"], cfg["surfaces"], cfg["offsets"], cfg["H"], cfg["E"],
-                                  contrast=cfg["contrast"], precision="fp32")
-        adjs = [Adjacency(csr, cfg["H"], cfg["E"]) for csr in eng32.csrs]
-        Vh = adjs[0].V
-        enh = torch.empty(Vh, dtype=torch.float32, device="cuda")
+        eng32 = plain_engine(name, cfg, settings)
+        adjs = [Adjacency(csr, h, e) for csr, (h, e) in zip(eng32.csrs, settings)]
+        offs = [int(o) for o in cfg["offsets"]]
+        enh = torch.empty(max(a.V for a in adjs), dtype=torch.float32, device="cuda")
+        signs = (1.0, -1.0) if n_signs == 2 else (1.0,)
 
         def plain_perm(perm):
             t = eng32.tvalues(perm[None])[0]
             vals = []
-            for sgn in (1.0, -1.0):
+            for sgn in signs:
                 best = None
-                for h, adj in enumerate(adjs):
-                    img = (sgn * t[h * Vh:(h + 1) * Vh]).contiguous()
-                    adj.run_device(img, enh, accumulate=False)
-                    m = enh.max() * (img.max() / 100.0)
+                for adj, off in zip(adjs, offs):
+                    img = (sgn * t[off:off + adj.V]).contiguous()
+                    out_s = enh[:adj.V]
+                    adj.run_device(img, out_s, accumulate=False)
+                    m = out_s.max() * (img.max() / 100.0)
                     best = m if best is None else torch.maximum(best, m)
                 vals.append(best)
             return torch.stack(vals)
@@ -283,13 +339,15 @@ def main():
         log(f"[bench] plain fp32 per-permutation path: {plain:.2f} perms/s")
         del eng32
 
-    # ---- CPU baseline -------------------------------------------------------
+    # ---- CPU baseline (GLM workloads) -----------------------------------------
     cpu = None
-    if not args.no_cpu:
-        nw = max(1, min(args.cpu_workers, os.cpu_count() or 1))
+    nw = max(1, min(args.cpu_workers, os.cpu_count() or 1))
+    if name == "c4":
+        nw = min(nw, 4)  # each process holds FP64 temporaries of the 3.5 M-element data (about 10 GB)
+    if not args.no_cpu and name != "c5":
         perms_np = perm_batches[args.warmup].cpu().numpy()
         log(f"[bench] CPU baseline: {nw} single-threaded workers x {args.cpu_perms_per_worker} permutation(s)")
-        cpu = cpu_baseline(cfg, perms_np, nw, args.cpu_perms_per_worker)
+        cpu = cpu_baseline(cfg, perms_np, nw, args.cpu_perms_per_worker, settings)
         log(f"[bench] CPU: aggregate {cpu['aggregate_rate']:.3f} perms/s, per core {cpu['per_core_rate']:.3f}")
         # the same permutations through the GPU path: agreement of the null values
         gpu_vals = eng.null_batch(perm_batches[args.warmup][: cpu["perms"]]).cpu().numpy()
@@ -299,17 +357,21 @@ def main():
         log(f"[bench] GPU vs CPU null values: max rel diff {rel.max():.2e}")
 
     # ---- roofline of the dominant kernel (TFCE sweep) -----------------------
-    rp, col = cfg["surfaces"][0]
-    V, nnz = cfg["V"], 2 * len(col)
-    n_thr = 100
-    bytes_per_sign = n_thr * (4 * (V + 1) + 4 * nnz + 8 * V) + 8 * V   # per permutation, both hemispheres
-    units = P * 2
+    bytes_per_sign = 0
+    for rp, col in cfg["surfaces"]:
+        V_s, nnz_s = len(rp) - 1, len(col)
+        bytes_per_sign += N_THR * (4 * (V_s + 1) + 4 * nnz_s + 8 * V_s) + 8 * V_s
+    units = P * n_signs
     achieved = bytes_per_sign * units / (tfce_ms * 1e-3) / 1e9
-    traffic, traffic_src = _measured_traffic("tfce_batch_kernel<false>", "c3")
-    k = cfg["X"].shape[1]
-    gemm_flops = 2.0 * k * n * V * P
+    traffic, traffic_src = _measured_traffic("tfce_batch_kernel<false>", name)
+    gemm_flops = flops_per_perm * P
+    workload, data = WORKLOADS[name]
+    if name == "c3":
+        metric = "permutations/s at 327,684 vertices, n=500: batched split-BF16 vs plain FP32"
+    else:
+        metric = f"permutations/s ({name}): batched split-BF16 vs plain FP32"
     out = {
-        "metric": "permutations/s at 327,684 vertices, n=500: batched split-BF16 vs plain FP32",
+        "metric": metric,
         "value": round(value, 2),
         "unit": "permutations/s",
         "n_gpus": 1,
@@ -321,10 +383,9 @@ def main():
         "vs_baseline": None,
         "dtype": "bf16x3 (fp32 accumulate; fp64 SSE/t epilogue; fp32 TFCE sums, int32 union-find)"
         if args.precision == "bf16x3" else args.precision,
-        "data": "synthetic (smoothed Gaussian on order-7 icospheres, 5 effect patches; random permutations)",
+        "data": data,
         "config": {
-            "workload": "c3: 2 x order-7 icosphere (327,684 vertices), n=500, k=6, contrast=group, "
-                        "TFCE H=2 E=1 dh=max/100, +/- t, max over hemispheres",
+            "workload": workload,
             "permutations_per_step": P,
             "precision": args.precision,
             "plain_fp32_perms_per_s": None if plain is None else round(plain, 3),
@@ -359,8 +420,8 @@ def main():
             "per_core": round(cpu["per_core_rate"], 4),
             "sample": f"{cpu['perms']} permutations of the same workload, one per single-threaded "
                       f"process: the reference's numpy tval_int code path (float64, per-element "
-                      f"se_of_slope loop) + its own fast_tfce.hpp (oracle/_ref) for +/-t on both "
-                      f"hemispheres + write_perm_maxTFCE_vertex scaling; shell echo excluded",
+                      f"se_of_slope loop) + its own fast_tfce.hpp (oracle/_ref) for +/-t on every "
+                      f"surface + write_perm_maxTFCE scaling; shell echo excluded",
             "max_rel_diff_vs_gpu_null": cpu.get("max_rel_diff_null"),
         },
     }

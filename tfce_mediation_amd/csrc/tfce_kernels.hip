@@ -51,17 +51,20 @@ __device__ __forceinline__ int uf_find(int32_t* parent, int x) {
   return x;
 }
 
-__device__ __forceinline__ void uf_union(int32_t* parent, int a, int b) {
+// Join the sets of a and b, starting from a (an element or any ancestor of it);
+// returns the root the two sets had when joined, a good start for a's next
+// union.  The larger root is hooked under the smaller by CAS; a failed CAS
+// (the root was hooked meanwhile) retries from the new roots.
+__device__ __forceinline__ int uf_link(int32_t* parent, int a, int b) {
   while (true) {
     a = uf_find(parent, a);
     b = uf_find(parent, b);
-    if (a == b) return;
+    if (a == b) return a;
     if (a < b) { int t = a; a = b; b = t; }
     int expected = a;
-    // hook root a (larger id) under b; fails if a stopped being a root
     if (__hip_atomic_compare_exchange_strong(parent + a, &expected, b, __ATOMIC_RELAXED,
                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
-      return;
+      return b;
   }
 }
 
@@ -341,6 +344,7 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
           const int v = order[i];
           const int e0 = S.row_ptr[v], e1 = S.row_ptr[v + 1];
           const float xv = symmetric ? 0.0f : sign * x[v];
+          int ri = i;  // root (or an ancestor) of i's set so far
           for (int e = e0; e < e1; e += 8) {
             int u[8], q[8];
 #pragma unroll
@@ -360,7 +364,7 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
                   take = xu > xv || (xu == xv && u[j] < v);
                 }
               }
-              if (take) uf_union(parent, i, q[j]);
+              if (take) ri = uf_link(parent, ri, q[j]);
             }
           }
         }

@@ -112,3 +112,47 @@ class PermutationEngine:
             out.append(self.null_batch(pb).cpu().numpy())
             done += P
         return np.concatenate(out, 0)
+
+
+class MediationEngine:
+    """Batched mediation randomise (vertex_tfce_mediation_randomise.py:82-91):
+    per permutation of x, the Sobel-family z of the image mediator model
+    (randomise.MediationModel), then TFCE of +z only on every surface and the
+    maximum over surfaces.  Same surface layout and graph capture as
+    PermutationEngine."""
+
+    def __init__(self, M, x, y, surfaces, offsets, H=2.0, E=1.0, medtype="M", alg="aroian",
+                 precision="bf16x3", settings=None, locality=True):
+        import torch
+        from .randomise import MediationModel
+        self.n = len(x)
+        self.cols = None
+        if locality:
+            width = int(M.shape[1])
+            surfaces, cols = locality_layout(surfaces, offsets, width)
+            if not np.array_equal(cols, np.arange(width)):
+                self.cols = cols
+                if isinstance(M, torch.Tensor):
+                    M = M.index_select(1, torch.as_tensor(cols, device=M.device))
+                else:
+                    M = np.asarray(M)[:, cols]
+        self.model = MediationModel(M, x, y, medtype, alg=alg, precision=precision)
+        self.ols = self.model.B
+        self.k = 3
+        self.V = self.model.V
+        self.vdensity = None
+        self.csrs = surfaces
+        adjs = [Adjacency(csr, *(settings[i] if settings is not None else (H, E))) for i, csr in enumerate(surfaces)]
+        self.surfaces = SurfaceSet(adjs, offsets)
+
+    def tvalues(self, perms):
+        """[P, width] Sobel z in the engine's element layout."""
+        return self.model.z(perms)
+
+    def null_batch(self, perms, per_surface=False):
+        """[P, 1] maxima of +z (max over surfaces), or [n_surf, P, 1]."""
+        per = self.surfaces.batch(self.tvalues(perms), 1)
+        return per if per_surface else per.max(0).values
+
+    graph = PermutationEngine.graph
+    run = PermutationEngine.run

@@ -125,47 +125,58 @@ class TailMLP:
 
 class LearnedStop:
     """Stopping rule over a stream of null values (pooled +/- maxima in
-    permutation order)."""
+    permutation order).
 
-    def __init__(self, model, m0=500, n_max=10000, delta=0.01, m_min=1000):
-        self.model, self.m0, self.n_max, self.delta, self.m_min = model, m0, n_max, delta, m_min
+    The model predicts the critical values once, from the first m0 values.
+    From m_min values on, every check_every values, the prediction is compared
+    with the empirical critical values of the values that came after the first
+    m0 (data the prediction has not seen); the loop stops when both levels
+    agree within delta (relative) at `consecutive` checks in a row, and then
+    reports the prediction.  Otherwise the plain empirical result of all n_max
+    values is reported."""
 
-    def decide(self, values):
-        """(stop, thresholds, source) for the values seen so far."""
+    def __init__(self, model, m0=500, n_max=10000, delta=0.0075, m_min=1000, consecutive=4, check_every=512):
+        self.model, self.m0, self.n_max, self.delta = model, m0, n_max, delta
+        self.m_min, self.consecutive, self.check_every = m_min, consecutive, check_every
+
+    def agrees(self, pred, values):
         values = np.asarray(values, dtype=np.float64).ravel()
-        if len(values) >= self.n_max:
-            return True, np.array([critical_value(values[:self.n_max], a) for a in ALPHAS]), "plain"
-        if len(values) < max(self.m0, self.m_min):
-            return False, None, None
-        pred = self.model.predict_critical(values[:self.m0])
-        emp = np.array([critical_value(values, a) for a in ALPHAS])
-        if np.all(np.abs(pred - emp) <= self.delta * emp):
-            return True, pred, "learned"
-        return False, None, None
+        emp = np.array([critical_value(values[self.m0:], a) for a in ALPHAS])
+        return bool(np.all(np.abs(pred - emp) <= self.delta * emp))
 
-    def simulate(self, stream, check_every=512):
+    def plain(self, values):
+        return np.array([critical_value(np.asarray(values, dtype=np.float64)[:self.n_max], a) for a in ALPHAS])
+
+    def simulate(self, stream):
         """Replay a recorded null stream: (values used, thresholds, source)."""
         stream = np.asarray(stream, dtype=np.float64).ravel()
-        m = max(self.m0, self.m_min)
-        while True:
-            stop, thr, src = self.decide(stream[:min(m, self.n_max)])
-            if stop:
-                return min(m, self.n_max), thr, src
-            m += check_every
+        n_max = min(self.n_max, len(stream))
+        pred = self.model.predict_critical(stream[:self.m0])
+        ok, m = 0, max(self.m_min, self.m0 + 2)
+        while m < n_max:
+            ok = ok + 1 if self.agrees(pred, stream[:m]) else 0
+            if ok >= self.consecutive:
+                return m, pred, "learned"
+            m += self.check_every
+        return n_max, self.plain(stream[:n_max]), "plain"
 
     def run(self, engine, batch=256, seed=0):
-        """Live loop on a PermutationEngine: batches until the rule stops."""
+        """Live loop on an engine (null_batch): batches until the rule stops;
+        returns (values used, thresholds, source)."""
         from . import synth
-        values = []
-        done = 0
-        while True:
-            P = batch
-            pb = synth.permutations(engine.n, P, seed + done, "cuda")
+        values, done, pred, ok, next_check = [], 0, None, 0, max(self.m_min, self.m0 + 2)
+        while len(values) < self.n_max:
+            pb = synth.permutations(engine.n, batch, seed + done, "cuda")
             values.extend(engine.null_batch(pb).cpu().numpy().ravel().tolist())
-            done += P
-            stop, thr, src = self.decide(values)
-            if stop:
-                return len(values), thr, src
+            done += batch
+            if pred is None and len(values) >= self.m0:
+                pred = self.model.predict_critical(values[:self.m0])
+            while pred is not None and next_check <= min(len(values), self.n_max - 1):
+                ok = ok + 1 if self.agrees(pred, values[:next_check]) else 0
+                if ok >= self.consecutive:
+                    return next_check, pred, "learned"
+                next_check += self.check_every
+        return self.n_max, self.plain(values), "plain"
 
 
 def generate_nulls(make_config, seeds, n_values, batch=256, precision="bf16x3"):
@@ -244,11 +255,11 @@ def random_workload(rng, kind=None, precision="bf16x3", log=None):
         kind = str(rng.choice(["vertex", "voxel", "med"], p=[0.45, 0.3, 0.25]))
     seed = int(rng.integers(1 << 30))
     g = synth._gen(seed, "cuda")
-    n = int(rng.integers(40, 301))
+    n = int(rng.integers(40, 501))
     if log is not None:
         log(f"[study]   workload {kind}, n={n}")
     if kind in ("vertex", "med"):
-        order = int(rng.choice([4, 5, 6], p=[0.2, 0.5, 0.3]))
+        order = int(rng.choice([4, 5, 6, 7], p=[0.1, 0.3, 0.3, 0.3]))
         csr = synth.icosphere_csr(order)
         Vh = csr[0].shape[0] - 1
         hemis = int(rng.choice([1, 2]))
@@ -296,8 +307,8 @@ def c5_workload(seed, precision="bf16x3"):
                         precision=precision)
 
 
-def study(n_train=64, n_train_perm=10000, n_eval=4, n_eval_perm=10000, m0=500, delta=0.01, steps=4000, seed=0,
-          log=print):
+def study(n_train=160, n_train_perm=10000, n_eval=6, n_eval_perm=10000, m0=500, delta=0.0075, steps=6000, seed=0,
+          log=print, streams=None):
     """Train on random small workloads, then evaluate on the full-size
     two-hemisphere GLM and mediation workloads against 10,000-permutation
     plain runs.  Returns a dict of per-null results."""
@@ -319,7 +330,10 @@ def study(n_train=64, n_train_perm=10000, n_eval=4, n_eval_perm=10000, m0=500, d
     losses = model.train(X, Y, steps=steps, lr=2e-3)
     log(f"[study] trained on {len(X)} examples: loss {losses[:50].mean():.4f} -> {losses[-200:].mean():.4f}")
     rule = LearnedStop(model, m0=m0, n_max=None, delta=delta)
-    out = {"train_nulls": n_train, "train_perm": n_train_perm, "m0": m0, "delta": delta, "eval": []}
+    out = {"train_nulls": n_train, "train_perm": n_train_perm, "m0": m0, "delta": delta, "eval": [],
+           "rule": dict(m0=m0, delta=delta, m_min=rule.m_min, consecutive=rule.consecutive,
+                        check_every=rule.check_every)}
+    kept = {}
     for name, make in (("c3_glm_327k", c3_workload), ("c5_mediation_327k", c5_workload)):
         for e in range(n_eval):
             ts = time.time()
@@ -330,6 +344,7 @@ def study(n_train=64, n_train_perm=10000, n_eval=4, n_eval_perm=10000, m0=500, d
             dt = time.time() - ts
             del wl
             per_perm = 2 if name.startswith("c3") else 1
+            kept["%s_%d" % (name, e)] = vals.astype(np.float32)
             truth = np.array([critical_value(vals, a) for a in ALPHAS])
             pred = model.predict_critical(vals[:m0])
             emp = np.array([critical_value(vals[:m0], a) for a in ALPHAS])
@@ -345,6 +360,8 @@ def study(n_train=64, n_train_perm=10000, n_eval=4, n_eval_perm=10000, m0=500, d
             log(f"[study] {name} #{e}: pred@{m0} err {np.round(rec['rel_err_pred'], 4)} emp@{m0} err "
                 f"{np.round(rec['rel_err_emp'], 4)}; stop after {rec['stop_perms']} perms ({src}), err "
                 f"{np.round(rec['stop_rel_err'], 4)}")
+    if streams:
+        np.savez(streams, **kept)
     return out
 
 
@@ -352,16 +369,17 @@ if __name__ == "__main__":
     import argparse
     import json
     ap = argparse.ArgumentParser(description="learned-stop study on synthetic workloads")
-    ap.add_argument("--train", type=int, default=64)
+    ap.add_argument("--train", type=int, default=160)
     ap.add_argument("--train-perm", type=int, default=10000)
-    ap.add_argument("--eval", type=int, default=4)
+    ap.add_argument("--eval", type=int, default=6)
     ap.add_argument("--eval-perm", type=int, default=10000)
-    ap.add_argument("--delta", type=float, default=0.01)
-    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--delta", type=float, default=0.0075)
+    ap.add_argument("--steps", type=int, default=6000)
     ap.add_argument("--out", default=None)
+    ap.add_argument("--streams", default=None, help="npz file for the evaluation null streams")
     args = ap.parse_args()
     res = study(args.train, args.train_perm, args.eval, args.eval_perm, delta=args.delta, steps=args.steps,
-                log=lambda *a: print(*a, flush=True))
+                log=lambda *a: print(*a, flush=True), streams=args.streams)
     if args.out:
         with open(args.out, "w") as f:
             json.dump(res, f, indent=1)

@@ -398,13 +398,16 @@ int tfce_ols_perm_t(const tfce_ols_model_t* md, const int32_t* d_perm, int32_t P
   if (!d_perm) P = 1;
   if (P < 1) return set_error(TFCE_EINVAL, "P must be >= 1");
   if (ldt < d->V) return set_error(TFCE_EINVAL, "ldt must be >= V");
-  const int rpp = md->m <= 4 ? 4 : 8;
+  // rows per permutation = permuted columns: a row tile holds floor(rows / m)
+  // whole permutations (the epilogue reads a permutation's rows from LDS, so
+  // they need not align with the 32-row MFMA tiles)
+  const int rpp = md->m > 0 ? md->m : 1;
   const int tile = d->precision == kPrecFP32 ? 0 : g_gemm_tile;
   const int tile_rows = d->precision == kPrecFP32 ? kGemmRows : gemm_tile_rows(tile);
   const int tile_cols = d->precision == kPrecFP32 ? kGemmCols : gemm_tile_cols(tile);
   const int per_tile = tile_rows / rpp;
-  const int64_t Ppad = ((int64_t)P + per_tile - 1) / per_tile * per_tile;
-  const int64_t Mpad = Ppad * rpp;
+  const int64_t n_mtiles = ((int64_t)P + per_tile - 1) / per_tile;
+  const int64_t Mpad = n_mtiles * tile_rows;
   const int pieces = d->pieces;
   const size_t abytes = (size_t)d->Kp * Mpad * (pieces ? 2 * pieces : 4);
   const size_t fbytes = md->shared ? 0 : (size_t)P * (64 * sizeof(float) + kOlsMaxContrasts * sizeof(double));
@@ -420,6 +423,7 @@ int tfce_ols_perm_t(const tfce_ols_model_t* md, const int32_t* d_perm, int32_t P
   ba.P = P;
   ba.Kp = d->Kp;
   ba.rpp = rpp;
+  ba.tile_rows = tile_rows;
   ba.Mpad = Mpad;
   ba.pieces = pieces;
   ba.f16 = d->precision == kPrecFP16x3 ? 1 : 0;
@@ -492,7 +496,7 @@ int tfce_ols_perm_t(const tfce_ols_model_t* md, const int32_t* d_perm, int32_t P
   ga.ldt = ldt;
   ga.yinv = d->yinv;
   for (int q = 0; q < kOlsMaxK; ++q) ga.zinv[q] = 1.0f / md->zscale[q];
-  ga.n_mtiles = (int32_t)(Mpad / tile_rows);
+  ga.n_mtiles = (int32_t)n_mtiles;
   ga.n_ntiles = (int32_t)(d->Npad / tile_cols);
   e = launch_ols_gemm(ga, d->precision, tile, stream);
   if (e != hipSuccess) return set_error(-(int)e, "ols gemm launch");

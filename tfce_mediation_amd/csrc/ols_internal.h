@@ -66,6 +66,7 @@ struct OlsBuildAArgs {
   const float* Z32;      // [n][m] permuted columns, float
   const int32_t* perm;   // [P][n] or null
   int32_t n, m, P, Kp, rpp;
+  int32_t tile_rows;     // a row tile holds tile_rows / rpp whole permutations
   int64_t Mpad;
   int32_t pieces;
   int32_t f16;           // pieces are fp16 of z * zscale[j]
@@ -81,7 +82,7 @@ struct OlsGemmArgs {
   const uint16_t* bbf[3];
   int32_t Kp;
   int64_t Mpad, Npad, V;
-  int32_t rpp;           // rows per permutation (4 or 8)
+  int32_t rpp;           // rows per permutation (= permuted columns, >= 1)
   int32_t P;             // valid permutations
   int32_t n, k, f, m;
   int32_t n_con;

@@ -8,7 +8,7 @@
 // How (my design): the design columns split into fixed ones F (the intercept,
 // never moved by a permutation) and permuted ones Z.  For a batch of P
 // permutations the only large product is Z_p' Y for every p at once - one dense
-// contraction [P*rpp x n] . [n x V] on MFMA, rpp = 4 or 8 rows per permutation.
+// contraction [P*rpp x n] . [n x V] on MFMA, rpp = one row per permuted column.
 // With L the Cholesky factor of the permuted Gram matrix (X_p'X_p = L L'),
 // h = L^-1 X_p'y gives |y - X a|^2 = y'y - |h|^2 and a = L^-T h, so the
 // epilogue needs only h_Z = Linv_ZF g_F + Linv_ZZ g_Z per (permutation, element):
@@ -217,14 +217,17 @@ __global__ __launch_bounds__(256) void ols_build_a_kernel(OlsBuildAArgs a) {
   if (idx >= (int64_t)nkg * a.Mpad) return;
   const int kg = (int)(idx / a.Mpad);
   const int64_t row = idx - (int64_t)kg * a.Mpad;
-  const int p = (int)(row / a.rpp), j = (int)(row % a.rpp);
+  const int per_tile = a.tile_rows / a.rpp;
+  const int64_t rin = row % a.tile_rows;
+  const int p = (int)((row / a.tile_rows) * per_tile + rin / a.rpp), j = (int)(rin % a.rpp);
+  const bool used = rin < (int64_t)per_tile * a.rpp;
   float vals[8];
   uint16_t pc[3][8];
 #pragma unroll
   for (int jj = 0; jj < 8; ++jj) {
     const int i = kg * 8 + jj;
     float z = 0.0f;
-    if (p < a.P && j < a.m && i < a.n) {
+    if (used && p < a.P && j < a.m && i < a.n) {
       const int src = a.perm ? a.perm[(int64_t)p * a.n + i] : i;
       z = (src >= 0 && src < a.n) ? a.Z32[src * a.m + j] : NAN;
     }
@@ -342,7 +345,7 @@ __device__ __forceinline__ void epi_run(const OlsGemmArgs& a, const float* ep, i
   const double r0 = a.r0[col], mu = a.mu[col];
   const float yinv = a.yinv ? a.yinv[col] : 1.0f;
   const int ppt = rows / a.rpp;
-  const int p0 = (int)(mrow0 / a.rpp);
+  const int p0 = (int)(mrow0 / rows) * ppt;
   for (int pp = slot; pp < ppt; pp += step) {
     const int p = p0 + pp;
     if (p >= a.P) break;

@@ -212,6 +212,10 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default contraction tile and sweep occupancy")
+    ap.add_argument("--tile", type=int, default=None,
+                    help="contraction tile (0: 256x128, 1: 512x64, 2: 512x128) instead of autotuning")
+    ap.add_argument("--wgs-per-cu", type=int, default=None,
+                    help="sweep workgroups per CU instead of autotuning")
     ap.add_argument("--no-graph", action="store_true",
                     help="skip the hipGraph replay measurement of the same steps")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -242,7 +246,12 @@ def main():
 
     # ---- launch shapes picked on this workload (not cached: measured here) ----
     tuned = None
-    if not args.no_autotune:
+    if args.tile is not None or args.wgs_per_cu is not None:
+        from tfce_mediation_amd import autotune
+        tuned = {"gemm_tile": 0 if args.tile is None else args.tile,
+                 "tfce_wgs_per_cu": 4 if args.wgs_per_cu is None else args.wgs_per_cu}
+        autotune.apply(tuned)
+    elif not args.no_autotune:
         from tfce_mediation_amd import autotune
         tuned = autotune.tune(eng, P, cache=None, log=log)
 

@@ -478,6 +478,7 @@ int tfce_ols_perm_t(const tfce_ols_model_t* md, const int32_t* d_perm, int32_t P
   ga.k = md->k;
   ga.f = md->f;
   ga.m = md->m;
+  ga.inv_dof = 1.0 / (double)(d->n - md->k);
   ga.n_con = md->n_con;
   for (int c = 0; c < kOlsMaxContrasts; ++c) {
     ga.con[c] = md->con[c];

@@ -85,6 +85,7 @@ struct OlsGemmArgs {
   int32_t rpp;           // rows per permutation (= permuted columns, >= 1)
   int32_t P;             // valid permutations
   int32_t n, k, f, m;
+  double inv_dof;        // 1 / (n - k)
   int32_t n_con;
   int32_t con[kOlsMaxContrasts];
   int32_t con_is_intercept[kOlsMaxContrasts];

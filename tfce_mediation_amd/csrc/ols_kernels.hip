@@ -266,38 +266,46 @@ __device__ __forceinline__ bf16x8 ld_bf16x8(const uint16_t* p) {
 // Epilogue for one (permutation, element): gz holds X_p'y for the permuted
 // columns (FP32 accumulators), everything else comes from per-element and
 // per-permutation tables.
+// UNIFORM: the caller guarantees p is the same in every lane of the wave, so
+// the permutation's factor is read with scalar loads.  SSE, se and t follow
+// tval_int: se = fl32(sqrt(SSE / (n - k) * [(X'X)^-1]_cc)) in FP64
+// (cynumstats.pyx:47-64), t = a / se in FP64, returned as float.
+// FF / MM > 0: the fixed / permuted column counts as compile-time constants
+// (every loop unrolls with constant bounds); 0: read them from the arguments.
+template <bool UNIFORM, int FF, int MM>
 __device__ __forceinline__ void ols_epilogue(const OlsGemmArgs& a, int p, int64_t col, const float* gz,
                                              const float* gF, const float* hF, double r0, double mu) {
-  const int fi = a.shared_factor ? 0 : p;
+  const int f = FF > 0 ? FF : a.f, m = MM > 0 ? MM : a.m, k = f + m;
+  const int fi = a.shared_factor ? 0 : (UNIFORM ? __builtin_amdgcn_readfirstlane(p) : p);
   const float* L = a.Linv + (int64_t)fi * 64;
   float hz[kOlsMaxK];
   double ssz = 0.0;
 #pragma unroll
   for (int i = 0; i < kOlsMaxK; ++i) {
     hz[i] = 0.0f;
-    if (i < a.m) {
-      const int row = a.f + i;
+    if (i < m) {
+      const int row = f + i;
       float s = 0.0f;
 #pragma unroll
       for (int j = 0; j < kOlsMaxK; ++j)
-        if (j < a.f) s = fmaf(L[row * 8 + j], gF[j], s);
+        if (j < f) s = fmaf(L[row * 8 + j], gF[j], s);
 #pragma unroll
       for (int j = 0; j < kOlsMaxK; ++j)
-        if (j <= i) s = fmaf(L[row * 8 + a.f + j], gz[j], s);
+        if (j <= i) s = fmaf(L[row * 8 + f + j], gz[j], s);
       hz[i] = s;
       ssz += (double)s * (double)s;
     }
   }
   double sse = r0 - ssz;
   if (sse < 0.0) sse = 0.0;
-  const double s2 = sse / (double)(a.n - a.k);
+  const double s2 = sse / (double)(a.n - k);
   for (int c = 0; c < a.n_con; ++c) {
     const int ci = a.con[c];
     float ac = 0.0f;
 #pragma unroll
     for (int j = 0; j < kOlsMaxK; ++j) {
-      if (j >= ci && j < a.k) {
-        const float hj = (j < a.f) ? hF[j < kOlsMaxK ? j : 0] : hz[(j - a.f) & (kOlsMaxK - 1)];
+      if (j >= ci && j < k) {
+        const float hj = (j < f) ? hF[j < kOlsMaxK ? j : 0] : hz[(j - f) & (kOlsMaxK - 1)];
         ac = fmaf(L[j * 8 + ci], hj, ac);
       }
     }
@@ -332,8 +340,29 @@ __device__ __forceinline__ void epi_put(float* ep, int ldc, const f32x16 (&acc)[
 
 // Thread (cl, slot): column cl of the block (global element `col`),
 // permutations slot, slot + step, ... of the workgroup's row tile.
-__device__ __forceinline__ void epi_run(const OlsGemmArgs& a, const float* ep, int ldc, int cl, int64_t col,
-                                        int64_t mrow0, int rows, int slot, int step) {
+template <int LDC, int FF, int MM>
+__device__ __forceinline__ void epi_loop(const OlsGemmArgs& a, const float* ep, int cl, int64_t col, int p0, int ppt,
+                                         int slot, int step, const float* gF, const float* hF, double r0, double mu,
+                                         float yinv) {
+  const int m = MM > 0 ? MM : a.m, rpp = MM > 0 ? MM : a.rpp;
+  for (int pp = slot; pp < ppt; pp += step) {
+    const int p = p0 + pp;
+    if (p >= a.P) break;
+    float gz[kOlsMaxK];
+#pragma unroll
+    for (int q = 0; q < kOlsMaxK; ++q) {
+      gz[q] = q < m ? ep[(pp * rpp + q) * LDC + cl] : 0.0f;
+      if (a.yinv) gz[q] = gz[q] * a.zinv[q] * yinv;  // powers of two: exact
+      if (!a.centered && q < m) gz[q] += (float)(a.colsumZ[q] * mu);
+    }
+    // with 64 columns per block a wave's lanes share one permutation
+    ols_epilogue<LDC == 64, FF, MM>(a, p, col, gz, gF, hF, r0, mu);
+  }
+}
+
+template <int LDC>
+__device__ __forceinline__ void epi_run(const OlsGemmArgs& a, const float* ep, int cl, int64_t col, int64_t mrow0,
+                                        int rows, int slot, int step) {
   const bool cvalid = col < a.V;
   if (!cvalid) return;
   float gF[kOlsMaxK], hF[kOlsMaxK];
@@ -346,18 +375,21 @@ __device__ __forceinline__ void epi_run(const OlsGemmArgs& a, const float* ep, i
   const float yinv = a.yinv ? a.yinv[col] : 1.0f;
   const int ppt = rows / a.rpp;
   const int p0 = (int)(mrow0 / rows) * ppt;
-  for (int pp = slot; pp < ppt; pp += step) {
-    const int p = p0 + pp;
-    if (p >= a.P) break;
-    float gz[kOlsMaxK];
-#pragma unroll
-    for (int q = 0; q < kOlsMaxK; ++q) {
-      gz[q] = q < a.m ? ep[(pp * a.rpp + q) * ldc + cl] : 0.0f;
-      if (a.yinv) gz[q] = gz[q] * a.zinv[q] * yinv;  // powers of two: exact
-      if (!a.centered && q < a.m) gz[q] += (float)(a.colsumZ[q] * mu);
+  // the design shapes of the drivers (one fixed intercept column and up to
+  // seven permuted ones, rpp = m) get unrolled copies; others the general loop
+  if (a.f == 1 && a.rpp == a.m) {
+    switch (a.m) {
+      case 1: epi_loop<LDC, 1, 1>(a, ep, cl, col, p0, ppt, slot, step, gF, hF, r0, mu, yinv); return;
+      case 2: epi_loop<LDC, 1, 2>(a, ep, cl, col, p0, ppt, slot, step, gF, hF, r0, mu, yinv); return;
+      case 3: epi_loop<LDC, 1, 3>(a, ep, cl, col, p0, ppt, slot, step, gF, hF, r0, mu, yinv); return;
+      case 4: epi_loop<LDC, 1, 4>(a, ep, cl, col, p0, ppt, slot, step, gF, hF, r0, mu, yinv); return;
+      case 5: epi_loop<LDC, 1, 5>(a, ep, cl, col, p0, ppt, slot, step, gF, hF, r0, mu, yinv); return;
+      case 6: epi_loop<LDC, 1, 6>(a, ep, cl, col, p0, ppt, slot, step, gF, hF, r0, mu, yinv); return;
+      case 7: epi_loop<LDC, 1, 7>(a, ep, cl, col, p0, ppt, slot, step, gF, hF, r0, mu, yinv); return;
+      default: break;
     }
-    ols_epilogue(a, p, col, gz, gF, hF, r0, mu);
   }
+  epi_loop<LDC, 0, 0>(a, ep, cl, col, p0, ppt, slot, step, gF, hF, r0, mu, yinv);
 }
 
 // Block -> tile mapping shared by both contraction kernels: the n_mtiles row
@@ -418,7 +450,7 @@ __global__ __launch_bounds__(256) void ols_gemm_fp32_kernel(OlsGemmArgs a) {
     __syncthreads();
     epi_put<2, 4>(ep, 32, acc, pass, w * 64, 0, h, r);
     __syncthreads();
-    epi_run(a, ep, 32, tid & 31, n0 + 32 * pass + (tid & 31), (int64_t)mt * kGemmRows, kGemmRows, tid >> 5, 8);
+    epi_run<32>(a, ep, tid & 31, n0 + 32 * pass + (tid & 31), (int64_t)mt * kGemmRows, kGemmRows, tid >> 5, 8);
   }
 }
 
@@ -443,7 +475,22 @@ struct GemmStage {
   static constexpr int kPlane = kA + kB;
   static constexpr int kBytes = NP * kPlane;
   static constexpr int kEpi = kRows * 32 * WC * 4;  // epilogue block: rows x (32 WC) floats
-  static constexpr int kSmem = 2 * kBytes > kEpi ? 2 * kBytes : kEpi;
+  // LDS stages: the loop is written for 2 or 3 (two copies in flight behind
+  // the one being read); 3 measured slower on MI355X at C3 (less occupancy,
+  // no gain in overlap), so 2.
+  static constexpr int kStages = 2;
+  static constexpr int kSmem = kStages * kBytes > kEpi ? kStages * kBytes : kEpi;
+  // fewest copies any wave issues per stage (the round-robin leaves the last
+  // waves one short when a chunk count is not a multiple of the wave count)
+  static constexpr int kMinCopies = NP * ((2 * kRows / 64) / (WR * WC) + (2 * kCols / 64) / (WR * WC));
+};
+
+// s_waitcnt immediate for vmcnt(n) with lgkmcnt and expcnt left free (gfx9
+// encoding: vmcnt[3:0] and [15:14], expcnt [6:4], lgkmcnt [11:8]).
+template <int N>
+struct VmCnt {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  static constexpr int kImm = (N & 0xF) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8);
 };
 
 // The 1 KiB copies of one stage (64 rows or columns of one k-group of one
@@ -499,12 +546,28 @@ __global__ __launch_bounds__(64 * WR * WC) void ols_gemm_split_kernel(OlsGemmArg
 #pragma unroll
       for (int q = 0; q < 16; ++q) acc[i][j][q] = 0.0f;
 
-  const int nkg = a.Kp / 8;
-  gemm_stage_copy<NP, WR, WC>(a, smem, 0, mrow0, ncol0, w);
-  for (int kg0 = 0, st = 0; kg0 < nkg; kg0 += 2, st ^= 1) {
-    __builtin_amdgcn_s_waitcnt(0xF70);  // vmcnt(0): this wave's copies of step kg0 have landed
-    __syncthreads();                    // ... and every other wave's
-    const uint8_t* cur = smem + st * S::kBytes;
+  const int nsteps = a.Kp / 16;  // k steps of two k-groups
+  constexpr int D = S::kStages;
+#pragma unroll
+  for (int s = 0; s < D - 1; ++s)
+    if (s < nsteps) gemm_stage_copy<NP, WR, WC>(a, smem + s * S::kBytes, 2 * s, mrow0, ncol0, w);
+  for (int step = 0; step < nsteps; ++step) {
+    // stage `step` has landed once at most the copies of the stages issued
+    // after it are outstanding (counted with the fewest copies per wave, so
+    // a wave with more copies waits a little longer than it must)
+    if (D == 3 && step + 1 < nsteps)
+      __builtin_amdgcn_s_waitcnt(VmCnt<S::kMinCopies>::kImm);
+    else
+      __builtin_amdgcn_s_waitcnt(VmCnt<0>::kImm);
+    // Every wave's copies of this stage have landed (each wave waited for its
+    // own above), and every wave has finished reading the stage the next copy
+    // reuses.  A bare s_barrier: __syncthreads' release fence would also wait
+    // for the copies still in flight.  The empty asm keeps the compiler from
+    // moving LDS reads across it.
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    const uint8_t* cur = smem + (step % D) * S::kBytes;
     typedef typename std::conditional<F16, f16x8, bf16x8>::type op8;
     op8 A[NP][4], B[NP][2];
 #pragma unroll
@@ -518,7 +581,8 @@ __global__ __launch_bounds__(64 * WR * WC) void ols_gemm_split_kernel(OlsGemmArg
         B[q][ct] = *reinterpret_cast<const op8*>(cur + q * S::kPlane + S::kA +
                                                   (h * S::kCols + wc * 64 + 32 * ct + r) * 16);
     }
-    if (kg0 + 2 < nkg) gemm_stage_copy<NP, WR, WC>(a, smem + (st ^ 1) * S::kBytes, kg0 + 2, mrow0, ncol0, w);
+    if (step + D - 1 < nsteps)
+      gemm_stage_copy<NP, WR, WC>(a, smem + ((step + D - 1) % D) * S::kBytes, 2 * (step + D - 1), mrow0, ncol0, w);
 #pragma unroll
     for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
@@ -551,8 +615,8 @@ __global__ __launch_bounds__(64 * WR * WC) void ols_gemm_split_kernel(OlsGemmArg
     epi_put<4, 2>(ep, ldc, acc, pass, wr * 128, wc * 32, h, r);
     __syncthreads();
     const int cl = tid % ldc;
-    epi_run(a, ep, ldc, cl, ncol0 + (cl >> 5) * 64 + 32 * pass + (cl & 31), mrow0, S::kRows, tid / ldc,
-            S::kThreads / ldc);
+    epi_run<ldc>(a, ep, cl, ncol0 + (cl >> 5) * 64 + 32 * pass + (cl & 31), mrow0, S::kRows, tid / ldc,
+                 S::kThreads / ldc);
   }
 }
 

@@ -386,8 +386,13 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
           for (int j = 0; j < kIlp; ++j) {
             const bool hooked = live[j] && r[j] != c[j];
             if (hooked) {
-              __hip_atomic_fetch_add(size + r[j], ld(size + c[j]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-              if (!FULL)
+              // Additions go to final roots only, so a hooked element of this
+              // level still has size 1 and (MAX mode) value 0: nothing to load,
+              // and its value cannot raise the root's.
+              const bool fresh = c[j] >= b0;
+              __hip_atomic_fetch_add(size + r[j], fresh ? 1 : ld(size + c[j]), __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_WORKGROUP);
+              if (!FULL && !fresh)
                 __hip_atomic_fetch_max(reinterpret_cast<int32_t*>(val) + r[j],
                                        ld(reinterpret_cast<const int32_t*>(val) + c[j]), __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_WORKGROUP);

@@ -22,6 +22,8 @@ struct tfce_adj {
   int device = 0;
   int32_t* d_row_ptr = nullptr;
   int32_t* d_col = nullptr;
+  int32_t* d_ell = nullptr;     // [V][ell_width], -1 padded; null for long lists
+  int32_t ell_width = 0;
   double* d_sizepow = nullptr;  // (double)s^E for s in [0, V] when E != 1
   SurfDev* d_surf = nullptr;    // one descriptor, offset 0
 };
@@ -168,6 +170,7 @@ int tfce_adj_create_ex(const int32_t* row_ptr, const int32_t* col, int32_t V, in
   auto cleanup = [&](int code, const std::string& m) {
     if (a->d_row_ptr) (void)hipFree(a->d_row_ptr);
     if (a->d_col) (void)hipFree(a->d_col);
+    if (a->d_ell) (void)hipFree(a->d_ell);
     if (a->d_sizepow) (void)hipFree(a->d_sizepow);
     if (a->d_surf) (void)hipFree(a->d_surf);
     delete a;
@@ -183,6 +186,23 @@ int tfce_adj_create_ex(const int32_t* row_ptr, const int32_t* col, int32_t V, in
     return cleanup(-(int)e, "upload row_ptr");
   if (nnz > 0 && (e = hipMemcpy(a->d_col, col, sizeof(int32_t) * nnz, hipMemcpyHostToDevice)) != hipSuccess)
     return cleanup(-(int)e, "upload col");
+  // Fixed-width neighbour table: the sweep reads an element's neighbours from
+  // one row instead of row_ptr then col (one random line fewer per element).
+  {
+    int32_t width = 0;
+    for (int32_t v = 0; v < V; ++v) width = std::max(width, row_ptr[v + 1] - row_ptr[v]);
+    width = (width + 3) & ~3;  // rows of whole 16-byte loads
+    if (width > 0 && width <= kEllMaxWidth) {
+      std::vector<int32_t> ell((size_t)V * width, -1);
+      for (int32_t v = 0; v < V; ++v)
+        for (int32_t e2 = row_ptr[v]; e2 < row_ptr[v + 1]; ++e2) ell[(size_t)v * width + (e2 - row_ptr[v])] = col[e2];
+      if ((e = hipMalloc(&a->d_ell, sizeof(int32_t) * ell.size())) != hipSuccess)
+        return cleanup(-(int)e, "hipMalloc(ell)");
+      if ((e = hipMemcpy(a->d_ell, ell.data(), sizeof(int32_t) * ell.size(), hipMemcpyHostToDevice)) != hipSuccess)
+        return cleanup(-(int)e, "upload ell");
+      a->ell_width = width;
+    }
+  }
   if (E != 1.0f) {
     if ((e = hipMalloc(&a->d_sizepow, sizeof(double) * ((size_t)V + 1))) != hipSuccess)
       return cleanup(-(int)e, "hipMalloc(sizepow)");
@@ -192,6 +212,8 @@ int tfce_adj_create_ex(const int32_t* row_ptr, const int32_t* col, int32_t V, in
   SurfDev sd;
   sd.row_ptr = a->d_row_ptr;
   sd.col = a->d_col;
+  sd.ell = a->d_ell;
+  sd.ell_width = a->ell_width;
   sd.sizepow = a->d_sizepow;
   sd.offset = 0;
   sd.V = V;
@@ -286,6 +308,7 @@ int tfce_adj_destroy(tfce_adj_t* a) {
   if (!a) return 0;
   (void)hipFree(a->d_row_ptr);
   (void)hipFree(a->d_col);
+  if (a->d_ell) (void)hipFree(a->d_ell);
   if (a->d_sizepow) (void)hipFree(a->d_sizepow);
   (void)hipFree(a->d_surf);
   delete a;
@@ -409,6 +432,8 @@ int tfce_surfset_create(const tfce_adj_t* const* adjs, int32_t n, const int64_t*
     SurfDev sd;
     sd.row_ptr = a->d_row_ptr;
     sd.col = a->d_col;
+    sd.ell = a->d_ell;
+    sd.ell_width = a->ell_width;
     sd.sizepow = a->d_sizepow;
     sd.offset = offsets[idx[k]];
     sd.V = a->V;

@@ -10,6 +10,7 @@ namespace tfcehip {
 constexpr int kMaxLevels = 1024;
 constexpr int32_t kNever = 0x7fffffff;
 constexpr int kTfceThreads = 256;
+constexpr int kEllMaxWidth = 32;  // longest neighbour list kept in the fixed-width table
 
 // Status codes written per work item (>= 0: number of levels visited).
 constexpr int32_t kStatusDegenerate = -2;  // max >= minT with a step <= 0: the reference never returns
@@ -23,6 +24,8 @@ constexpr int32_t kSurfZeroInScale = 4;   // the rescale max includes those zero
 struct SurfDev {
   const int32_t* row_ptr;  // V+1
   const int32_t* col;      // nnz (indices local to the surface)
+  const int32_t* ell;      // [V][ell_width] neighbour lists padded with -1, or null
+  int32_t ell_width;       // largest list length (0: use the CSR)
   const double* sizepow;   // [V+1] (double)s^E, null when E == 1
   int64_t offset;          // first element of this surface inside a map row
   int32_t V;

@@ -268,6 +268,8 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
     float* enhn = (FULL && a.enhn) ? a.enhn + (int64_t)itr.p * a.map_stride + S.offset : nullptr;
     const float* vd = (FULL && a.vdensity) ? a.vdensity + S.offset : nullptr;
     const bool symmetric = (S.flags & kSurfSymmetric) != 0;
+    const int32_t* ell = S.ell;
+    const int W = S.ell_width;
     const int32_t* rec = a.prep + item * kPrepWords;
     const int status = rec[kPrepL];
     const int L = status > 0 ? status : 0;
@@ -342,13 +344,22 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
         // up to 8 neighbours are gathered before the first union
         for (int i = b0 + tid; i < b1; i += kTfceThreads) {
           const int v = order[i];
-          const int e0 = S.row_ptr[v], e1 = S.row_ptr[v + 1];
+          // fixed-width table: one aligned row, no row_ptr indirection
+          const int e0 = ell ? 0 : S.row_ptr[v], e1 = ell ? W : S.row_ptr[v + 1];
+          const int4* row = ell ? reinterpret_cast<const int4*>(ell + (size_t)v * W) : nullptr;
           const float xv = symmetric ? 0.0f : sign * x[v];
           int ri = i;  // root (or an ancestor) of i's set so far
           for (int e = e0; e < e1; e += 8) {
             int u[8], q[8];
+            if (ell) {
+              const int4 lo = row[e >> 2];
+              const int4 hi = (e + 4 < W) ? row[(e >> 2) + 1] : make_int4(-1, -1, -1, -1);
+              u[0] = lo.x; u[1] = lo.y; u[2] = lo.z; u[3] = lo.w;
+              u[4] = hi.x; u[5] = hi.y; u[6] = hi.z; u[7] = hi.w;
+            } else {
 #pragma unroll
-            for (int j = 0; j < 8; ++j) u[j] = (e + j < e1) ? S.col[e + j] : -1;
+              for (int j = 0; j < 8; ++j) u[j] = (e + j < e1) ? S.col[e + j] : -1;
+            }
 #pragma unroll
             for (int j = 0; j < 8; ++j) q[j] = u[j] >= 0 ? pos[u[j]] : kNever;
 #pragma unroll
@@ -366,6 +377,7 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
               }
               if (take) ri = uf_link(parent, ri, q[j]);
             }
+            if (u[7] < 0) break;  // padding reached (the table is -1 padded)
           }
         }
         __syncthreads();

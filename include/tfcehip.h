@@ -148,6 +148,17 @@ int tfce_ols_info(const tfce_ols_model_t *model, int32_t *k, int32_t *n_fixed, i
 int tfce_ols_perm_t(const tfce_ols_model_t *model, const int32_t *d_perm, int32_t P, float *d_t,
                     int64_t ldt, hipStream_t stream);
 
+/* Replaces the ancova branch of the voxel randomise loop
+ * (voxel_tfce_multiple_regression_randomise.py:79-89: calcF of X[perm],
+ * negative F set to 0, square root; calcF is cynumstats.pyx:31-36):
+ * d_f[p*ldf + v] = fl32(sqrt(max(F, 0))) with F = ((tss[v] - SSE)/(k-1)) /
+ * (SSE/(n-k)) for the whole model of X[perm_p] (k >= 2).  d_tss[V] (FP64) is
+ * the caller's total sum of squares, sum_i (y_iv - mean(y))^2 about the grand
+ * mean of every element, as calcF takes it.  The contrasts of the model are
+ * not used. */
+int tfce_ols_perm_f(const tfce_ols_model_t *model, const int32_t *d_perm, int32_t P, const double *d_tss,
+                    float *d_f, int64_t ldf, hipStream_t stream);
+
 /* Workgroup tile of the split-precision contraction (process-wide; the
  * autotuner's knob): 0 = 256 rows x 128 elements (default), 1 = 512 x 64,
  * 2 = 512 x 128.  No counterpart in the reference. */

@@ -391,8 +391,9 @@ int tfce_ols_set_tile(int32_t tile) {
 
 int tfce_ols_tile(void) { return g_gemm_tile; }
 
-int tfce_ols_perm_t(const tfce_ols_model_t* md, const int32_t* d_perm, int32_t P, float* d_t, int64_t ldt,
-                    hipStream_t stream) {
+// t-values (tss == null) or sqrt(max(F, 0)) for P permutations
+static int perm_run(const tfce_ols_model_t* md, const int32_t* d_perm, int32_t P, float* d_t, int64_t ldt,
+                    const double* tss, hipStream_t stream) {
   if (!md || !d_t) return set_error(TFCE_EINVAL, "NULL argument");
   const tfce_ols_data* d = md->data;
   if (!d_perm) P = 1;
@@ -499,9 +500,22 @@ int tfce_ols_perm_t(const tfce_ols_model_t* md, const int32_t* d_perm, int32_t P
   for (int q = 0; q < kOlsMaxK; ++q) ga.zinv[q] = 1.0f / md->zscale[q];
   ga.n_mtiles = (int32_t)n_mtiles;
   ga.n_ntiles = (int32_t)(d->Npad / tile_cols);
+  ga.tss = tss;
   e = launch_ols_gemm(ga, d->precision, tile, stream);
   if (e != hipSuccess) return set_error(-(int)e, "ols gemm launch");
   return 0;
+}
+
+int tfce_ols_perm_t(const tfce_ols_model_t* md, const int32_t* d_perm, int32_t P, float* d_t, int64_t ldt,
+                    hipStream_t stream) {
+  return perm_run(md, d_perm, P, d_t, ldt, nullptr, stream);
+}
+
+int tfce_ols_perm_f(const tfce_ols_model_t* md, const int32_t* d_perm, int32_t P, const double* d_tss,
+                    float* d_f, int64_t ldf, hipStream_t stream) {
+  if (!md || !d_tss) return set_error(TFCE_EINVAL, "NULL argument");
+  if (md->k < 2) return set_error(TFCE_EINVAL, "the F statistic needs k >= 2 design columns");
+  return perm_run(md, d_perm, P, d_f, ldf, d_tss, stream);
 }
 
 int tfce_ols_direct(const float* d_Y, int32_t n, int64_t V, int64_t ldy, const double* X, int32_t k,

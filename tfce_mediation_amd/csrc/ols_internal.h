@@ -103,6 +103,7 @@ struct OlsGemmArgs {
   int32_t n_mtiles, n_ntiles;
   const float* yinv;     // [V] per-element inverse data scale, or null
   float zinv[8];         // per permuted column inverse design scale (fp16 mode)
+  const double* tss;     // [V] total sum of squares: write sqrt(max(F, 0)) to t[p][col]; null: t-values
 };
 
 constexpr int kOlsDirectMaxN = 2048;  // subjects held in LDS by the FP64 kernel

@@ -298,6 +298,13 @@ __device__ __forceinline__ void ols_epilogue(const OlsGemmArgs& a, int p, int64_
   }
   double sse = r0 - ssz;
   if (sse < 0.0) sse = 0.0;
+  if (a.tss) {
+    // F of the whole model against the caller's total sum of squares
+    // (cynumstats.pyx:31-36), as sqrt(max(F, 0)) (voxel randomise :85-87)
+    const double F = ((a.tss[col] - sse) / (double)(k - 1)) / (sse / (double)(a.n - k));
+    a.t[(int64_t)p * a.ldt + col] = (float)(F < 0.0 ? 0.0 : sqrt(F));
+    return;
+  }
   const double s2 = sse / (double)(a.n - k);
   for (int c = 0; c < a.n_con; ++c) {
     const int ci = a.con[c];

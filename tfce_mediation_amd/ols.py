@@ -6,6 +6,9 @@ contrasts for every permutation of a batch, ``[n_contrasts, P, V]`` float32 on
 the device.  Row i of the permuted design is ``X[perm[i]]`` as in
 ``vertex_tfce_multiple_regression_randomise.py:105`` (all columns) or, with
 ``perm_columns``, only the listed columns move (``-v``, :93-97).
+``fstat(perms, tss)`` gives sqrt(max(F, 0)) of the whole model instead (the
+ancova branch, voxel_tfce_multiple_regression_randomise.py:79-89), with
+``total_ss(Y)`` the total sum of squares calcF uses (cynumstats.pyx:31-36).
 """
 import ctypes
 
@@ -83,3 +86,27 @@ class PermutedOLS:
         _lib.check(_lib.lib().tfce_ols_perm_t(self._model, _lib.ptr(perms), P, _lib.ptr(out), self.V,
                                              _lib.stream_ptr(stream)))
         return out
+
+    def fstat(self, perms, tss, out=None, stream=None):
+        """sqrt(max(F, 0)) [P, V] float32 of the model of X[perm] for every
+        permutation of perms [P, n]; tss [V] float64 on the device."""
+        import torch
+        assert perms.is_cuda and perms.dtype == torch.int32 and perms.is_contiguous()
+        assert perms.dim() == 2 and perms.shape[1] == self.n
+        assert tss.is_cuda and tss.dtype == torch.float64 and tss.numel() == self.V and tss.is_contiguous()
+        P = int(perms.shape[0])
+        if out is None:
+            out = torch.empty((P, self.V), dtype=torch.float32, device="cuda")
+        assert out.is_contiguous() and out.shape == (P, self.V)
+        _lib.check(_lib.lib().tfce_ols_perm_f(self._model, _lib.ptr(perms), P, _lib.ptr(tss), _lib.ptr(out),
+                                             self.V, _lib.stream_ptr(stream)))
+        return out
+
+
+def total_ss(Y):
+    """sum_i (y_iv - mean(Y))^2 per element [V] (float64, device): the TSS of
+    calcF (cynumstats.pyx:34), about the grand mean of every element of Y."""
+    import torch
+    Yd = Y if isinstance(Y, torch.Tensor) else torch.as_tensor(np.ascontiguousarray(Y, dtype=np.float32))
+    Yd = Yd.to(device="cuda", dtype=torch.float64)
+    return ((Yd - Yd.mean()) ** 2).sum(0).contiguous()

@@ -1,6 +1,10 @@
 """Drop-in for the permutation-path subset of ``tfce_mediation.pyfunc``.
 
 * ``create_adjac_vertex`` / ``create_adjac_voxel``   pyfunc.py:37-76
+* ``write_vertStat_img`` / ``write_voxelStat_img``     pyfunc.py:80-103
+  (observed TFCE images; nibabel is not a dependency, so images go through a
+  ``save(fname, data, affine)`` hook, by default ``numpy.save`` to the same
+  file name with ``.npy`` in place of ``.mgh`` / ``.nii.gz``)
 * ``write_perm_maxTFCE_vertex`` / ``_voxel``          pyfunc.py:107-126
 * ``calc_sobelz`` / ``calc_indirect``                 pyfunc.py:130-162, 2678-2709
 * ``find_nearest``                                    pyfunc.py:254-261
@@ -10,6 +14,7 @@ files in the same text format as the reference, with Python file I/O instead
 of one ``os.system('echo ...')`` per value.
 """
 import math
+import os
 
 import numpy as np
 
@@ -30,6 +35,80 @@ def _tfce_max_scaled(calc, stat_full, vdensity):
     if vd is None and float(vdensity) != 1.0:
         out = np.float32(out) * vdensity
     return out
+
+
+def save_npy(fname, data, affine=None):
+    """Default image writer: ``numpy.save`` under the image name with its
+    extension (.mgh, .nii, .nii.gz) replaced by .npy; the affine is dropped."""
+    for ext in (".nii.gz", ".mgh", ".nii"):
+        if fname.endswith(ext):
+            fname = fname[: -len(ext)]
+            break
+    np.save(fname + ".npy", np.asarray(data))
+
+
+def _append_max(statline):
+    with open("max_TFCE_contrast_values.csv", "a") as f:
+        f.write(statline + "\n")
+
+
+def _tfce_full(calc, image):
+    enh = np.zeros_like(image)
+    calc.run(image, enh)
+    return enh
+
+
+def write_vertStat_img(statname, vertStat, outdata_mask, affine_mask, surf, hemi, bin_mask, TFCEfunc,
+                       all_vertex, density_corr=1, TFCE=True, save=save_npy):
+    """Observed statistic image of one hemisphere (pyfunc.py:80-92).
+
+    The masked statistic is scattered into a zero image of ``all_vertex``
+    elements; with TFCE the image ``TFCE(stat) * (max(finite masked stat)/100)
+    * density_corr`` is saved as ``<statname>_<surf>_<hemi>_TFCE.mgh`` and its
+    maximum appended to max_TFCE_contrast_values.csv ("%s_%s_%s,%f"); then the
+    statistic image is saved as ``<statname>_<surf>_<hemi>.mgh``.  The product
+    is the reference's numpy expression on the caller's dtypes (a float64
+    statistic gives a float64 scale and product), rounded where it is stored in
+    ``outdata_mask`` ([all_vertex, 1, 1] float32, made when None is passed).
+    Returns (statistic image, scaled TFCE image or None).
+    """
+    vertStat = np.asarray(vertStat)
+    if outdata_mask is None:
+        outdata_mask = np.zeros((all_vertex, 1, 1), dtype=np.float32)
+    vertStat_out = np.zeros(all_vertex).astype(np.float32, order="C")
+    vertStat_out[bin_mask] = vertStat
+    tfce_img = None
+    if TFCE:
+        enh = _tfce_full(TFCEfunc, vertStat_out)
+        outdata_mask[:, 0, 0] = enh * (vertStat[np.isfinite(vertStat)].max() / 100) * density_corr
+        tfce_img = outdata_mask[:, 0, 0].copy()
+        _append_max("%s_%s_%s,%f" % (statname, surf, hemi,
+                                      outdata_mask[np.isfinite(outdata_mask[:, 0, 0])].max()))
+        save("%s_%s_%s_TFCE.mgh" % (statname, surf, hemi), outdata_mask, affine_mask)
+    outdata_mask[:, 0, 0] = vertStat_out
+    save("%s_%s_%s.mgh" % (statname, surf, hemi), outdata_mask, affine_mask)
+    return vertStat_out, tfce_img
+
+
+def write_voxelStat_img(statname, voxelStat, out_path, data_index, affine, TFCEfunc, imgext=".nii.gz",
+                        TFCE=True, save=save_npy):
+    """Observed statistic volume (pyfunc.py:94-103): with TFCE,
+    ``TFCE(stat) * (max(stat)/100)`` (float32) goes into ``out_path[data_index]``,
+    is saved as ``<statname>_TFCE<imgext>`` and its volume maximum appended to
+    max_TFCE_contrast_values.csv ("%s,%f"); then the statistic itself is saved
+    as ``<statname><imgext>``.  Returns (statistic, scaled TFCE or None) over
+    the in-mask voxels."""
+    tfce_vals = None
+    if TFCE:
+        voxelStat_out = np.asarray(voxelStat).astype(np.float32, order="C")
+        enh = _tfce_full(TFCEfunc, voxelStat_out)
+        out_path[data_index] = enh * (voxelStat_out.max() / 100)
+        tfce_vals = np.asarray(out_path[data_index]).copy()
+        save("%s_TFCE%s" % (statname, imgext), out_path, affine)
+        _append_max("%s,%f" % (statname, out_path.max()))
+    out_path[data_index] = voxelStat
+    save("%s%s" % (statname, imgext), out_path, affine)
+    return np.asarray(out_path[data_index]).copy(), tfce_vals
 
 
 def write_perm_maxTFCE_vertex(statname, vertStat, num_vertex, bin_mask_lh, bin_mask_rh, calcTFCE_lh,

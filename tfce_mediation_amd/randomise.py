@@ -29,7 +29,7 @@ import numpy as np
 
 from . import synth
 from .adjacency import induced_subgraph, locality_layout, to_csr
-from .ols import PermutedOLS
+from .ols import PermutedOLS, total_ss
 from .tfce import Adjacency, SurfaceSet, TFCE_ADJ_IMPLICIT_ZERO, TFCE_ADJ_ZERO_IN_SCALE
 
 
@@ -130,12 +130,17 @@ def vertex_regress(ny, pred_x, adjac_lh, adjac_rh, bin_mask_lh, bin_mask_rh, opt
 
 
 def voxel_regress(ny, pred_x, adjac, optstfce, perms, specifyvars=None, out_dir=None, batch=256,
-                  precision="bf16x3"):
-    """Voxel randomise: null rows [n_perm, 2] per contrast."""
+                  precision="bf16x3", ancova=False):
+    """Voxel randomise: null rows [n_perm, 2] per contrast.  With ``ancova``
+    (the -f branch, voxel_tfce_multiple_regression_randomise.py:79-89) the
+    image is sqrt(max(F, 0)) of the whole model of X[perm] and the null is
+    {"fstat": [n_perm]} (positive TFCE only), ``perm_fstat_TFCE_maxVoxel.csv``;
+    the reference runs twice as many permutations in that branch (:80), which
+    is the caller's choice of ``perms`` here."""
     n = ny.shape[0]
     X = np.column_stack([np.ones(n), pred_x])
     k = X.shape[1]
-    if specifyvars is not None:
+    if specifyvars is not None and not ancova:
         start, stop = specifyvars[0], specifyvars[1] + 1
         perm_cols, contrasts = list(range(start, stop)), [j + 1 for j in range(stop - start)]
     else:
@@ -143,6 +148,15 @@ def voxel_regress(ny, pred_x, adjac, optstfce, perms, specifyvars=None, out_dir=
     (csr,), ny, _ = _in_layout([to_csr(adjac)], [0], ny)
     surfs = SurfaceSet([Adjacency(csr, float(optstfce[0]), float(optstfce[1]))], [0])
     model = PermutedOLS(ny, X, contrasts=contrasts, perm_columns=perm_cols, precision=precision)
+    if ancova:
+        tss = total_ss(ny)
+        null = []
+        for _, pb in _batches(perms, batch):
+            null.append(surfs.batch(model.fstat(pb, tss), 1)[0, :, 0].cpu().numpy())
+        nulls = {"fstat": np.concatenate(null, 0)}
+        if out_dir is not None:
+            _write(os.path.join(out_dir, "perm_fstat_TFCE_maxVoxel.csv"), nulls["fstat"], "%1.4f")
+        return nulls
     nulls = {c: [] for c in contrasts}
     for _, pb in _batches(perms, batch):
         t = model.t(pb)

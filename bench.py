@@ -216,6 +216,9 @@ def main():
                     help="contraction tile (0: 256x128, 1: 512x64, 2: 512x128) instead of autotuning")
     ap.add_argument("--wgs-per-cu", type=int, default=None,
                     help="sweep workgroups per CU instead of autotuning")
+    ap.add_argument("--pipeline", action="store_true",
+                    help="run the contraction of step s+1 on a second stream during the sweep of step s "
+                         "(engine.pipeline) instead of one after the other on one stream")
     ap.add_argument("--no-graph", action="store_true",
                     help="skip the hipGraph replay measurement of the same steps")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -271,28 +274,37 @@ def main():
         del src, dst
 
     # ---- timed region -------------------------------------------------------
+    # each step's contraction then its sweep on one stream (--pipeline: the
+    # contraction of step s+1 on a second stream during the sweep of step s)
     stream = torch.cuda.current_stream()
-    ev = []
+    timing = {}
+    timed = perm_batches[args.warmup:]
+    if args.pipeline:
+        eng.pipeline(perm_batches[:args.warmup])  # the second stream's scratch grows here
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for s in range(args.steps):
-        perms = perm_batches[args.warmup + s]
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e2 = torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        t = eng.tvalues(perms)
-        e1.record(stream)
-        per = eng.surfaces.batch(t, sign_mask, vdensity=eng.vdensity)
-        e2.record(stream)
-        per.max(0).values
-        ev.append((e0, e1, e2))
+    if not args.pipeline:
+        for perms in timed:
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e2 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            t = eng.tvalues(perms)
+            e1.record(stream)
+            per = eng.surfaces.batch(t, sign_mask, vdensity=eng.vdensity)
+            e2.record(stream)
+            per.max(0).values
+            timing.setdefault("contraction", []).append((e0, e1))
+            timing.setdefault("sweep", []).append((e1, e2))
+    else:
+        eng.pipeline(timed, timing=timing)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    gemm_ms = float(np.mean([a.elapsed_time(b) for a, b, _ in ev]))
-    tfce_ms = float(np.mean([b.elapsed_time(c) for _, b, c in ev]))
+    gemm_ms = float(np.mean([a.elapsed_time(b) for a, b in timing["contraction"]]))
+    tfce_ms = float(np.mean([a.elapsed_time(b) for a, b in timing["sweep"]]))
     value = args.steps * P / elapsed
-    log(f"[bench] batched {args.precision}: {value:.1f} perms/s  ({elapsed / args.steps * 1e3:.2f} ms/step; "
+    mode = "pipelined" if args.pipeline else "sequential"
+    log(f"[bench] batched {args.precision} ({mode}): {value:.1f} perms/s  ({elapsed / args.steps * 1e3:.2f} ms/step; "
         f"contraction {gemm_ms:.2f} ms, TFCE {tfce_ms:.2f} ms)")
 
     # ---- the same steps replayed from a captured hipGraph ---------------------
@@ -402,6 +414,8 @@ def main():
             "hipgraph_replay_perms_per_s": None if graph_rate is None else round(graph_rate, 1),
             "autotuned": None if tuned is None else {"gemm_tile": ["256x128", "512x64", "512x128"][tuned["gemm_tile"]],
                                                      "tfce_wgs_per_cu": tuned["tfce_wgs_per_cu"]},
+            "schedule": mode + (" (contraction of step s+1 on a second stream during the sweep of step s)"
+                                if args.pipeline else " (contraction then sweep, one stream)"),
             "contraction_ms_per_step": round(gemm_ms, 3),
             "tfce_ms_per_step": round(tfce_ms, 3),
             "contraction_tflops": round(gemm_flops / (gemm_ms * 1e-3) / 1e12, 2),

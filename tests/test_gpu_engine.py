@@ -54,3 +54,35 @@ def test_graph_replay_matches_eager():
         ref = eng.null_batch(perms)
         torch.cuda.synchronize()
         np.testing.assert_array_equal(got.cpu().numpy(), ref.cpu().numpy())
+
+
+def test_pipeline_matches_batch_by_batch():
+    """Contraction of batch b+1 on a second stream during the sweep of batch b:
+    every batch's null equals null_batch on it, including a short last batch,
+    for the t engine (+/-t) and the mediation engine (+z)."""
+    import torch
+    from tfce_mediation_amd import synth
+    from tfce_mediation_amd.engine import MediationEngine, PermutationEngine
+    print("[gpu] engine pipeline", flush=True)
+    cfg = synth.config1(seed=12, n=72, order=5, device="cuda")
+    eng = PermutationEngine(cfg["Y"], cfg["X"], cfg["surfaces"], cfg["offsets"], cfg["H"], cfg["E"],
+                            contrast=cfg["contrast"], precision="bf16x3")
+    x = cfg["X"][:, -1].cpu().numpy() if isinstance(cfg["X"], torch.Tensor) else np.asarray(cfg["X"])[:, -1]
+    rng = np.random.default_rng(4)
+    y = 0.3 * x + rng.standard_normal(len(x))
+    med = MediationEngine(cfg["Y"], x, y, cfg["surfaces"], cfg["offsets"], cfg["H"], cfg["E"],
+                          precision="bf16x3")
+    for e in (eng, med):
+        batches = [synth.permutations(cfg["n"], P, 40 + i, "cuda") for i, P in enumerate((64, 64, 64, 37))]
+        timing = {}
+        outs = e.pipeline(batches, timing=timing)
+        torch.cuda.synchronize()
+        assert len(timing["sweep"]) == len(timing["contraction"]) == 4
+        for pb, got in zip(batches, outs):
+            ref = e.null_batch(pb)
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(got.cpu().numpy(), ref.cpu().numpy())
+        for pipelined in (False, True):
+            full = e.run(sum(int(b.shape[0]) for b in batches), batch=64, perms=torch.cat(batches).contiguous(),
+                         pipelined=pipelined)
+            np.testing.assert_array_equal(full, np.concatenate([o.cpu().numpy() for o in outs]))

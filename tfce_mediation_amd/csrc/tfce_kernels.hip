@@ -137,6 +137,94 @@ __device__ __forceinline__ void find_batch(int32_t* parent, int (&x)[kIlp], cons
   }
 }
 
+// Roots of up to N elements at once (x[j] < 0: none), the loads of the
+// chains interleaved; path splitting as in find_batch.
+template <int N>
+__device__ __forceinline__ void find_many(int32_t* parent, int (&x)[N]) {
+  int prev[N];
+  bool go[N];
+  bool any = false;
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    go[j] = x[j] >= 0;
+    prev[j] = -1;
+    any |= go[j];
+  }
+  while (any) {
+    int p[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) p[j] = go[j] ? ld(parent + x[j]) : 0;
+    any = false;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      if (!go[j]) continue;
+      if (p[j] == x[j]) {
+        go[j] = false;
+      } else {
+        if (prev[j] >= 0) st(parent + prev[j], p[j]);
+        prev[j] = x[j];
+        x[j] = p[j];
+        any = true;
+      }
+    }
+  }
+}
+
+// 3b for one element i (vertex v) and up to 8 of its neighbours u[] (-1: none):
+// gather their positions, find the roots of the joined ones and of i's set
+// (ri, an ancestor) as interleaved chains, then hook every other distinct root
+// under the smallest with independent CASes; a failed CAS (the root was hooked
+// meanwhile) falls back to uf_link.  Returns an ancestor of i's set.
+__device__ __forceinline__ int join_chunk(int32_t* parent, const int32_t* pos, const float* x, const int (&u)[8],
+                                          int i, int v, int ri, int b0, int b1, bool symmetric, float sign,
+                                          float xv) {
+  int q[8], r[9];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) q[j] = u[j] >= 0 ? pos[u[j]] : kNever;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    bool take = q[j] < b1 && q[j] != i;
+    if (take && q[j] >= b0) {
+      if (symmetric) {
+        take = q[j] < i;  // a same-level edge is joined once
+      } else {
+        // listed from v only: the reference uses it iff u was activated
+        // first (descending value, ties by index)
+        const float xu = sign * x[u[j]];
+        take = xu > xv || (xu == xv && u[j] < v);
+      }
+    }
+    r[j] = take ? q[j] : -1;
+  }
+  r[8] = ri;
+  find_many<9>(parent, r);
+  int tgt = r[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j)
+    if (r[j] >= 0 && r[j] < tgt) tgt = r[j];
+  bool need[9], ok[9];
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    need[j] = r[j] >= 0 && r[j] != tgt;
+#pragma unroll
+    for (int k = 0; k < j; ++k)
+      if (r[k] == r[j]) need[j] = false;
+  }
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    ok[j] = true;
+    if (need[j]) {
+      int expected = r[j];  // a root, larger than tgt
+      ok[j] = __hip_atomic_compare_exchange_strong(parent + r[j], &expected, tgt, __ATOMIC_RELAXED,
+                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 9; ++j)
+    if (!ok[j]) tgt = uf_link(parent, r[j], tgt);
+  return tgt;
+}
+
 __device__ __forceinline__ int wave_incl_scan(int v) {
   const int lane = threadIdx.x & 63;
 #pragma unroll
@@ -340,44 +428,52 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
           val[i] = (FULL && enhn) ? enhn[order[i]] : 0.0f;
         }
         __syncthreads();
-        // 3b: join new vertices with their active neighbours; the positions of
-        // up to 8 neighbours are gathered before the first union
+        // 3b: join new vertices with their active neighbours, 8 at a time
+        // (join_chunk).  With the fixed-width table a thread's next element's
+        // first row and the vertex of the one after are loaded one iteration
+        // ahead.
+        const int4 none4 = make_int4(-1, -1, -1, -1);
+        int v_nxt = (b0 + tid < b1) ? order[b0 + tid] : 0;
+        int v_nn = (b0 + tid + kTfceThreads < b1) ? order[b0 + tid + kTfceThreads] : 0;
+        int4 lo_nxt = none4, hi_nxt = none4;
+        if (ell && b0 + tid < b1) {
+          const int4* rn = reinterpret_cast<const int4*>(ell + (size_t)v_nxt * W);
+          lo_nxt = rn[0];
+          if (W > 4) hi_nxt = rn[1];
+        }
         for (int i = b0 + tid; i < b1; i += kTfceThreads) {
-          const int v = order[i];
-          // fixed-width table: one aligned row, no row_ptr indirection
-          const int e0 = ell ? 0 : S.row_ptr[v], e1 = ell ? W : S.row_ptr[v + 1];
-          const int4* row = ell ? reinterpret_cast<const int4*>(ell + (size_t)v * W) : nullptr;
+          const int v = v_nxt;
+          const int4 lo0 = lo_nxt, hi0 = hi_nxt;
+          if (ell && i + kTfceThreads < b1) {
+            const int4* rn = reinterpret_cast<const int4*>(ell + (size_t)v_nn * W);
+            lo_nxt = rn[0];
+            if (W > 4) hi_nxt = rn[1];
+          }
+          v_nxt = v_nn;
+          v_nn = (i + 2 * kTfceThreads < b1) ? order[i + 2 * kTfceThreads] : 0;
           const float xv = symmetric ? 0.0f : sign * x[v];
           int ri = i;  // root (or an ancestor) of i's set so far
-          for (int e = e0; e < e1; e += 8) {
-            int u[8], q[8];
-            if (ell) {
+          if (ell) {
+            int u[8] = {lo0.x, lo0.y, lo0.z, lo0.w, hi0.x, hi0.y, hi0.z, hi0.w};
+            ri = join_chunk(parent, pos, x, u, i, v, ri, b0, b1, symmetric, sign, xv);
+            // rows longer than 8: the table is -1 padded, so a chunk that ends
+            // in padding is the last
+            const int4* row = reinterpret_cast<const int4*>(ell + (size_t)v * W);
+            for (int e = 8; e < W && u[7] >= 0; e += 8) {
               const int4 lo = row[e >> 2];
-              const int4 hi = (e + 4 < W) ? row[(e >> 2) + 1] : make_int4(-1, -1, -1, -1);
+              const int4 hi = (e + 4 < W) ? row[(e >> 2) + 1] : none4;
               u[0] = lo.x; u[1] = lo.y; u[2] = lo.z; u[3] = lo.w;
               u[4] = hi.x; u[5] = hi.y; u[6] = hi.z; u[7] = hi.w;
-            } else {
+              ri = join_chunk(parent, pos, x, u, i, v, ri, b0, b1, symmetric, sign, xv);
+            }
+          } else {
+            const int e0 = S.row_ptr[v], e1 = S.row_ptr[v + 1];
+            for (int e = e0; e < e1; e += 8) {
+              int u[8];
 #pragma unroll
               for (int j = 0; j < 8; ++j) u[j] = (e + j < e1) ? S.col[e + j] : -1;
+              ri = join_chunk(parent, pos, x, u, i, v, ri, b0, b1, symmetric, sign, xv);
             }
-#pragma unroll
-            for (int j = 0; j < 8; ++j) q[j] = u[j] >= 0 ? pos[u[j]] : kNever;
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-              bool take = q[j] < b1 && q[j] != i;
-              if (take && q[j] >= b0) {
-                if (symmetric) {
-                  take = q[j] < i;  // a same-level edge is joined once
-                } else {
-                  // listed from v only: the reference uses it iff u was
-                  // activated first (descending value, ties by index)
-                  const float xu = sign * x[u[j]];
-                  take = xu > xv || (xu == xv && u[j] < v);
-                }
-              }
-              if (take) ri = uf_link(parent, ri, q[j]);
-            }
-            if (u[7] < 0) break;  // padding reached (the table is -1 padded)
           }
         }
         __syncthreads();

@@ -72,6 +72,65 @@ class PermutationEngine:
         per = self.surfaces.batch(t, 3, vdensity=self.vdensity)
         return per if per_surface else per.max(0).values
 
+    sign_mask = 3  # +t and -t
+
+    def _contract(self, perms, out, stream):
+        self.ols.t(perms, out=out[None], stream=stream)
+
+    def pipeline(self, perm_batches, timing=None):
+        """Null maxima of several batches, [P, n_signs] device tensors in batch
+        order, with the contraction of batch b+1 (on a second stream) running
+        while the sweep of batch b runs on the current stream.  Two map
+        buffers alternate; the contraction into a buffer waits for the sweep
+        that last read it.  Same values as null_batch, batch by batch.  With
+        ``timing`` (a dict), HIP events around every contraction (on its
+        stream) and every sweep are appended to timing["contraction"] and
+        timing["sweep"] as (start, end) pairs."""
+        import torch
+        main = torch.cuda.current_stream()
+        side = getattr(self, "_side", None)
+        if side is None:
+            side = self._side = torch.cuda.Stream()
+        P = max(int(p.shape[0]) for p in perm_batches)
+        bufs = [torch.empty((P, self.V), dtype=torch.float32, device="cuda") for _ in range(2)]
+        swept = [None, None]
+        side.wait_stream(main)  # the permutation tensors and buffers exist
+
+        def event(stream):
+            e = torch.cuda.Event(enable_timing=timing is not None)
+            e.record(stream)
+            return e
+
+        def contract(b):
+            pb = perm_batches[b]
+            slot = b % 2
+            out = bufs[slot][:pb.shape[0]]
+            if swept[slot] is not None:
+                side.wait_event(swept[slot])
+            e0 = event(side) if timing is not None else None
+            self._contract(pb, out, side)
+            ready = event(side)
+            if timing is not None:
+                timing.setdefault("contraction", []).append((e0, ready))
+            return out, ready
+
+        outs = []
+        nxt = contract(0) if perm_batches else None
+        for b in range(len(perm_batches)):
+            t, ready = nxt
+            main.wait_event(ready)
+            if b + 1 < len(perm_batches):
+                nxt = contract(b + 1)
+            s0 = event(main) if timing is not None else None
+            per = self.surfaces.batch(t, self.sign_mask, vdensity=self.vdensity, stream=main)
+            done = event(main)
+            swept[b % 2] = done
+            if timing is not None:
+                timing.setdefault("sweep", []).append((s0, done))
+            outs.append(per.max(0).values)
+        main.wait_stream(side)
+        return outs
+
     def graph(self, P):
         """One batch of P permutations captured as a HIP graph.  Returns
         run(perms) -> [P, 2] device tensor (the graph's own output buffer,
@@ -99,19 +158,28 @@ class PermutationEngine:
         run.graph = g
         return run
 
-    def run(self, n_perm, batch=256, seed=0, perms=None):
-        """Null of n_perm permutations: [n_perm, 2] (host numpy)."""
+    def run(self, n_perm, batch=256, seed=0, perms=None, pipelined=False):
+        """Null of n_perm permutations: [n_perm, n_signs] (host numpy), batch
+        after batch on the current stream, or through ``pipeline``.  Measured
+        on MI355X at 327,684 vertices the one-stream order is the faster one:
+        a contraction launched beside the sweep takes compute units the
+        sweep's workgroups would occupy and delays them by more than the
+        contraction takes alone."""
         import torch
-        out = []
+        batches = []
         done = 0
         while done < n_perm:
             P = min(batch, n_perm - done)
             pb = perms[done:done + P] if perms is not None else synth.permutations(self.n, P, seed + done, "cuda")
             if not isinstance(pb, torch.Tensor):
                 pb = torch.as_tensor(np.ascontiguousarray(pb, dtype=np.int32)).cuda()
-            out.append(self.null_batch(pb).cpu().numpy())
+            batches.append(pb.contiguous())
             done += P
-        return np.concatenate(out, 0)
+        if not batches:
+            return np.zeros((0, 2 if self.sign_mask == 3 else 1), dtype=np.float32)
+        if pipelined:
+            return np.concatenate([o.cpu().numpy() for o in self.pipeline(batches)], 0)
+        return np.concatenate([self.null_batch(pb).cpu().numpy() for pb in batches], 0)
 
 
 class MediationEngine:
@@ -145,9 +213,14 @@ class MediationEngine:
         adjs = [Adjacency(csr, *(settings[i] if settings is not None else (H, E))) for i, csr in enumerate(surfaces)]
         self.surfaces = SurfaceSet(adjs, offsets)
 
+    sign_mask = 1  # +z only
+
     def tvalues(self, perms):
         """[P, width] Sobel z in the engine's element layout."""
         return self.model.z(perms)
+
+    def _contract(self, perms, out, stream):
+        self.model.z(perms, out=out, stream=stream)
 
     def null_batch(self, perms, per_surface=False):
         """[P, 1] maxima of +z (max over surfaces), or [n_surf, P, 1]."""
@@ -156,3 +229,4 @@ class MediationEngine:
 
     graph = PermutationEngine.graph
     run = PermutationEngine.run
+    pipeline = PermutationEngine.pipeline

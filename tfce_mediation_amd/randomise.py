@@ -203,14 +203,18 @@ class MediationModel:
             raise ValueError("Invalid mediation type")
         self.V = self.B.V
 
-    def z(self, perms=None):
+    def z(self, perms=None, out=None, stream=None):
+        """[P, V] z on ``stream`` (default: the current stream), into ``out`` if given."""
         import torch
         from . import _lib
-        tb = self.B.t(perms)[0]
-        ta = self.A.t(perms)[0] if self.A is not None else None
-        z = torch.empty_like(tb)
-        _lib.check(_lib.lib().tfce_sobel_z(_lib.ptr(ta), _lib.ptr(tb), tb.numel(), self.alg,
-                                           float(self.ta_const or 0.0), _lib.ptr(z), _lib.stream_ptr()))
+        s = stream if stream is not None else torch.cuda.current_stream()
+        with torch.cuda.stream(s):
+            tb = self.B.t(perms, stream=s)[0]
+            ta = self.A.t(perms, stream=s)[0] if self.A is not None else None
+            z = out if out is not None else torch.empty_like(tb)
+            assert z.shape == tb.shape and z.is_contiguous()
+            _lib.check(_lib.lib().tfce_sobel_z(_lib.ptr(ta), _lib.ptr(tb), tb.numel(), self.alg,
+                                               float(self.ta_const or 0.0), _lib.ptr(z), _lib.stream_ptr(s)))
         return z
 
 

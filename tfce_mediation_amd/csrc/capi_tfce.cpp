@@ -32,7 +32,7 @@ struct tfce_surfset {
   int32_t n = 0;
   int device = 0;
   int32_t v_max = 0;
-  int64_t e_max = 0;
+  int32_t np_w = 0;  // widest neighbour-position table among the surfaces
   std::vector<SurfDev> host;  // in launch order (largest first)
   std::vector<int32_t> user_index;  // launch position -> caller's surface index
   SurfDev* d_surfs = nullptr;
@@ -323,7 +323,13 @@ int tfce_adj_info(const tfce_adj_t* a, int32_t* V, int64_t* nnz, int32_t* symmet
   return 0;
 }
 
-static int run_batch(const SurfDev* d_surfs, int32_t n_surf, int32_t v_max, int64_t e_max, const float* d_maps,
+// Width of the per-image neighbour-position table the sweep keeps for a surface
+// (symmetric, fixed-width table of at most kNposMaxWidth), 0 if none.
+static int32_t np_width(const tfce_adj* a) {
+  return ((a->flags & kSurfSymmetric) && a->ell_width > 0 && a->ell_width <= kNposMaxWidth) ? a->ell_width : 0;
+}
+
+static int run_batch(const SurfDev* d_surfs, int32_t n_surf, int32_t v_max, int32_t np_w, const float* d_maps,
                      int64_t map_stride, int P, int sign_mask, const float* d_T0,
                      const float* d_vdensity, bool full, float* d_enhn, float min_t, float dh,
                      int32_t labels_level, int32_t* d_labels, float* d_scaled, float* d_raw,
@@ -335,7 +341,7 @@ static int run_batch(const SurfDev* d_surfs, int32_t n_surf, int32_t v_max, int6
   const int64_t n_items = (int64_t)n_surf * P * n_signs;
   Workspace* ws = nullptr;
   const int grid = tfce_grid(n_items);
-  const int64_t slot_words = tfce_slot_words(v_max);
+  const int64_t slot_words = tfce_slot_words(v_max, np_w);
   const size_t slots_bytes = (size_t)grid * slot_words * sizeof(int32_t);
   const size_t prep_bytes = (size_t)n_items * kPrepWords * sizeof(int32_t);
   // raw_max is mandatory for the kernel: park it in scratch when not wanted
@@ -368,6 +374,7 @@ static int run_batch(const SurfDev* d_surfs, int32_t n_surf, int32_t v_max, int6
   a.scratch = reinterpret_cast<int32_t*>(ws->ptr);
   a.slot_stride = slot_words;
   a.v_max = v_max;
+  a.np_w = np_w;
   a.queue = ws->queue;
   a.n_items = n_items;
   a.prep = reinterpret_cast<int32_t*>(wbase + slots_bytes);
@@ -391,7 +398,7 @@ int tfce_run_ex(tfce_adj_t* adj, const float* d_image, float* d_enhn, float min_
     hipError_t e = launch_fill_f32(d_enhn, adj->V, 0.0f, stream);
     if (e != hipSuccess) return set_error(-(int)e, "fill enhn");
   }
-  return run_batch(adj->d_surf, 1, adj->V, adj->nnz, d_image, adj->V, 1, 1, nullptr, nullptr, true, d_enhn, min_t,
+  return run_batch(adj->d_surf, 1, adj->V, np_width(adj), d_image, adj->V, 1, 1, nullptr, nullptr, true, d_enhn, min_t,
                    dh, labels_level, d_labels, nullptr, nullptr, nullptr, d_status, stream);
 }
 
@@ -410,7 +417,7 @@ int tfce_batch_max(tfce_adj_t* adj, const float* d_maps, int P, int sign_mask, c
     hipError_t e = launch_fill_f32(d_max, (int64_t)P * 2, 0.0f, stream);
     if (e != hipSuccess) return set_error(-(int)e, "fill max");
   }
-  return run_batch(adj->d_surf, 1, adj->V, adj->nnz, d_maps, adj->V, P, sign_mask, nullptr, d_vdensity,
+  return run_batch(adj->d_surf, 1, adj->V, np_width(adj), d_maps, adj->V, P, sign_mask, nullptr, d_vdensity,
                    d_vdensity != nullptr, nullptr, 0.0f, 0.0f, -1, nullptr, d_max, nullptr, nullptr, nullptr,
                    stream, sign_mask == 3 ? 1 : 2, sign_mask == 2 ? 1 : 0);
 }
@@ -443,7 +450,7 @@ int tfce_surfset_create(const tfce_adj_t* const* adjs, int32_t n, const int64_t*
     s->host.push_back(sd);
     s->user_index.push_back(idx[k]);
     s->v_max = std::max(s->v_max, a->V);
-    s->e_max = std::max(s->e_max, a->nnz);
+    s->np_w = std::max(s->np_w, np_width(a));
   }
   hipError_t e = hipMalloc(&s->d_surfs, sizeof(SurfDev) * n);
   if (e == hipSuccess) e = hipMemcpy(s->d_surfs, s->host.data(), sizeof(SurfDev) * n, hipMemcpyHostToDevice);
@@ -469,7 +476,7 @@ int tfce_batch_ex(const tfce_surfset_t* s, const float* d_maps, int64_t map_stri
   if (!s || !d_maps || !d_scaled) return set_error(TFCE_EINVAL, "NULL argument");
   if (flags & ~(uint32_t)(TFCE_BATCH_SHARED_THRESHOLDS | TFCE_BATCH_ZERO_IN_IMAGE))
     return set_error(TFCE_EINVAL, "unknown batch flag");
-  return run_batch(s->d_surfs, s->n, s->v_max, s->e_max, d_maps, map_stride, P, sign_mask, nullptr, d_vdensity,
+  return run_batch(s->d_surfs, s->n, s->v_max, s->np_w, d_maps, map_stride, P, sign_mask, nullptr, d_vdensity,
                    d_vdensity != nullptr, nullptr, 0.0f, 0.0f, -1, nullptr, d_scaled, d_raw, d_own, d_status,
                    stream, 1, 0, flags);
 }
@@ -484,7 +491,7 @@ int tfce_batch(const tfce_surfset_t* s, const float* d_maps, int64_t map_stride,
                const float* d_T0, const float* d_vdensity, float* d_scaled, float* d_raw, float* d_own,
                int32_t* d_status, hipStream_t stream) {
   if (!s || !d_maps || !d_scaled) return set_error(TFCE_EINVAL, "NULL argument");
-  return run_batch(s->d_surfs, s->n, s->v_max, s->e_max, d_maps, map_stride, P, sign_mask, d_T0, d_vdensity,
+  return run_batch(s->d_surfs, s->n, s->v_max, s->np_w, d_maps, map_stride, P, sign_mask, d_T0, d_vdensity,
                    d_vdensity != nullptr, nullptr, 0.0f, 0.0f, -1, nullptr, d_scaled, d_raw, d_own, d_status,
                    stream);
 }

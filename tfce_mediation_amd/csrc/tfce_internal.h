@@ -11,6 +11,7 @@ constexpr int kMaxLevels = 1024;
 constexpr int32_t kNever = 0x7fffffff;
 constexpr int kTfceThreads = 256;
 constexpr int kEllMaxWidth = 32;  // longest neighbour list kept in the fixed-width table
+constexpr int kNposMaxWidth = 16;  // widest table whose per-image neighbour positions a slot holds
 
 // Status codes written per work item (>= 0: number of levels visited).
 constexpr int32_t kStatusDegenerate = -2;  // max >= minT with a step <= 0: the reference never returns
@@ -59,6 +60,7 @@ struct TfceBatchArgs {
   int32_t* scratch;
   int64_t slot_stride;     // int32 words per slot
   int32_t v_max;
+  int32_t np_w;            // words per element of the slot's neighbour-position table (0: none)
   int32_t* queue;          // dynamic work counter, zeroed before launch
   int32_t* prep;           // [n_items][kPrepWords] per-item maximum and level tables
   int64_t n_items;
@@ -78,6 +80,8 @@ hipError_t launch_shared_T0(const TfceBatchArgs& a, float* T0, int zero_in_image
 hipError_t launch_fill_f32(float* p, int64_t n, float v, hipStream_t stream);
 
 // Slot layout: 8 arrays of the largest surface's element count.
-inline int64_t tfce_slot_words(int32_t v_max) { return (int64_t)8 * ((v_max + 63) & ~63); }
+inline int64_t tfce_slot_words(int32_t v_max, int32_t np_w) {
+  return (int64_t)(8 + np_w) * ((v_max + 63) & ~63);
+}
 
 }  // namespace tfcehip

@@ -170,6 +170,8 @@ __device__ __forceinline__ void find_many(int32_t* parent, int (&x)[N]) {
   }
 }
 
+__device__ __forceinline__ int join_roots(int32_t* parent, int (&r)[9], int ri);
+
 // 3b for one element i (vertex v) and up to 8 of its neighbours u[] (-1: none):
 // gather their positions, find the roots of the joined ones and of i's set
 // (ri, an ancestor) as interleaved chains, then hook every other distinct root
@@ -196,6 +198,12 @@ __device__ __forceinline__ int join_chunk(int32_t* parent, const int32_t* pos, c
     }
     r[j] = take ? q[j] : -1;
   }
+  return join_roots(parent, r, ri);
+}
+
+// The root-joining half of join_chunk: r[0..7] are positions to join with
+// i's set (ri, an ancestor of it), -1 for none.
+__device__ __forceinline__ int join_roots(int32_t* parent, int (&r)[9], int ri) {
   r[8] = ri;
   find_many<9>(parent, r);
   int tgt = r[8];
@@ -337,6 +345,7 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
   float* val = reinterpret_cast<float*>(slot + 4 * vstride);
   int32_t* roots[2] = {slot + 5 * vstride, slot + 6 * vstride};
   float* incv = reinterpret_cast<float*>(slot + 7 * vstride);
+  int32_t* npos = slot + 8 * vstride;  // [element][W] neighbour positions (a.np_w > 0)
 
   while (true) {
     if (tid == 0) {
@@ -411,6 +420,30 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
     if (tid == 0) s_nroots_new = 0;
     __syncthreads();
     const int A = (L > 0) ? s_start[L] : 0;
+    // Symmetric surface with a fixed-width table: the positions of every
+    // active element's neighbours, one row per element in position order.  The
+    // pass runs in vertex order, where rows of the table and neighbours'
+    // positions are near each other, and the union step then reads one row per
+    // element, consecutive across a wave, instead of a table row and a gather.
+    const bool use_np = symmetric && ell != nullptr && W <= a.np_w;
+    if (use_np && L > 0) {
+      for (int v = tid; v < V; v += kTfceThreads) {
+        const int qv = pos[v];
+        if (qv == kNever) continue;
+        const int4* er = reinterpret_cast<const int4*>(ell + (size_t)v * W);
+        int4* nr = reinterpret_cast<int4*>(npos + (size_t)qv * W);
+        for (int c = 0; c < (W >> 2); ++c) {
+          const int4 uu = er[c];
+          int4 o;
+          o.x = uu.x >= 0 ? pos[uu.x] : kNever;
+          o.y = uu.y >= 0 ? pos[uu.y] : kNever;
+          o.z = uu.z >= 0 ? pos[uu.z] : kNever;
+          o.w = uu.w >= 0 ? pos[uu.w] : kNever;
+          nr[c] = o;
+        }
+      }
+      __syncthreads();
+    }
 
     // ---- phase 3: threshold sweep -------------------------------------------
     int n_roots = 0;
@@ -433,6 +466,37 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
         // first row and the vertex of the one after are loaded one iteration
         // ahead.
         const int4 none4 = make_int4(-1, -1, -1, -1);
+        if (use_np) {
+          // rows of neighbour positions: a joined neighbour is one with a
+          // smaller position (active before this level, or earlier in it)
+          int4 lo_n = none4, hi_n = none4;
+          if (b0 + tid < b1) {
+            const int4* rn = reinterpret_cast<const int4*>(npos + (size_t)(b0 + tid) * W);
+            lo_n = rn[0];
+            if (W > 4) hi_n = rn[1];
+          }
+          for (int i = b0 + tid; i < b1; i += kTfceThreads) {
+            const int4 lo = lo_n, hi = hi_n;
+            if (i + kTfceThreads < b1) {
+              const int4* rn = reinterpret_cast<const int4*>(npos + (size_t)(i + kTfceThreads) * W);
+              lo_n = rn[0];
+              if (W > 4) hi_n = rn[1];
+            }
+            int r[9] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w, 0};
+#pragma unroll
+            for (int j = 0; j < 8; ++j) r[j] = r[j] < i ? r[j] : -1;
+            int ri = join_roots(parent, r, i);
+            const int4* row = reinterpret_cast<const int4*>(npos + (size_t)i * W);
+            for (int e = 8; e < W; e += 8) {
+              const int4 l2 = row[e >> 2];
+              const int4 h2 = (e + 4 < W) ? row[(e >> 2) + 1] : make_int4(kNever, kNever, kNever, kNever);
+              int r2[9] = {l2.x, l2.y, l2.z, l2.w, h2.x, h2.y, h2.z, h2.w, 0};
+#pragma unroll
+              for (int j = 0; j < 8; ++j) r2[j] = r2[j] < i ? r2[j] : -1;
+              ri = join_roots(parent, r2, ri);
+            }
+          }
+        } else {
         int v_nxt = (b0 + tid < b1) ? order[b0 + tid] : 0;
         int v_nn = (b0 + tid + kTfceThreads < b1) ? order[b0 + tid + kTfceThreads] : 0;
         int4 lo_nxt = none4, hi_nxt = none4;
@@ -475,6 +539,7 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
               ri = join_chunk(parent, pos, x, u, i, v, ri, b0, b1, symmetric, sign, xv);
             }
           }
+        }
         }
         __syncthreads();
         // 3c: merge sizes (and MAX values) into surviving roots; collect roots

@@ -427,19 +427,31 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
     // element, consecutive across a wave, instead of a table row and a gather.
     const bool use_np = symmetric && ell != nullptr && W <= a.np_w;
     if (use_np && L > 0) {
-      for (int v = tid; v < V; v += kTfceThreads) {
-        const int qv = pos[v];
-        if (qv == kNever) continue;
-        const int4* er = reinterpret_cast<const int4*>(ell + (size_t)v * W);
-        int4* nr = reinterpret_cast<int4*>(npos + (size_t)qv * W);
+      // four vertices per thread and pass, every load of a 4-wide column
+      // chunk issued before its stores
+      for (int v0 = tid; v0 < V; v0 += 4 * kTfceThreads) {
+        int qv[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int v = v0 + j * kTfceThreads;
+          qv[j] = v < V ? pos[v] : kNever;
+        }
         for (int c = 0; c < (W >> 2); ++c) {
-          const int4 uu = er[c];
-          int4 o;
-          o.x = uu.x >= 0 ? pos[uu.x] : kNever;
-          o.y = uu.y >= 0 ? pos[uu.y] : kNever;
-          o.z = uu.z >= 0 ? pos[uu.z] : kNever;
-          o.w = uu.w >= 0 ? pos[uu.w] : kNever;
-          nr[c] = o;
+          int4 uu[4], o[4];
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            uu[j] = qv[j] != kNever ? reinterpret_cast<const int4*>(ell + (size_t)(v0 + j * kTfceThreads) * W)[c]
+                                    : make_int4(-1, -1, -1, -1);
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            o[j].x = uu[j].x >= 0 ? pos[uu[j].x] : kNever;
+            o[j].y = uu[j].y >= 0 ? pos[uu[j].y] : kNever;
+            o[j].z = uu[j].z >= 0 ? pos[uu[j].z] : kNever;
+            o[j].w = uu[j].w >= 0 ? pos[uu[j].w] : kNever;
+          }
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            if (qv[j] != kNever) reinterpret_cast<int4*>(npos + (size_t)qv[j] * W)[c] = o[j];
         }
       }
       __syncthreads();

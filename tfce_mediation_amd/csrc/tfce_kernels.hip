@@ -381,11 +381,24 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
       s_cur[l] = 0;
     }
     __syncthreads();
+    // eight elements per thread and pass, loads before stores
     if (L > 0) {
-      for (int v = tid; v < V; v += kTfceThreads) {
-        const int lev = level_of(sign * x[v], s_T, L, T0, dT);
-        if (lev != kNever) atomicAdd(&s_cur[lev], 1);
-        pos[v] = lev;
+      for (int v0 = tid; v0 < V; v0 += 8 * kTfceThreads) {
+        float xs[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int v = v0 + j * kTfceThreads;
+          xs[j] = v < V ? x[v] : 0.0f;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int v = v0 + j * kTfceThreads;
+          if (v < V) {
+            const int lev = level_of(sign * xs[j], s_T, L, T0, dT);
+            if (lev != kNever) atomicAdd(&s_cur[lev], 1);
+            pos[v] = lev;
+          }
+        }
       }
     }
     __syncthreads();
@@ -408,12 +421,21 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
     __syncthreads();
     // ---- phase 2: level order ----------------------------------------------
     if (L > 0) {
-      for (int v = tid; v < V; v += kTfceThreads) {
-        const int lev = pos[v];
-        if (lev != kNever) {
-          const int q = atomicAdd(&s_cur[lev], 1);
-          order[q] = v;
-          pos[v] = q;
+      for (int v0 = tid; v0 < V; v0 += 8 * kTfceThreads) {
+        int lv[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int v = v0 + j * kTfceThreads;
+          lv[j] = v < V ? pos[v] : kNever;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          if (lv[j] != kNever) {
+            const int v = v0 + j * kTfceThreads;
+            const int q = atomicAdd(&s_cur[lv[j]], 1);
+            order[q] = v;
+            pos[v] = q;
+          }
         }
       }
     }
@@ -435,6 +457,26 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
         for (int j = 0; j < 4; ++j) {
           const int v = v0 + j * kTfceThreads;
           qv[j] = v < V ? pos[v] : kNever;
+        }
+        if (W == 8) {
+          // the common mesh width: both column chunks at once
+          int4 uu[8], o[8];
+#pragma unroll
+          for (int j = 0; j < 8; ++j)
+            uu[j] = qv[j >> 1] != kNever
+                        ? reinterpret_cast<const int4*>(ell + (size_t)(v0 + (j >> 1) * kTfceThreads) * 8)[j & 1]
+                        : make_int4(-1, -1, -1, -1);
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            o[j].x = uu[j].x >= 0 ? pos[uu[j].x] : kNever;
+            o[j].y = uu[j].y >= 0 ? pos[uu[j].y] : kNever;
+            o[j].z = uu[j].z >= 0 ? pos[uu[j].z] : kNever;
+            o[j].w = uu[j].w >= 0 ? pos[uu[j].w] : kNever;
+          }
+#pragma unroll
+          for (int j = 0; j < 8; ++j)
+            if (qv[j >> 1] != kNever) reinterpret_cast<int4*>(npos + (size_t)qv[j >> 1] * 8)[j & 1] = o[j];
+          continue;
         }
         for (int c = 0; c < (W >> 2); ++c) {
           int4 uu[4], o[4];

@@ -219,6 +219,8 @@ def main():
     ap.add_argument("--pipeline", action="store_true",
                     help="run the contraction of step s+1 on a second stream during the sweep of step s "
                          "(engine.pipeline) instead of one after the other on one stream")
+    ap.add_argument("--no-precision-check", action="store_true",
+                    help="skip the t-vs-FP64 error check and the FP16x3 timing")
     ap.add_argument("--no-graph", action="store_true",
                     help="skip the hipGraph replay measurement of the same steps")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -324,6 +326,42 @@ def main():
             graph_rate = None
         del run
 
+    # ---- t against FP64, and the FP16 split scheme -----------------------------
+    # t of 4 permutations of this workload (engine layout) against the FP64
+    # single-design path (tval_int semantics on X[perm], cynumstats) mapped
+    # back to the engine's columns; then the same steps with the contraction
+    # in FP16x3 (hi/lo fp16 of power-of-two-scaled operands, 3 products).
+    t_err = {}
+    fp16_rate = None
+    if name != "c5" and not args.no_precision_check:
+        from tfce_mediation_amd.cynumstats import _direct
+        X = np.asarray(cfg["X"], dtype=np.float64)
+        perms_chk = perm_batches[args.warmup][:4]
+        cols = eng.cols if eng.cols is not None else np.arange(eng.V)
+        ref = np.stack([_direct(X[p], cfg["Y"], t=True)["t"][eng.contrast][cols]
+                        for p in perms_chk.cpu().numpy()])
+
+        def terr(tg):
+            tg = tg.astype(np.float64)
+            big = np.abs(ref) >= 1
+            return {"norm": float(np.linalg.norm(tg - ref) / np.linalg.norm(ref)),
+                    "point_abs_t_ge_1": float(np.max(np.abs(tg - ref)[big] / np.abs(ref[big])))}
+        t_err[args.precision] = terr(eng.tvalues(perms_chk).cpu().numpy())
+        if args.precision != "fp16x3":
+            _, eng16, _, _, _ = make_workload(name, "fp16x3")
+            t_err["fp16x3"] = terr(eng16.tvalues(perms_chk).cpu().numpy())
+            for s in range(args.warmup):
+                eng16.null_batch(perm_batches[s])
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for perms in timed:
+                eng16.null_batch(perms)
+            torch.cuda.synchronize()
+            fp16_rate = args.steps * P / (time.perf_counter() - t0)
+            del eng16
+        log(f"[bench] t vs FP64 (4 permutations): {t_err}; fp16x3 steps: "
+            f"{'-' if fp16_rate is None else f'{fp16_rate:.1f} perms/s'}")
+
     # ---- plain FP32 per-permutation path ------------------------------------
     # the reference's call structure: one t map per permutation, then one
     # CreateAdjSet.run per (sign, surface) and the write_perm_maxTFCE rescale
@@ -412,6 +450,8 @@ def main():
             "plain_fp32_perms_per_s": None if plain is None else round(plain, 3),
             "batched_over_plain": None if plain is None else round(value / plain, 2),
             "hipgraph_replay_perms_per_s": None if graph_rate is None else round(graph_rate, 1),
+            "t_rel_err_vs_fp64": t_err or None,
+            "fp16x3_perms_per_s": None if fp16_rate is None else round(fp16_rate, 1),
             "autotuned": None if tuned is None else {"gemm_tile": ["256x128", "512x64", "512x128"][tuned["gemm_tile"]],
                                                      "tfce_wgs_per_cu": tuned["tfce_wgs_per_cu"]},
             "schedule": mode + (" (contraction of step s+1 on a second stream during the sweep of step s)"

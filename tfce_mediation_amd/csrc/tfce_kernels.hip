@@ -14,10 +14,14 @@
 //     1. bucket the vertices by activation level (counting sort, LDS
 //        histogram); compact id = position in that order, so the active set
 //        at level l is the id prefix [0, end_l) and "is u active" is
-//        `pos[u] < end_l`;
+//        `pos[u] < end_l`; for symmetric surfaces with a fixed-width
+//        neighbour table, one row of neighbour positions per active element
+//        (a joined neighbour is then simply one with a smaller position);
 //     2. per level: hook the newly active vertices into a lock-free union-find
-//        (CAS hooking the larger root under the smaller), merge component sizes
-//        into the surviving roots with atomics, then
+//        (roots of an element's joined neighbours found as interleaved
+//        chains, every other root hooked under the smallest by CAS, larger
+//        under smaller), merge component sizes into the surviving roots with
+//        atomics, then
 //          MAX mode : add the level increment to one value per root; a merged
 //                     root keeps the larger value (atomicMax on float bits); the
 //                     largest TFCE of a component is held by its earliest vertex,

@@ -70,6 +70,18 @@ int cu_count(int device) {
   return n;
 }
 
+size_t device_memory(int device) {
+  static std::map<int, size_t> cache;
+  static std::mutex m;
+  std::lock_guard<std::mutex> g(m);
+  auto it = cache.find(device);
+  if (it != cache.end()) return it->second;
+  size_t bytes = 0;
+  if (hipDeviceTotalMem(&bytes, device) != hipSuccess || bytes == 0) bytes = (size_t)64 << 30;
+  cache[device] = bytes;
+  return bytes;
+}
+
 // Grow-only scratch for (device, stream).  Growing synchronises the stream so
 // no kernel of that stream still uses the old buffer.
 int workspace_get(hipStream_t stream, size_t bytes, Workspace** out) {
@@ -340,8 +352,11 @@ static int run_batch(const SurfDev* d_surfs, int32_t n_surf, int32_t v_max, int3
   const int n_signs = (sign_mask == 3) ? 2 : 1;
   const int64_t n_items = (int64_t)n_surf * P * n_signs;
   Workspace* ws = nullptr;
-  const int grid = tfce_grid(n_items);
   const int64_t slot_words = tfce_slot_words(v_max, np_w);
+  // resident workgroups, each with its own slot; the slots stay within a
+  // quarter of the device's memory (fewer workgroups for very large surfaces)
+  const int64_t mem_cap = (int64_t)(device_memory(current_device()) / 4 / ((size_t)slot_words * sizeof(int32_t)));
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(tfce_grid(n_items), mem_cap));
   const size_t slots_bytes = (size_t)grid * slot_words * sizeof(int32_t);
   const size_t prep_bytes = (size_t)n_items * kPrepWords * sizeof(int32_t);
   // raw_max is mandatory for the kernel: park it in scratch when not wanted

@@ -205,9 +205,24 @@ __device__ __forceinline__ int join_chunk(int32_t* parent, const int32_t* pos, c
   return join_roots(parent, r, ri);
 }
 
+// Compare-exchange for a descending sorting network.
+__device__ __forceinline__ void cx_desc(int& a, int& b) {
+  const int hi = max(a, b), lo = min(a, b);
+  a = hi;
+  b = lo;
+}
+
 // The root-joining half of join_chunk: r[0..7] are positions to join with
-// i's set (ri, an ancestor of it), -1 for none.
+// i's set (ri, an ancestor of it), -1 for none.  The positions are sorted
+// descending first (Batcher's 19-comparator network), so the empty slots are
+// the last ones and a slot that no lane of the wave uses issues no memory
+// instruction in the chains and CASes below.
 __device__ __forceinline__ int join_roots(int32_t* parent, int (&r)[9], int ri) {
+  cx_desc(r[0], r[1]); cx_desc(r[2], r[3]); cx_desc(r[4], r[5]); cx_desc(r[6], r[7]);
+  cx_desc(r[0], r[2]); cx_desc(r[1], r[3]); cx_desc(r[4], r[6]); cx_desc(r[5], r[7]);
+  cx_desc(r[1], r[2]); cx_desc(r[5], r[6]); cx_desc(r[0], r[4]); cx_desc(r[3], r[7]);
+  cx_desc(r[1], r[5]); cx_desc(r[2], r[6]); cx_desc(r[1], r[4]); cx_desc(r[3], r[6]);
+  cx_desc(r[2], r[4]); cx_desc(r[3], r[5]); cx_desc(r[3], r[4]);
   r[8] = ri;
   find_many<9>(parent, r);
   int tgt = r[8];

@@ -223,8 +223,16 @@ __device__ __forceinline__ int join_roots(int32_t* parent, int (&r)[9], int ri) 
   cx_desc(r[1], r[2]); cx_desc(r[5], r[6]); cx_desc(r[0], r[4]); cx_desc(r[3], r[7]);
   cx_desc(r[1], r[5]); cx_desc(r[2], r[6]); cx_desc(r[1], r[4]); cx_desc(r[3], r[6]);
   cx_desc(r[2], r[4]); cx_desc(r[3], r[5]); cx_desc(r[3], r[4]);
+  // i's own set enters as ri without a lookup: if ri is no longer a root its
+  // CAS below fails and uf_link takes over, and if it is the smallest, the
+  // other roots hooked under it still land in its set under a smaller id
+  int rn[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) rn[j] = r[j];
+  find_many<8>(parent, rn);
+#pragma unroll
+  for (int j = 0; j < 8; ++j) r[j] = rn[j];
   r[8] = ri;
-  find_many<9>(parent, r);
   int tgt = r[8];
 #pragma unroll
   for (int j = 0; j < 8; ++j)

@@ -639,10 +639,24 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
 #pragma unroll
           for (int j = 0; j < kIlp; ++j) {
             const bool hooked = live[j] && r[j] != c[j];
-            if (hooked) {
-              // Additions go to final roots only, so a hooked element of this
-              // level still has size 1 and (MAX mode) value 0: nothing to load,
-              // and its value cannot raise the root's.
+            // Additions go to final roots only, so a hooked element of this
+            // level still has size 1 and (MAX mode) value 0: nothing to load,
+            // and its value cannot raise the root's.  Such elements of one
+            // wave mostly join the same root: the lanes sharing the first
+            // one's root add their count with one atomic (same-address atomics
+            // serialise in L2).
+            const bool fresh_hook = hooked && c[j] >= b0;
+            const unsigned long long fm = __ballot(fresh_hook);
+            bool done = false;
+            if (fm != 0ull) {
+              const int leader = __ffsll((long long)fm) - 1;
+              const int rl = __shfl(r[j], leader, 64);
+              const unsigned long long same = __ballot(fresh_hook && r[j] == rl);
+              if ((threadIdx.x & 63) == leader)
+                __hip_atomic_fetch_add(size + rl, __popcll(same), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+              done = fresh_hook && r[j] == rl;
+            }
+            if (hooked && !done) {
               const bool fresh = c[j] >= b0;
               __hip_atomic_fetch_add(size + r[j], fresh ? 1 : ld(size + c[j]), __ATOMIC_RELAXED,
                                      __HIP_MEMORY_SCOPE_WORKGROUP);

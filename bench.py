@@ -470,6 +470,8 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
             "traffic_source": traffic_src,
+            "traffic_gbs": None if traffic is None else round(traffic / (tfce_ms * 1e-3) / 1e9, 1),
+            "traffic_frac_of_peak": None if traffic is None else round(traffic / (tfce_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "algorithmic_bytes_per_unit": bytes_per_sign,
             "units_per_launch": units,
             "note": "level-sweep byte model (100 thresholds x (CSR + map + labels) per sign); the kernel "

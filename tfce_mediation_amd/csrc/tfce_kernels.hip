@@ -113,36 +113,9 @@ __device__ __forceinline__ void wave_append(bool take, int c, int32_t* list, int
 
 constexpr int kIlp = 4;  // independent find chains per thread in the merge phases
 
-// Roots of up to kIlp elements at once; the loads of the chains interleave.
-// Path splitting: a visited node is re-pointed to its grandparent.
-__device__ __forceinline__ void find_batch(int32_t* parent, int (&x)[kIlp], const bool (&live)[kIlp]) {
-  bool go[kIlp];
-  int prev[kIlp];
-#pragma unroll
-  for (int j = 0; j < kIlp; ++j) {
-    go[j] = live[j];
-    prev[j] = -1;
-  }
-  while (go[0] || go[1] || go[2] || go[3]) {
-    int p[kIlp];
-#pragma unroll
-    for (int j = 0; j < kIlp; ++j) p[j] = go[j] ? ld(parent + x[j]) : 0;
-#pragma unroll
-    for (int j = 0; j < kIlp; ++j) {
-      if (!go[j]) continue;
-      if (p[j] == x[j]) {
-        go[j] = false;
-      } else {
-        if (prev[j] >= 0) st(parent + prev[j], p[j]);
-        prev[j] = x[j];
-        x[j] = p[j];
-      }
-    }
-  }
-}
-
 // Roots of up to N elements at once (x[j] < 0: none), the loads of the
-// chains interleaved; path splitting as in find_batch.
+// chains interleaved.  Path splitting: a visited node is re-pointed to its
+// grandparent.
 template <int N>
 __device__ __forceinline__ void find_many(int32_t* parent, int (&x)[N]) {
   int prev[N];
@@ -172,6 +145,17 @@ __device__ __forceinline__ void find_many(int32_t* parent, int (&x)[N]) {
       }
     }
   }
+}
+
+// Roots of the live ones of kIlp elements (find_many on them).
+__device__ __forceinline__ void find_batch(int32_t* parent, int (&x)[kIlp], const bool (&live)[kIlp]) {
+  int y[kIlp];
+#pragma unroll
+  for (int j = 0; j < kIlp; ++j) y[j] = live[j] ? x[j] : -1;
+  find_many<kIlp>(parent, y);
+#pragma unroll
+  for (int j = 0; j < kIlp; ++j)
+    if (live[j]) x[j] = y[j];
 }
 
 __device__ __forceinline__ int join_roots(int32_t* parent, int (&r)[9], int ri);

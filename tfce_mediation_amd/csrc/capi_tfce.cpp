@@ -23,6 +23,7 @@ struct tfce_adj {
   int32_t* d_row_ptr = nullptr;
   int32_t* d_col = nullptr;
   int32_t* d_ell = nullptr;     // [V][ell_width], -1 padded; null for long lists
+  uint8_t* d_nadj = nullptr;    // [V][ell_width] adjacency among a row's neighbours (width <= 8)
   int32_t ell_width = 0;
   double* d_sizepow = nullptr;  // (double)s^E for s in [0, V] when E != 1
   SurfDev* d_surf = nullptr;    // one descriptor, offset 0
@@ -183,6 +184,7 @@ int tfce_adj_create_ex(const int32_t* row_ptr, const int32_t* col, int32_t V, in
     if (a->d_row_ptr) (void)hipFree(a->d_row_ptr);
     if (a->d_col) (void)hipFree(a->d_col);
     if (a->d_ell) (void)hipFree(a->d_ell);
+    if (a->d_nadj) (void)hipFree(a->d_nadj);
     if (a->d_sizepow) (void)hipFree(a->d_sizepow);
     if (a->d_surf) (void)hipFree(a->d_surf);
     delete a;
@@ -213,6 +215,32 @@ int tfce_adj_create_ex(const int32_t* row_ptr, const int32_t* col, int32_t V, in
       if ((e = hipMemcpy(a->d_ell, ell.data(), sizeof(int32_t) * ell.size(), hipMemcpyHostToDevice)) != hipSuccess)
         return cleanup(-(int)e, "upload ell");
       a->ell_width = width;
+      // Which neighbours of a row are adjacent to each other (symmetric
+      // surfaces, rows of at most 8): two such neighbours that were both
+      // active before a level are already in one set, so the union step joins
+      // one of each adjacent group.
+      if (symmetric && width <= 8 && V < kPosMask) {
+        std::vector<int32_t> srt(col, col + nnz);
+        for (int32_t v = 0; v < V; ++v) std::sort(srt.begin() + row_ptr[v], srt.begin() + row_ptr[v + 1]);
+        auto adjacent = [&](int32_t x, int32_t y) {
+          return std::binary_search(srt.begin() + row_ptr[x], srt.begin() + row_ptr[x + 1], y);
+        };
+        std::vector<uint8_t> nadj((size_t)V * width, 0);
+        for (int32_t v = 0; v < V; ++v)
+          for (int32_t j = 0; j < width; ++j) {
+            const int32_t uj = ell[(size_t)v * width + j];
+            if (uj < 0) continue;
+            uint8_t m = 0;
+            for (int32_t k = 0; k < width; ++k) {
+              const int32_t uk = ell[(size_t)v * width + k];
+              if (k != j && uk >= 0 && adjacent(uj, uk)) m |= (uint8_t)(1u << k);
+            }
+            nadj[(size_t)v * width + j] = m;
+          }
+        if ((e = hipMalloc(&a->d_nadj, nadj.size())) != hipSuccess) return cleanup(-(int)e, "hipMalloc(nadj)");
+        if ((e = hipMemcpy(a->d_nadj, nadj.data(), nadj.size(), hipMemcpyHostToDevice)) != hipSuccess)
+          return cleanup(-(int)e, "upload nadj");
+      }
     }
   }
   if (E != 1.0f) {
@@ -226,6 +254,7 @@ int tfce_adj_create_ex(const int32_t* row_ptr, const int32_t* col, int32_t V, in
   sd.col = a->d_col;
   sd.ell = a->d_ell;
   sd.ell_width = a->ell_width;
+  sd.nadj = a->d_nadj;
   sd.sizepow = a->d_sizepow;
   sd.offset = 0;
   sd.V = V;
@@ -321,6 +350,7 @@ int tfce_adj_destroy(tfce_adj_t* a) {
   (void)hipFree(a->d_row_ptr);
   (void)hipFree(a->d_col);
   if (a->d_ell) (void)hipFree(a->d_ell);
+  if (a->d_nadj) (void)hipFree(a->d_nadj);
   if (a->d_sizepow) (void)hipFree(a->d_sizepow);
   (void)hipFree(a->d_surf);
   delete a;
@@ -456,6 +486,7 @@ int tfce_surfset_create(const tfce_adj_t* const* adjs, int32_t n, const int64_t*
     sd.col = a->d_col;
     sd.ell = a->d_ell;
     sd.ell_width = a->ell_width;
+    sd.nadj = a->d_nadj;
     sd.sizepow = a->d_sizepow;
     sd.offset = offsets[idx[k]];
     sd.V = a->V;

@@ -11,7 +11,12 @@ constexpr int kMaxLevels = 1024;
 constexpr int32_t kNever = 0x7fffffff;
 constexpr int kTfceThreads = 256;
 constexpr int kEllMaxWidth = 32;  // longest neighbour list kept in the fixed-width table
-constexpr int kNposMaxWidth = 16;  // widest table whose per-image neighbour positions a slot holds
+constexpr int kNposMaxWidth = 16;
+// Neighbour-position entries of a surface with a local adjacency table: the
+// position in the low 24 bits (kPosNone: never active), the neighbour's
+// adjacency byte within the row in the high 8.
+constexpr int32_t kPosMask = 0xffffff;
+constexpr int32_t kPosNone = 0xffffff;  // widest table whose per-image neighbour positions a slot holds
 
 // Status codes written per work item (>= 0: number of levels visited).
 constexpr int32_t kStatusDegenerate = -2;  // max >= minT with a step <= 0: the reference never returns
@@ -27,6 +32,8 @@ struct SurfDev {
   const int32_t* col;      // nnz (indices local to the surface)
   const int32_t* ell;      // [V][ell_width] neighbour lists padded with -1, or null
   int32_t ell_width;       // largest list length (0: use the CSR)
+  const uint8_t* nadj;     // [V][ell_width] (width <= 8): bit k of entry j set iff neighbours j and k
+                           // are adjacent; null if not built
   const double* sizepow;   // [V+1] (double)s^E, null when E == 1
   int64_t offset;          // first element of this surface inside a map row
   int32_t V;

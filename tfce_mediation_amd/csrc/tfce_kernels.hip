@@ -160,6 +160,44 @@ __device__ __forceinline__ void find_batch(int32_t* parent, int (&x)[kIlp], cons
 
 __device__ __forceinline__ int join_roots(int32_t* parent, int (&r)[9], int ri);
 
+// Neighbour-position entry of a surface with a local adjacency table.
+__device__ __forceinline__ int np_encode(int q, const uint8_t* nadj_row, int j) {
+  return (q == kNever ? kPosNone : q) | ((int)nadj_row[j] << 24);
+}
+
+// Candidates of element i from its row of encoded entries (width <= 8): every
+// joined neighbour of this level, and one neighbour per group of mutually
+// adjacent neighbours that were active before it (such a group is one set
+// already: its edges were joined at earlier levels).
+__device__ __forceinline__ void prune_row(const int (&e)[8], int i, int b0, int (&r)[9]) {
+  unsigned old_m = 0, keep = 0, adj[8];
+  int p[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    p[j] = e[j] & kPosMask;
+    adj[j] = (unsigned)e[j] >> 24;
+    if (p[j] < i) {
+      if (p[j] < b0) old_m |= 1u << j;
+      else keep |= 1u << j;
+    }
+  }
+  unsigned rem = old_m;
+  while (rem) {
+    const unsigned seed = rem & (0u - rem);
+    unsigned comp = seed, prev;
+    do {
+      prev = comp;
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (comp & (1u << j)) comp |= adj[j] & old_m;
+    } while (comp != prev);
+    keep |= seed;
+    rem &= ~comp;
+  }
+#pragma unroll
+  for (int j = 0; j < 8; ++j) r[j] = (keep >> j) & 1u ? p[j] : -1;
+}
+
 // 3b for one element i (vertex v) and up to 8 of its neighbours u[] (-1: none):
 // gather their positions, find the roots of the joined ones and of i's set
 // (ri, an ancestor) as interleaved chains, then hook every other distinct root
@@ -484,6 +522,17 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
             o[j].z = uu[j].z >= 0 ? pos[uu[j].z] : kNever;
             o[j].w = uu[j].w >= 0 ? pos[uu[j].w] : kNever;
           }
+          if (S.nadj) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+              if (qv[j >> 1] == kNever) continue;
+              const uint8_t* nr = S.nadj + (size_t)(v0 + (j >> 1) * kTfceThreads) * 8 + (j & 1) * 4;
+              o[j].x = np_encode(o[j].x, nr, 0);
+              o[j].y = np_encode(o[j].y, nr, 1);
+              o[j].z = np_encode(o[j].z, nr, 2);
+              o[j].w = np_encode(o[j].w, nr, 3);
+            }
+          }
 #pragma unroll
           for (int j = 0; j < 8; ++j)
             if (qv[j >> 1] != kNever) reinterpret_cast<int4*>(npos + (size_t)qv[j >> 1] * 8)[j & 1] = o[j];
@@ -501,6 +550,17 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
             o[j].y = uu[j].y >= 0 ? pos[uu[j].y] : kNever;
             o[j].z = uu[j].z >= 0 ? pos[uu[j].z] : kNever;
             o[j].w = uu[j].w >= 0 ? pos[uu[j].w] : kNever;
+          }
+          if (S.nadj) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              if (qv[j] == kNever) continue;
+              const uint8_t* nr = S.nadj + (size_t)(v0 + j * kTfceThreads) * W + c * 4;
+              o[j].x = np_encode(o[j].x, nr, 0);
+              o[j].y = np_encode(o[j].y, nr, 1);
+              o[j].z = np_encode(o[j].z, nr, 2);
+              o[j].w = np_encode(o[j].w, nr, 3);
+            }
           }
 #pragma unroll
           for (int j = 0; j < 4; ++j)
@@ -548,8 +608,13 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
               if (W > 4) hi_n = rn[1];
             }
             int r[9] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w, 0};
+            if (S.nadj) {
+              const int e8[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+              prune_row(e8, i, b0, r);
+            } else {
 #pragma unroll
-            for (int j = 0; j < 8; ++j) r[j] = r[j] < i ? r[j] : -1;
+              for (int j = 0; j < 8; ++j) r[j] = r[j] < i ? r[j] : -1;
+            }
             int ri = join_roots(parent, r, i);
             const int4* row = reinterpret_cast<const int4*>(npos + (size_t)i * W);
             for (int e = 8; e < W; e += 8) {

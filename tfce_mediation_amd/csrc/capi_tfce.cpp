@@ -215,11 +215,11 @@ int tfce_adj_create_ex(const int32_t* row_ptr, const int32_t* col, int32_t V, in
       if ((e = hipMemcpy(a->d_ell, ell.data(), sizeof(int32_t) * ell.size(), hipMemcpyHostToDevice)) != hipSuccess)
         return cleanup(-(int)e, "upload ell");
       a->ell_width = width;
-      // Which neighbours of a row are adjacent to each other (symmetric
-      // surfaces, rows of at most 8): two such neighbours that were both
-      // active before a level are already in one set, so the union step joins
-      // one of each adjacent group.
-      if (symmetric && width <= 8 && V < kPosMask) {
+      // Which neighbours of a row are adjacent to each other, within each
+      // group of 8 consecutive entries (symmetric surfaces): two such
+      // neighbours that were both active before a level are already in one
+      // set, so the union step joins one of each adjacent group.
+      if (symmetric && V < kPosMask) {
         std::vector<int32_t> srt(col, col + nnz);
         for (int32_t v = 0; v < V; ++v) std::sort(srt.begin() + row_ptr[v], srt.begin() + row_ptr[v + 1]);
         auto adjacent = [&](int32_t x, int32_t y) {
@@ -231,9 +231,10 @@ int tfce_adj_create_ex(const int32_t* row_ptr, const int32_t* col, int32_t V, in
             const int32_t uj = ell[(size_t)v * width + j];
             if (uj < 0) continue;
             uint8_t m = 0;
-            for (int32_t k = 0; k < width; ++k) {
+            const int32_t c0 = j & ~7;
+            for (int32_t k = c0; k < std::min(width, c0 + 8); ++k) {
               const int32_t uk = ell[(size_t)v * width + k];
-              if (k != j && uk >= 0 && adjacent(uj, uk)) m |= (uint8_t)(1u << k);
+              if (k != j && uk >= 0 && adjacent(uj, uk)) m |= (uint8_t)(1u << (k - c0));
             }
             nadj[(size_t)v * width + j] = m;
           }

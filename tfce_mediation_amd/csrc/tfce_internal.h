@@ -32,8 +32,8 @@ struct SurfDev {
   const int32_t* col;      // nnz (indices local to the surface)
   const int32_t* ell;      // [V][ell_width] neighbour lists padded with -1, or null
   int32_t ell_width;       // largest list length (0: use the CSR)
-  const uint8_t* nadj;     // [V][ell_width] (width <= 8): bit k of entry j set iff neighbours j and k
-                           // are adjacent; null if not built
+  const uint8_t* nadj;     // [V][ell_width]: bit k of entry j set iff neighbours j and 8*(j/8)+k are
+                           // adjacent (groups of 8 entries); null if not built
   const double* sizepow;   // [V+1] (double)s^E, null when E == 1
   int64_t offset;          // first element of this surface inside a map row
   int32_t V;

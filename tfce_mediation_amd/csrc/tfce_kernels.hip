@@ -158,6 +158,15 @@ __device__ __forceinline__ void find_batch(int32_t* parent, int (&x)[kIlp], cons
     if (live[j]) x[j] = y[j];
 }
 
+// Adjacency bytes of entries e..e+7 of vertex v's table row (zeros past the
+// row's end), as two words.
+__device__ __forceinline__ void chunk_adj(const uint8_t* nadj, int v, int W, int e, unsigned& wlo, unsigned& whi) {
+  if (!nadj) return;
+  const uint32_t* p = reinterpret_cast<const uint32_t*>(nadj + (size_t)v * W + e);
+  wlo = p[0];
+  whi = (e + 4 < W) ? p[1] : 0u;
+}
+
 __device__ __forceinline__ int join_roots(int32_t* parent, int (&r)[9], int ri);
 
 // Neighbour-position entry of a surface with a local adjacency table.
@@ -169,13 +178,11 @@ __device__ __forceinline__ int np_encode(int q, const uint8_t* nadj_row, int j) 
 // joined neighbour of this level, and one neighbour per group of mutually
 // adjacent neighbours that were active before it (such a group is one set
 // already: its edges were joined at earlier levels).
-__device__ __forceinline__ void prune_row(const int (&e)[8], int i, int b0, int (&r)[9]) {
-  unsigned old_m = 0, keep = 0, adj[8];
-  int p[8];
+__device__ __forceinline__ void prune8(const int (&p)[8], const unsigned (&adj)[8], int i, int b0,
+                                       int (&r)[9]) {
+  unsigned old_m = 0, keep = 0;
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
-    p[j] = e[j] & kPosMask;
-    adj[j] = (unsigned)e[j] >> 24;
     if (p[j] < i) {
       if (p[j] < b0) old_m |= 1u << j;
       else keep |= 1u << j;
@@ -198,6 +205,17 @@ __device__ __forceinline__ void prune_row(const int (&e)[8], int i, int b0, int 
   for (int j = 0; j < 8; ++j) r[j] = (keep >> j) & 1u ? p[j] : -1;
 }
 
+__device__ __forceinline__ void prune_row(const int (&e)[8], int i, int b0, int (&r)[9]) {
+  int p[8];
+  unsigned adj[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    p[j] = e[j] & kPosMask;
+    adj[j] = (unsigned)e[j] >> 24;
+  }
+  prune8(p, adj, i, b0, r);
+}
+
 // 3b for one element i (vertex v) and up to 8 of its neighbours u[] (-1: none):
 // gather their positions, find the roots of the joined ones and of i's set
 // (ri, an ancestor) as interleaved chains, then hook every other distinct root
@@ -205,10 +223,22 @@ __device__ __forceinline__ void prune_row(const int (&e)[8], int i, int b0, int 
 // meanwhile) falls back to uf_link.  Returns an ancestor of i's set.
 __device__ __forceinline__ int join_chunk(int32_t* parent, const int32_t* pos, const float* x, const int (&u)[8],
                                           int i, int v, int ri, int b0, int b1, bool symmetric, float sign,
-                                          float xv) {
+                                          float xv, bool prune, unsigned wlo, unsigned whi) {
   int q[8], r[9];
 #pragma unroll
   for (int j = 0; j < 8; ++j) q[j] = u[j] >= 0 ? pos[u[j]] : kNever;
+  if (prune) {
+    // symmetric with adjacency bytes (wlo, whi: the chunk's 8 bytes): joined
+    // = smaller position, old groups pruned
+    unsigned adj[8];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      adj[j] = (wlo >> (8 * j)) & 0xffu;
+      adj[4 + j] = (whi >> (8 * j)) & 0xffu;
+    }
+    prune8(q, adj, i, b0, r);
+    return join_roots(parent, r, ri);
+  }
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
     bool take = q[j] < b1 && q[j] != i;
@@ -621,8 +651,13 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
               const int4 l2 = row[e >> 2];
               const int4 h2 = (e + 4 < W) ? row[(e >> 2) + 1] : make_int4(kNever, kNever, kNever, kNever);
               int r2[9] = {l2.x, l2.y, l2.z, l2.w, h2.x, h2.y, h2.z, h2.w, 0};
+              if (S.nadj) {
+                const int e8[8] = {l2.x, l2.y, l2.z, l2.w, h2.x, h2.y, h2.z, h2.w};
+                prune_row(e8, i, b0, r2);
+              } else {
 #pragma unroll
-              for (int j = 0; j < 8; ++j) r2[j] = r2[j] < i ? r2[j] : -1;
+                for (int j = 0; j < 8; ++j) r2[j] = r2[j] < i ? r2[j] : -1;
+              }
               ri = join_roots(parent, r2, ri);
             }
           }
@@ -649,7 +684,9 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
           int ri = i;  // root (or an ancestor) of i's set so far
           if (ell) {
             int u[8] = {lo0.x, lo0.y, lo0.z, lo0.w, hi0.x, hi0.y, hi0.z, hi0.w};
-            ri = join_chunk(parent, pos, x, u, i, v, ri, b0, b1, symmetric, sign, xv);
+            unsigned wlo = 0, whi = 0;
+            chunk_adj(S.nadj, v, W, 0, wlo, whi);
+            ri = join_chunk(parent, pos, x, u, i, v, ri, b0, b1, symmetric, sign, xv, S.nadj != nullptr, wlo, whi);
             // rows longer than 8: the table is -1 padded, so a chunk that ends
             // in padding is the last
             const int4* row = reinterpret_cast<const int4*>(ell + (size_t)v * W);
@@ -658,7 +695,9 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
               const int4 hi = (e + 4 < W) ? row[(e >> 2) + 1] : none4;
               u[0] = lo.x; u[1] = lo.y; u[2] = lo.z; u[3] = lo.w;
               u[4] = hi.x; u[5] = hi.y; u[6] = hi.z; u[7] = hi.w;
-              ri = join_chunk(parent, pos, x, u, i, v, ri, b0, b1, symmetric, sign, xv);
+              unsigned wlo = 0, whi = 0;
+              chunk_adj(S.nadj, v, W, e, wlo, whi);
+              ri = join_chunk(parent, pos, x, u, i, v, ri, b0, b1, symmetric, sign, xv, S.nadj != nullptr, wlo, whi);
             }
           } else {
             const int e0 = S.row_ptr[v], e1 = S.row_ptr[v + 1];
@@ -666,7 +705,7 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
               int u[8];
 #pragma unroll
               for (int j = 0; j < 8; ++j) u[j] = (e + j < e1) ? S.col[e + j] : -1;
-              ri = join_chunk(parent, pos, x, u, i, v, ri, b0, b1, symmetric, sign, xv);
+              ri = join_chunk(parent, pos, x, u, i, v, ri, b0, b1, symmetric, sign, xv, false, 0u, 0u);
             }
           }
         }

@@ -59,7 +59,10 @@ __device__ __forceinline__ int uf_find(int32_t* parent, int x) {
 // returns the root the two sets had when joined, a good start for a's next
 // union.  The larger root is hooked under the smaller by CAS; a failed CAS
 // (the root was hooked meanwhile) retries from the new roots.
-__device__ __forceinline__ int uf_link(int32_t* parent, int a, int b) {
+// A root of this level's elements (id >= b0) that gets hooked gives its own
+// unit of size to its new parent at once; whatever it had received itself is
+// passed on by the merge step.
+__device__ __forceinline__ int uf_link(int32_t* parent, int32_t* size, int a, int b, int b0) {
   while (true) {
     a = uf_find(parent, a);
     b = uf_find(parent, b);
@@ -67,8 +70,10 @@ __device__ __forceinline__ int uf_link(int32_t* parent, int a, int b) {
     if (a < b) { int t = a; a = b; b = t; }
     int expected = a;
     if (__hip_atomic_compare_exchange_strong(parent + a, &expected, b, __ATOMIC_RELAXED,
-                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
+                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
+      if (a >= b0) __hip_atomic_fetch_add(size + b, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       return b;
+    }
   }
 }
 
@@ -167,7 +172,7 @@ __device__ __forceinline__ void chunk_adj(const uint8_t* nadj, int v, int W, int
   whi = (e + 4 < W) ? p[1] : 0u;
 }
 
-__device__ __forceinline__ int join_roots(int32_t* parent, int (&r)[9], int ri);
+__device__ __forceinline__ int join_roots(int32_t* parent, int32_t* size, int (&r)[9], int ri, int b0);
 
 // Neighbour-position entry of a surface with a local adjacency table.
 __device__ __forceinline__ int np_encode(int q, const uint8_t* nadj_row, int j) {
@@ -221,7 +226,8 @@ __device__ __forceinline__ void prune_row(const int (&e)[8], int i, int b0, int 
 // (ri, an ancestor) as interleaved chains, then hook every other distinct root
 // under the smallest with independent CASes; a failed CAS (the root was hooked
 // meanwhile) falls back to uf_link.  Returns an ancestor of i's set.
-__device__ __forceinline__ int join_chunk(int32_t* parent, const int32_t* pos, const float* x, const int (&u)[8],
+__device__ __forceinline__ int join_chunk(int32_t* parent, int32_t* size, const int32_t* pos, const float* x,
+                                          const int (&u)[8],
                                           int i, int v, int ri, int b0, int b1, bool symmetric, float sign,
                                           float xv, bool prune, unsigned wlo, unsigned whi) {
   int q[8], r[9];
@@ -237,7 +243,7 @@ __device__ __forceinline__ int join_chunk(int32_t* parent, const int32_t* pos, c
       adj[4 + j] = (whi >> (8 * j)) & 0xffu;
     }
     prune8(q, adj, i, b0, r);
-    return join_roots(parent, r, ri);
+    return join_roots(parent, size, r, ri, b0);
   }
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
@@ -254,7 +260,7 @@ __device__ __forceinline__ int join_chunk(int32_t* parent, const int32_t* pos, c
     }
     r[j] = take ? q[j] : -1;
   }
-  return join_roots(parent, r, ri);
+  return join_roots(parent, size, r, ri, b0);
 }
 
 // Compare-exchange for a descending sorting network.
@@ -269,7 +275,7 @@ __device__ __forceinline__ void cx_desc(int& a, int& b) {
 // descending first (Batcher's 19-comparator network), so the empty slots are
 // the last ones and a slot that no lane of the wave uses issues no memory
 // instruction in the chains and CASes below.
-__device__ __forceinline__ int join_roots(int32_t* parent, int (&r)[9], int ri) {
+__device__ __forceinline__ int join_roots(int32_t* parent, int32_t* size, int (&r)[9], int ri, int b0) {
   cx_desc(r[0], r[1]); cx_desc(r[2], r[3]); cx_desc(r[4], r[5]); cx_desc(r[6], r[7]);
   cx_desc(r[0], r[2]); cx_desc(r[1], r[3]); cx_desc(r[4], r[6]); cx_desc(r[5], r[7]);
   cx_desc(r[1], r[2]); cx_desc(r[5], r[6]); cx_desc(r[0], r[4]); cx_desc(r[3], r[7]);
@@ -306,9 +312,29 @@ __device__ __forceinline__ int join_roots(int32_t* parent, int (&r)[9], int ri) 
                                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
   }
+  // units of the hooked roots of this level's elements, added to tgt; the
+  // lanes of a wave adding to the first adding lane's target combine into one
+  // atomic (sums by bit-sliced ballots, valid with inactive lanes)
+  int cnt = 0;
+#pragma unroll
+  for (int j = 0; j < 9; ++j) cnt += (need[j] && ok[j] && r[j] >= b0) ? 1 : 0;
+  const int tgt0 = tgt;
 #pragma unroll
   for (int j = 0; j < 9; ++j)
-    if (!ok[j]) tgt = uf_link(parent, r[j], tgt);
+    if (!ok[j]) tgt = uf_link(parent, size, r[j], tgt, b0);
+  const unsigned long long am = __ballot(cnt > 0);
+  if (am != 0ull) {
+    const int leader = __ffsll((long long)am) - 1;
+    const int tl = __shfl(tgt0, leader, 64);
+    const bool same = cnt > 0 && tgt0 == tl;
+    int tot = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) tot += __popcll(__ballot(same && ((cnt >> b) & 1))) << b;
+    if ((threadIdx.x & 63) == leader)
+      __hip_atomic_fetch_add(size + tl, tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    else if (cnt > 0 && !same)
+      __hip_atomic_fetch_add(size + tgt0, cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
   return tgt;
 }
 
@@ -645,7 +671,7 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
 #pragma unroll
               for (int j = 0; j < 8; ++j) r[j] = r[j] < i ? r[j] : -1;
             }
-            int ri = join_roots(parent, r, i);
+            int ri = join_roots(parent, size, r, i, b0);
             const int4* row = reinterpret_cast<const int4*>(npos + (size_t)i * W);
             for (int e = 8; e < W; e += 8) {
               const int4 l2 = row[e >> 2];
@@ -658,7 +684,7 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
 #pragma unroll
                 for (int j = 0; j < 8; ++j) r2[j] = r2[j] < i ? r2[j] : -1;
               }
-              ri = join_roots(parent, r2, ri);
+              ri = join_roots(parent, size, r2, ri, b0);
             }
           }
         } else {
@@ -686,7 +712,7 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
             int u[8] = {lo0.x, lo0.y, lo0.z, lo0.w, hi0.x, hi0.y, hi0.z, hi0.w};
             unsigned wlo = 0, whi = 0;
             chunk_adj(S.nadj, v, W, 0, wlo, whi);
-            ri = join_chunk(parent, pos, x, u, i, v, ri, b0, b1, symmetric, sign, xv, S.nadj != nullptr, wlo, whi);
+            ri = join_chunk(parent, size, pos, x, u, i, v, ri, b0, b1, symmetric, sign, xv, S.nadj != nullptr, wlo, whi);
             // rows longer than 8: the table is -1 padded, so a chunk that ends
             // in padding is the last
             const int4* row = reinterpret_cast<const int4*>(ell + (size_t)v * W);
@@ -697,7 +723,7 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
               u[4] = hi.x; u[5] = hi.y; u[6] = hi.z; u[7] = hi.w;
               unsigned wlo = 0, whi = 0;
               chunk_adj(S.nadj, v, W, e, wlo, whi);
-              ri = join_chunk(parent, pos, x, u, i, v, ri, b0, b1, symmetric, sign, xv, S.nadj != nullptr, wlo, whi);
+              ri = join_chunk(parent, size, pos, x, u, i, v, ri, b0, b1, symmetric, sign, xv, S.nadj != nullptr, wlo, whi);
             }
           } else {
             const int e0 = S.row_ptr[v], e1 = S.row_ptr[v + 1];
@@ -705,55 +731,56 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
               int u[8];
 #pragma unroll
               for (int j = 0; j < 8; ++j) u[j] = (e + j < e1) ? S.col[e + j] : -1;
-              ri = join_chunk(parent, pos, x, u, i, v, ri, b0, b1, symmetric, sign, xv, false, 0u, 0u);
+              ri = join_chunk(parent, size, pos, x, u, i, v, ri, b0, b1, symmetric, sign, xv, false, 0u, 0u);
             }
           }
         }
         }
         __syncthreads();
-        // 3c: merge sizes (and MAX values) into surviving roots; collect roots
+        // 3c: merge sizes (and MAX values) into surviving roots; collect roots.
+        // An element of this level gave its own unit when it was hooked (3b);
+        // what it received while it was a root (size - 1) goes to its final
+        // root here, so the common case - hooked, never a root to anyone -
+        // costs two consecutive loads and no find.  An old root that was
+        // hooked passes its whole size and (MAX mode) its value.
         const int n_cand = n_roots + (b1 - b0);
         for (int k0 = 0; k0 < n_cand; k0 += kIlp * kTfceThreads) {
-          int c[kIlp], r[kIlp];
-          bool live[kIlp];
+          int c[kIlp], r[kIlp], sz[kIlp];
+          bool live[kIlp], fresh[kIlp], surv[kIlp], need[kIlp];
 #pragma unroll
           for (int j = 0; j < kIlp; ++j) {
             const int k = k0 + j * kTfceThreads + tid;
             live[j] = k < n_cand;
-            c[j] = live[j] ? ((k < n_roots) ? rold[k] : b0 + (k - n_roots)) : 0;
-            r[j] = c[j];
+            fresh[j] = live[j] && k >= n_roots;
+            c[j] = live[j] ? (fresh[j] ? b0 + (k - n_roots) : rold[k]) : 0;
           }
-          find_batch(parent, r, live);
 #pragma unroll
           for (int j = 0; j < kIlp; ++j) {
-            const bool hooked = live[j] && r[j] != c[j];
-            // Additions go to final roots only, so a hooked element of this
-            // level still has size 1 and (MAX mode) value 0: nothing to load,
-            // and its value cannot raise the root's.  Such elements of one
-            // wave mostly join the same root: the lanes sharing the first
-            // one's root add their count with one atomic (same-address atomics
-            // serialise in L2).
-            const bool fresh_hook = hooked && c[j] >= b0;
-            const unsigned long long fm = __ballot(fresh_hook);
-            bool done = false;
-            if (fm != 0ull) {
-              const int leader = __ffsll((long long)fm) - 1;
-              const int rl = __shfl(r[j], leader, 64);
-              const unsigned long long same = __ballot(fresh_hook && r[j] == rl);
-              if ((threadIdx.x & 63) == leader)
-                __hip_atomic_fetch_add(size + rl, __popcll(same), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-              done = fresh_hook && r[j] == rl;
+            r[j] = fresh[j] ? ld(parent + c[j]) : c[j];
+            sz[j] = fresh[j] ? ld(size + c[j]) : 0;
+          }
+#pragma unroll
+          for (int j = 0; j < kIlp; ++j) {
+            surv[j] = fresh[j] && r[j] == c[j];  // old roots: known after the find
+            need[j] = live[j] && (!fresh[j] || (r[j] != c[j] && sz[j] > 1));
+          }
+          find_batch(parent, r, need);
+#pragma unroll
+          for (int j = 0; j < kIlp; ++j) {
+            if (live[j] && !fresh[j]) {
+              if (r[j] != c[j]) {
+                __hip_atomic_fetch_add(size + r[j], ld(size + c[j]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (!FULL)
+                  __hip_atomic_fetch_max(reinterpret_cast<int32_t*>(val) + r[j],
+                                         ld(reinterpret_cast<const int32_t*>(val) + c[j]), __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+              } else {
+                surv[j] = true;
+              }
+            } else if (need[j]) {
+              __hip_atomic_fetch_add(size + r[j], sz[j] - 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
-            if (hooked && !done) {
-              const bool fresh = c[j] >= b0;
-              __hip_atomic_fetch_add(size + r[j], fresh ? 1 : ld(size + c[j]), __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_WORKGROUP);
-              if (!FULL && !fresh)
-                __hip_atomic_fetch_max(reinterpret_cast<int32_t*>(val) + r[j],
-                                       ld(reinterpret_cast<const int32_t*>(val) + c[j]), __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-            }
-            wave_append(live[j] && !hooked, c[j], rnew, &s_nroots_new);
+            wave_append(surv[j], c[j], rnew, &s_nroots_new);
           }
         }
         __syncthreads();

@@ -56,8 +56,30 @@ def check_close(got, ref, E, H):
         assert ulp_diff(got, ref).max() <= 4
 
 
+def mesh_with_extra_edges(order, n_extra, hops, seed):
+    """Icosphere plus random symmetric edges between vertices a few mesh hops
+    apart: irregular degrees (rows wider than 8), as on native meshes."""
+    rp, col = synth.icosphere_csr(order)
+    V = len(rp) - 1
+    rng = np.random.default_rng(seed)
+    nbrs = [set(col[rp[v]:rp[v + 1]].tolist()) for v in range(V)]
+    for _ in range(n_extra):
+        a = int(rng.integers(V))
+        b = a
+        for _ in range(hops):
+            b = int(rng.choice(sorted(nbrs[b])))
+        if b != a:
+            nbrs[a].add(b)
+            nbrs[b].add(a)
+    return adjacency.to_csr([sorted(x) for x in nbrs])
+
+
 GRAPHS = {
     "ico3": lambda: synth.icosphere_csr(3),
+    # rows of width 12: two groups of 8 in the neighbour-position rows
+    "ico4_w12": lambda: mesh_with_extra_edges(4, 3000, 2, 1),
+    # rows of width 20: the table path, pruning per group of 8
+    "ico4_w20": lambda: mesh_with_extra_edges(4, 9000, 2, 2),
     "ico5": lambda: synth.icosphere_csr(5),
     "vox26": lambda: adjacency.voxel_csr(synth.ellipsoid_mask((21, 25, 19), (10, 12, 9), (9, 11, 8.5)), 26),
     "vox6_walls": lambda: adjacency.voxel_csr(np.ones((7, 8, 6), bool), 6),

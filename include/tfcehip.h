@@ -46,7 +46,12 @@ typedef struct tfce_adj tfce_adj_t;
  * (tfce.pyx:27-28).  row_ptr has V+1 entries, col holds indices in [0, V).
  * Symmetry is detected here; an asymmetric list is honoured the way
  * fast_tfce.hpp:44-66 uses it (an edge counts when it is listed by the later-
- * activated end).  Blocking: uploads the CSR before returning. */
+ * activated end).  Blocking: uploads the CSR before returning, with a
+ * fixed-width copy of the lists (longest list <= 32) and, for a symmetric
+ * adjacency, which entries of each list are adjacent to each other (within
+ * groups of 8 entries; host work O(nnz * 8 log degree)).  The sweep uses both
+ * to read a vertex's neighbours in one row and to join each group of
+ * already-connected neighbours once. */
 int tfce_adj_create(const int32_t *row_ptr, const int32_t *col, int32_t V, int64_t nnz,
                     float H, float E, tfce_adj_t **out);
 int tfce_adj_create_ex(const int32_t *row_ptr, const int32_t *col, int32_t V, int64_t nnz,

@@ -119,9 +119,10 @@ __device__ __forceinline__ void wave_append(bool take, int c, int32_t* list, int
 constexpr int kIlp = 4;  // independent find chains per thread in the merge phases
 
 // Roots of up to N elements at once (x[j] < 0: none), the loads of the
-// chains interleaved.  Path splitting: a visited node is re-pointed to its
-// grandparent.
-template <int N>
+// chains interleaved.  SPLIT: path splitting, a visited node is re-pointed to
+// its grandparent (the merge step compresses; the union step only reads, its
+// writes cost more than the shorter chains save).
+template <int N, bool SPLIT = true>
 __device__ __forceinline__ void find_many(int32_t* parent, int (&x)[N]) {
   int prev[N];
   bool go[N];
@@ -143,7 +144,7 @@ __device__ __forceinline__ void find_many(int32_t* parent, int (&x)[N]) {
       if (p[j] == x[j]) {
         go[j] = false;
       } else {
-        if (prev[j] >= 0) st(parent + prev[j], p[j]);
+        if (SPLIT && prev[j] >= 0) st(parent + prev[j], p[j]);
         prev[j] = x[j];
         x[j] = p[j];
         any = true;
@@ -287,7 +288,7 @@ __device__ __forceinline__ int join_roots(int32_t* parent, int32_t* size, int (&
   int rn[8];
 #pragma unroll
   for (int j = 0; j < 8; ++j) rn[j] = r[j];
-  find_many<8>(parent, rn);
+  find_many<8, false>(parent, rn);
 #pragma unroll
   for (int j = 0; j < 8; ++j) r[j] = rn[j];
   r[8] = ri;

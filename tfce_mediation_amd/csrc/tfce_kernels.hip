@@ -526,6 +526,10 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
     }
     __syncthreads();
     // ---- phase 2: level order ----------------------------------------------
+    // With neighbour-position rows the MAX sweep never reads the level order
+    // (vertex of a position): only the positions are written then.
+    const bool use_np = symmetric && ell != nullptr && W <= a.np_w;
+    const bool need_order = FULL || !use_np;
     if (L > 0) {
       for (int v0 = tid; v0 < V; v0 += 8 * kTfceThreads) {
         int lv[8];
@@ -539,7 +543,7 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
           if (lv[j] != kNever) {
             const int v = v0 + j * kTfceThreads;
             const int q = atomicAdd(&s_cur[lv[j]], 1);
-            order[q] = v;
+            if (need_order) order[q] = v;
             pos[v] = q;
           }
         }
@@ -553,7 +557,6 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
     // pass runs in vertex order, where rows of the table and neighbours'
     // positions are near each other, and the union step then reads one row per
     // element, consecutive across a wave, instead of a table row and a gather.
-    const bool use_np = symmetric && ell != nullptr && W <= a.np_w;
     if (use_np && L > 0) {
       // four vertices per thread and pass, every load of a 4-wide column
       // chunk issued before its stores

@@ -643,7 +643,10 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
         for (int i = b0 + tid; i < b1; i += kTfceThreads) {
           parent[i] = i;
           size[i] = 1;
-          val[i] = (FULL && enhn) ? enhn[order[i]] : 0.0f;
+          // (MAX mode: a value is first read as a root's at its own level's
+          // increment, which takes 0 for roots of that level, so none is
+          // written here)
+          if (FULL) val[i] = enhn ? enhn[order[i]] : 0.0f;
         }
         __syncthreads();
         // 3b: join new vertices with their active neighbours, 8 at a time
@@ -807,7 +810,7 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
 #pragma unroll
           for (int j = 0; j < kIlp; ++j) {
             sz[j] = r[j] >= 0 ? ld(size + r[j]) : 0;
-            cv[j] = r[j] >= 0 ? __int_as_float(ld(reinterpret_cast<const int32_t*>(val) + r[j])) : 0.0f;
+            cv[j] = (r[j] >= 0 && r[j] < b0) ? __int_as_float(ld(reinterpret_cast<const int32_t*>(val) + r[j])) : 0.0f;
           }
 #pragma unroll
           for (int j = 0; j < kIlp; ++j)

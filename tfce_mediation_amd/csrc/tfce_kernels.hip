@@ -502,7 +502,6 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
           if (v < V) {
             const int lev = level_of(sign * xs[j], s_T, L, T0, dT);
             if (lev != kNever) atomicAdd(&s_cur[lev], 1);
-            pos[v] = lev;
           }
         }
       }
@@ -531,12 +530,21 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
     const bool use_np = symmetric && ell != nullptr && W <= a.np_w;
     const bool need_order = FULL || !use_np;
     if (L > 0) {
+      // the levels again from the map (cheaper than storing and re-reading
+      // them); every element's position is written here, kNever if inactive
       for (int v0 = tid; v0 < V; v0 += 8 * kTfceThreads) {
+        float xs[8];
         int lv[8];
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
           const int v = v0 + j * kTfceThreads;
-          lv[j] = v < V ? pos[v] : kNever;
+          xs[j] = v < V ? x[v] : 0.0f;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int v = v0 + j * kTfceThreads;
+          lv[j] = v < V ? level_of(sign * xs[j], s_T, L, T0, dT) : kNever;
+          if (v < V && lv[j] == kNever) pos[v] = kNever;
         }
 #pragma unroll
         for (int j = 0; j < 8; ++j) {

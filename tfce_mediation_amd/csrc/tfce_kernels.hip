@@ -703,53 +703,53 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
             }
           }
         } else {
-        int v_nxt = (b0 + tid < b1) ? order[b0 + tid] : 0;
-        int v_nn = (b0 + tid + kTfceThreads < b1) ? order[b0 + tid + kTfceThreads] : 0;
-        int4 lo_nxt = none4, hi_nxt = none4;
-        if (ell && b0 + tid < b1) {
-          const int4* rn = reinterpret_cast<const int4*>(ell + (size_t)v_nxt * W);
-          lo_nxt = rn[0];
-          if (W > 4) hi_nxt = rn[1];
-        }
-        for (int i = b0 + tid; i < b1; i += kTfceThreads) {
-          const int v = v_nxt;
-          const int4 lo0 = lo_nxt, hi0 = hi_nxt;
-          if (ell && i + kTfceThreads < b1) {
-            const int4* rn = reinterpret_cast<const int4*>(ell + (size_t)v_nn * W);
+          int v_nxt = (b0 + tid < b1) ? order[b0 + tid] : 0;
+          int v_nn = (b0 + tid + kTfceThreads < b1) ? order[b0 + tid + kTfceThreads] : 0;
+          int4 lo_nxt = none4, hi_nxt = none4;
+          if (ell && b0 + tid < b1) {
+            const int4* rn = reinterpret_cast<const int4*>(ell + (size_t)v_nxt * W);
             lo_nxt = rn[0];
             if (W > 4) hi_nxt = rn[1];
           }
-          v_nxt = v_nn;
-          v_nn = (i + 2 * kTfceThreads < b1) ? order[i + 2 * kTfceThreads] : 0;
-          const float xv = symmetric ? 0.0f : sign * x[v];
-          int ri = i;  // root (or an ancestor) of i's set so far
-          if (ell) {
-            int u[8] = {lo0.x, lo0.y, lo0.z, lo0.w, hi0.x, hi0.y, hi0.z, hi0.w};
-            unsigned wlo = 0, whi = 0;
-            chunk_adj(S.nadj, v, W, 0, wlo, whi);
-            ri = join_chunk(parent, size, pos, x, u, i, v, ri, b0, b1, symmetric, sign, xv, S.nadj != nullptr, wlo, whi);
-            // rows longer than 8: the table is -1 padded, so a chunk that ends
-            // in padding is the last
-            const int4* row = reinterpret_cast<const int4*>(ell + (size_t)v * W);
-            for (int e = 8; e < W && u[7] >= 0; e += 8) {
-              const int4 lo = row[e >> 2];
-              const int4 hi = (e + 4 < W) ? row[(e >> 2) + 1] : none4;
-              u[0] = lo.x; u[1] = lo.y; u[2] = lo.z; u[3] = lo.w;
-              u[4] = hi.x; u[5] = hi.y; u[6] = hi.z; u[7] = hi.w;
-              unsigned wlo = 0, whi = 0;
-              chunk_adj(S.nadj, v, W, e, wlo, whi);
-              ri = join_chunk(parent, size, pos, x, u, i, v, ri, b0, b1, symmetric, sign, xv, S.nadj != nullptr, wlo, whi);
+          for (int i = b0 + tid; i < b1; i += kTfceThreads) {
+            const int v = v_nxt;
+            const int4 lo0 = lo_nxt, hi0 = hi_nxt;
+            if (ell && i + kTfceThreads < b1) {
+              const int4* rn = reinterpret_cast<const int4*>(ell + (size_t)v_nn * W);
+              lo_nxt = rn[0];
+              if (W > 4) hi_nxt = rn[1];
             }
-          } else {
-            const int e0 = S.row_ptr[v], e1 = S.row_ptr[v + 1];
-            for (int e = e0; e < e1; e += 8) {
-              int u[8];
+            v_nxt = v_nn;
+            v_nn = (i + 2 * kTfceThreads < b1) ? order[i + 2 * kTfceThreads] : 0;
+            const float xv = symmetric ? 0.0f : sign * x[v];
+            int ri = i;  // root (or an ancestor) of i's set so far
+            if (ell) {
+              int u[8] = {lo0.x, lo0.y, lo0.z, lo0.w, hi0.x, hi0.y, hi0.z, hi0.w};
+              unsigned wlo = 0, whi = 0;
+              chunk_adj(S.nadj, v, W, 0, wlo, whi);
+              ri = join_chunk(parent, size, pos, x, u, i, v, ri, b0, b1, symmetric, sign, xv, S.nadj != nullptr, wlo, whi);
+              // rows longer than 8: the table is -1 padded, so a chunk that ends
+              // in padding is the last
+              const int4* row = reinterpret_cast<const int4*>(ell + (size_t)v * W);
+              for (int e = 8; e < W && u[7] >= 0; e += 8) {
+                const int4 lo = row[e >> 2];
+                const int4 hi = (e + 4 < W) ? row[(e >> 2) + 1] : none4;
+                u[0] = lo.x; u[1] = lo.y; u[2] = lo.z; u[3] = lo.w;
+                u[4] = hi.x; u[5] = hi.y; u[6] = hi.z; u[7] = hi.w;
+                unsigned wlo = 0, whi = 0;
+                chunk_adj(S.nadj, v, W, e, wlo, whi);
+                ri = join_chunk(parent, size, pos, x, u, i, v, ri, b0, b1, symmetric, sign, xv, S.nadj != nullptr, wlo, whi);
+              }
+            } else {
+              const int e0 = S.row_ptr[v], e1 = S.row_ptr[v + 1];
+              for (int e = e0; e < e1; e += 8) {
+                int u[8];
 #pragma unroll
-              for (int j = 0; j < 8; ++j) u[j] = (e + j < e1) ? S.col[e + j] : -1;
-              ri = join_chunk(parent, size, pos, x, u, i, v, ri, b0, b1, symmetric, sign, xv, false, 0u, 0u);
+                for (int j = 0; j < 8; ++j) u[j] = (e + j < e1) ? S.col[e + j] : -1;
+                ri = join_chunk(parent, size, pos, x, u, i, v, ri, b0, b1, symmetric, sign, xv, false, 0u, 0u);
+              }
             }
           }
-        }
         }
         __syncthreads();
         // 3c: merge sizes (and MAX values) into surviving roots; collect roots.

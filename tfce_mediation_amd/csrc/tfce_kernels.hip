@@ -120,8 +120,9 @@ constexpr int kIlp = 4;  // independent find chains per thread in the merge phas
 
 // Roots of up to N elements at once (x[j] < 0: none), the loads of the
 // chains interleaved.  SPLIT: path splitting, a visited node is re-pointed to
-// its grandparent (the merge step compresses; the union step only reads, its
-// writes cost more than the shorter chains save).
+// its grandparent.  The union step with neighbour-position rows only reads
+// (its writes cost more than the shorter chains save there; the merge step
+// compresses); with the wide voxel tables splitting pays.
 template <int N, bool SPLIT = true>
 __device__ __forceinline__ void find_many(int32_t* parent, int (&x)[N]) {
   int prev[N];
@@ -173,6 +174,7 @@ __device__ __forceinline__ void chunk_adj(const uint8_t* nadj, int v, int W, int
   whi = (e + 4 < W) ? p[1] : 0u;
 }
 
+template <bool SPLIT>
 __device__ __forceinline__ int join_roots(int32_t* parent, int32_t* size, int (&r)[9], int ri, int b0);
 
 // Neighbour-position entry of a surface with a local adjacency table.
@@ -244,7 +246,7 @@ __device__ __forceinline__ int join_chunk(int32_t* parent, int32_t* size, const 
       adj[4 + j] = (whi >> (8 * j)) & 0xffu;
     }
     prune8(q, adj, i, b0, r);
-    return join_roots(parent, size, r, ri, b0);
+    return join_roots<true>(parent, size, r, ri, b0);
   }
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
@@ -261,7 +263,7 @@ __device__ __forceinline__ int join_chunk(int32_t* parent, int32_t* size, const 
     }
     r[j] = take ? q[j] : -1;
   }
-  return join_roots(parent, size, r, ri, b0);
+  return join_roots<true>(parent, size, r, ri, b0);
 }
 
 // Compare-exchange for a descending sorting network.
@@ -276,6 +278,7 @@ __device__ __forceinline__ void cx_desc(int& a, int& b) {
 // descending first (Batcher's 19-comparator network), so the empty slots are
 // the last ones and a slot that no lane of the wave uses issues no memory
 // instruction in the chains and CASes below.
+template <bool SPLIT>
 __device__ __forceinline__ int join_roots(int32_t* parent, int32_t* size, int (&r)[9], int ri, int b0) {
   cx_desc(r[0], r[1]); cx_desc(r[2], r[3]); cx_desc(r[4], r[5]); cx_desc(r[6], r[7]);
   cx_desc(r[0], r[2]); cx_desc(r[1], r[3]); cx_desc(r[4], r[6]); cx_desc(r[5], r[7]);
@@ -288,7 +291,7 @@ __device__ __forceinline__ int join_roots(int32_t* parent, int32_t* size, int (&
   int rn[8];
 #pragma unroll
   for (int j = 0; j < 8; ++j) rn[j] = r[j];
-  find_many<8, false>(parent, rn);
+  find_many<8, SPLIT>(parent, rn);
 #pragma unroll
   for (int j = 0; j < 8; ++j) r[j] = rn[j];
   r[8] = ri;
@@ -686,7 +689,7 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
 #pragma unroll
               for (int j = 0; j < 8; ++j) r[j] = r[j] < i ? r[j] : -1;
             }
-            int ri = join_roots(parent, size, r, i, b0);
+            int ri = join_roots<false>(parent, size, r, i, b0);
             const int4* row = reinterpret_cast<const int4*>(npos + (size_t)i * W);
             for (int e = 8; e < W; e += 8) {
               const int4 l2 = row[e >> 2];
@@ -699,7 +702,7 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
 #pragma unroll
                 for (int j = 0; j < 8; ++j) r2[j] = r2[j] < i ? r2[j] : -1;
               }
-              ri = join_roots(parent, size, r2, ri, b0);
+              ri = join_roots<false>(parent, size, r2, ri, b0);
             }
           }
         } else {

@@ -149,17 +149,19 @@ def cpu_baseline(cfg, perms_np, n_workers, per_worker, settings):
                 aggregate_rate=n_done / max(worker_times), values=values, used_reference=res[0][2])
 
 
-def _measured_traffic(kernel, workload):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
-    summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or (None, None)."""
+def _measured_traffic(kernel, workload, images):
+    """HBM bytes per launch of `kernel` over `images` images, from the
+    committed rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE,
+    copy-calibrated, measured over the images its line names and scaled per
+    image), or (None, None)."""
     path = os.path.join(ROOT, "profiles", "%s_tfce_hbm_traffic.md" % workload)
     try:
         with open(path) as f:
             for line in f:
                 parts = line.split()
-                if len(parts) == 4 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel \
+                if len(parts) == 5 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel \
                         and parts[2] == workload:
-                    return int(parts[3]), os.path.relpath(path, ROOT)
+                    return int(parts[3]) * images // int(parts[4]), os.path.relpath(path, ROOT)
     except OSError:
         pass
     return None, None
@@ -422,7 +424,7 @@ def main():
         bytes_per_sign += N_THR * (4 * (V_s + 1) + 4 * nnz_s + 8 * V_s) + 8 * V_s
     units = P * n_signs
     achieved = bytes_per_sign * units / (tfce_ms * 1e-3) / 1e9
-    traffic, traffic_src = _measured_traffic("tfce_batch_kernel<false>", name)
+    traffic, traffic_src = _measured_traffic("tfce_batch_kernel<false>", name, units * len(cfg["surfaces"]))
     gemm_flops = flops_per_perm * P
     workload, data = WORKLOADS[name]
     if name == "c3":

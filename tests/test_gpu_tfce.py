@@ -74,8 +74,22 @@ def mesh_with_extra_edges(order, n_extra, hops, seed):
     return adjacency.to_csr([sorted(x) for x in nbrs])
 
 
+def shuffled_rows(row_ptr, col, seed):
+    """The same graph with every row's neighbours in random order."""
+    rng = np.random.default_rng(seed)
+    col = col.copy()
+    for v in range(len(row_ptr) - 1):
+        rng.shuffle(col[row_ptr[v]:row_ptr[v + 1]])
+    return row_ptr, col
+
+
 GRAPHS = {
     "ico3": lambda: synth.icosphere_csr(3),
+    # rows of at most six neighbours in no particular order: the library puts
+    # them in ring order; masked: open rings (boundary vertices) and short rows
+    "ico4_shuffled": lambda: shuffled_rows(*synth.icosphere_csr(4), 7),
+    "ico5_masked": lambda: adjacency.induced_subgraph(*synth.icosphere_csr(5),
+                                                      np.random.default_rng(3).random(10242) < 0.85),
     # rows of width 12: two groups of 8 in the neighbour-position rows
     "ico4_w12": lambda: mesh_with_extra_edges(4, 3000, 2, 1),
     # rows of width 20: the table path, pruning per group of 8
@@ -141,7 +155,7 @@ def canon(labels):
     return out
 
 
-@pytest.mark.parametrize("gname", ["ico5", "vox26", "vox6_walls"])
+@pytest.mark.parametrize("gname", ["ico5", "ico5_masked", "vox26", "vox6_walls"])
 def test_partitions_equal_at_every_level(torch, gname):
     from tfce_mediation_amd.tfce import Adjacency
     row_ptr, col = GRAPHS[gname]()
@@ -163,7 +177,7 @@ def test_partitions_equal_at_every_level(torch, gname):
         np.testing.assert_array_equal(canon(lab.cpu().numpy()), ref)
 
 
-@pytest.mark.parametrize("gname,E", [("ico5", 1.0), ("vox26", 0.5)])
+@pytest.mark.parametrize("gname,E", [("ico5", 1.0), ("ico5_masked", 1.0), ("vox26", 0.5)])
 def test_batch_max_matches_oracle(torch, gname, E):
     from tfce_mediation_amd.tfce import Adjacency
     row_ptr, col = GRAPHS[gname]()

@@ -225,22 +225,55 @@ int tfce_adj_create_ex(const int32_t* row_ptr, const int32_t* col, int32_t V, in
         auto adjacent = [&](int32_t x, int32_t y) {
           return std::binary_search(srt.begin() + row_ptr[x], srt.begin() + row_ptr[x + 1], y);
         };
-        std::vector<uint8_t> nadj((size_t)V * width, 0);
-        for (int32_t v = 0; v < V; ++v)
-          for (int32_t j = 0; j < width; ++j) {
-            const int32_t uj = ell[(size_t)v * width + j];
-            if (uj < 0) continue;
-            uint8_t m = 0;
-            const int32_t c0 = j & ~7;
-            for (int32_t k = c0; k < std::min(width, c0 + 8); ++k) {
-              const int32_t uk = ell[(size_t)v * width + k];
-              if (k != j && uk >= 0 && adjacent(uj, uk)) m |= (uint8_t)(1u << (k - c0));
+        int32_t max_deg = 0;
+        for (int32_t v = 0; v < V; ++v) max_deg = std::max(max_deg, row_ptr[v + 1] - row_ptr[v]);
+        if (max_deg <= kRingMaxDeg && V < kRingNone && width == 8) {
+          // Rows of at most six neighbours: each row is put in ring order
+          // (walk to an unvisited neighbour adjacent to the last one while
+          // there is one) and the adjacency of ring-consecutive neighbours
+          // kept as one link byte.  Adjacencies the walk does not place next
+          // to each other are left out, which only keeps more candidates.
+          std::vector<uint8_t> link(V, 0);
+          for (int32_t v = 0; v < V; ++v) {
+            int32_t* row = ell.data() + (size_t)v * width;
+            const int deg = row_ptr[v + 1] - row_ptr[v];
+            for (int j = 1; j < deg; ++j) {
+              for (int k = j; k < deg; ++k)
+                if (adjacent(row[j - 1], row[k])) {
+                  std::swap(row[j], row[k]);
+                  break;
+                }
             }
-            nadj[(size_t)v * width + j] = m;
+            uint8_t m = 0;
+            for (int j = 0; j + 1 < deg; ++j)
+              if (adjacent(row[j], row[j + 1])) m |= (uint8_t)(1u << j);
+            if (deg >= 3 && adjacent(row[deg - 1], row[0])) m |= (uint8_t)(1u << 5);
+            link[v] = m;
           }
-        if ((e = hipMalloc(&a->d_nadj, nadj.size())) != hipSuccess) return cleanup(-(int)e, "hipMalloc(nadj)");
-        if ((e = hipMemcpy(a->d_nadj, nadj.data(), nadj.size(), hipMemcpyHostToDevice)) != hipSuccess)
-          return cleanup(-(int)e, "upload nadj");
+          if ((e = hipMemcpy(a->d_ell, ell.data(), sizeof(int32_t) * ell.size(), hipMemcpyHostToDevice)) != hipSuccess)
+            return cleanup(-(int)e, "upload ring-ordered ell");
+          if ((e = hipMalloc(&a->d_nadj, link.size())) != hipSuccess) return cleanup(-(int)e, "hipMalloc(links)");
+          if ((e = hipMemcpy(a->d_nadj, link.data(), link.size(), hipMemcpyHostToDevice)) != hipSuccess)
+            return cleanup(-(int)e, "upload links");
+          a->flags |= kSurfRing;
+        } else {
+          std::vector<uint8_t> nadj((size_t)V * width, 0);
+          for (int32_t v = 0; v < V; ++v)
+            for (int32_t j = 0; j < width; ++j) {
+              const int32_t uj = ell[(size_t)v * width + j];
+              if (uj < 0) continue;
+              uint8_t m = 0;
+              const int32_t c0 = j & ~7;
+              for (int32_t k = c0; k < std::min(width, c0 + 8); ++k) {
+                const int32_t uk = ell[(size_t)v * width + k];
+                if (k != j && uk >= 0 && adjacent(uj, uk)) m |= (uint8_t)(1u << (k - c0));
+              }
+              nadj[(size_t)v * width + j] = m;
+            }
+          if ((e = hipMalloc(&a->d_nadj, nadj.size())) != hipSuccess) return cleanup(-(int)e, "hipMalloc(nadj)");
+          if ((e = hipMemcpy(a->d_nadj, nadj.data(), nadj.size(), hipMemcpyHostToDevice)) != hipSuccess)
+            return cleanup(-(int)e, "upload nadj");
+        }
       }
     }
   }
@@ -369,6 +402,7 @@ int tfce_adj_info(const tfce_adj_t* a, int32_t* V, int64_t* nnz, int32_t* symmet
 // Width of the per-image neighbour-position table the sweep keeps for a surface
 // (symmetric, fixed-width table of at most kNposMaxWidth), 0 if none.
 static int32_t np_width(const tfce_adj* a) {
+  if (a->flags & kSurfRing) return kRingRowWords;
   return ((a->flags & kSurfSymmetric) && a->ell_width > 0 && a->ell_width <= kNposMaxWidth) ? a->ell_width : 0;
 }
 

@@ -213,6 +213,49 @@ __device__ __forceinline__ void prune8(const int (&p)[8], const unsigned (&adj)[
   for (int j = 0; j < 8; ++j) r[j] = (keep >> j) & 1u ? p[j] : -1;
 }
 
+// Ring-packed neighbour-position row of one element: the positions of
+// entries 0..5 (a.x .. b.y; kNever: inactive) in 18 bits each from bit 0, the
+// link byte from bit 108 and the row length from bit 114.
+__device__ __forceinline__ int4 ring_pack(const int4 a, const int4 b, unsigned link, int deg) {
+  auto c = [](int q) -> uint64_t { return (uint64_t)(uint32_t)(q == kNever ? kRingNone : q); };
+  const uint64_t lo = c(a.x) | c(a.y) << 18 | c(a.z) << 36 | c(a.w) << 54;
+  const uint64_t hi = c(a.w) >> 10 | c(b.x) << 8 | c(b.y) << 26 | (uint64_t)link << 44 | (uint64_t)deg << 50;
+  return make_int4((int)(uint32_t)lo, (int)(uint32_t)(lo >> 32), (int)(uint32_t)hi, (int)(uint32_t)(hi >> 32));
+}
+
+// Positions (kRingNone: inactive or no entry) and the adjacency masks among
+// them of a ring-packed row.
+__device__ __forceinline__ void ring_unpack(const int4 w, int (&p)[8], unsigned (&adj)[8]) {
+  const uint64_t lo = (uint64_t)(uint32_t)w.x | (uint64_t)(uint32_t)w.y << 32;
+  const uint64_t hi = (uint64_t)(uint32_t)w.z | (uint64_t)(uint32_t)w.w << 32;
+  const uint64_t M = kRingNone;
+  p[0] = (int)(lo & M);
+  p[1] = (int)((lo >> 18) & M);
+  p[2] = (int)((lo >> 36) & M);
+  p[3] = (int)(((lo >> 54) | (hi << 10)) & M);
+  p[4] = (int)((hi >> 8) & M);
+  p[5] = (int)((hi >> 26) & M);
+  p[6] = p[7] = kRingNone;
+  const unsigned link = (unsigned)(hi >> 44) & 63u;
+  const int deg = (int)(hi >> 50) & 7;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) adj[j] = 0;
+#pragma unroll
+  for (int j = 0; j < 5; ++j)
+    if ((link >> j) & 1u) {
+      adj[j] |= 2u << j;
+      adj[j + 1] |= 1u << j;
+    }
+  if (link & 32u) {
+#pragma unroll
+    for (int j = 2; j < kRingMaxDeg; ++j)
+      if (j == deg - 1) {
+        adj[j] |= 1u;
+        adj[0] |= 1u << j;
+      }
+  }
+}
+
 __device__ __forceinline__ void prune_row(const int (&e)[8], int i, int b0, int (&r)[9]) {
   int p[8];
   unsigned adj[8];
@@ -530,7 +573,8 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
     // ---- phase 2: level order ----------------------------------------------
     // With neighbour-position rows the MAX sweep never reads the level order
     // (vertex of a position): only the positions are written then.
-    const bool use_np = symmetric && ell != nullptr && W <= a.np_w;
+    const bool ring = (S.flags & kSurfRing) != 0;
+    const bool use_np = symmetric && ell != nullptr && (ring ? kRingRowWords : W) <= a.np_w;
     const bool need_order = FULL || !use_np;
     if (L > 0) {
       // the levels again from the map (cheaper than storing and re-reading
@@ -577,6 +621,36 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
         for (int j = 0; j < 4; ++j) {
           const int v = v0 + j * kTfceThreads;
           qv[j] = v < V ? pos[v] : kNever;
+        }
+        if (ring) {
+          // one 16-byte row per element: positions, links, length
+          int4 uu[8];
+#pragma unroll
+          for (int j = 0; j < 8; ++j)
+            uu[j] = qv[j >> 1] != kNever
+                        ? reinterpret_cast<const int4*>(ell + (size_t)(v0 + (j >> 1) * kTfceThreads) * 8)[j & 1]
+                        : make_int4(-1, -1, -1, -1);
+          unsigned link[4];
+#pragma unroll
+          for (int t = 0; t < 4; ++t) link[t] = qv[t] != kNever ? S.nadj[v0 + t * kTfceThreads] : 0u;
+          int4 o[8];
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            o[2 * t].x = uu[2 * t].x >= 0 ? pos[uu[2 * t].x] : kNever;
+            o[2 * t].y = uu[2 * t].y >= 0 ? pos[uu[2 * t].y] : kNever;
+            o[2 * t].z = uu[2 * t].z >= 0 ? pos[uu[2 * t].z] : kNever;
+            o[2 * t].w = uu[2 * t].w >= 0 ? pos[uu[2 * t].w] : kNever;
+            o[2 * t + 1].x = uu[2 * t + 1].x >= 0 ? pos[uu[2 * t + 1].x] : kNever;
+            o[2 * t + 1].y = uu[2 * t + 1].y >= 0 ? pos[uu[2 * t + 1].y] : kNever;
+          }
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            if (qv[t] == kNever) continue;
+            const int4 a0 = uu[2 * t], a1 = uu[2 * t + 1];
+            const int deg = (a0.x >= 0) + (a0.y >= 0) + (a0.z >= 0) + (a0.w >= 0) + (a1.x >= 0) + (a1.y >= 0);
+            reinterpret_cast<int4*>(npos)[qv[t]] = ring_pack(o[2 * t], o[2 * t + 1], link[t], deg);
+          }
+          continue;
         }
         if (W == 8) {
           // the common mesh width: both column chunks at once
@@ -665,7 +739,20 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
         // first row and the vertex of the one after are loaded one iteration
         // ahead.
         const int4 none4 = make_int4(-1, -1, -1, -1);
-        if (use_np) {
+        if (use_np && ring) {
+          // one packed row per element, the next one loaded ahead
+          const int4* rows = reinterpret_cast<const int4*>(npos);
+          int4 w_n = (b0 + tid < b1) ? rows[b0 + tid] : none4;
+          for (int i = b0 + tid; i < b1; i += kTfceThreads) {
+            const int4 w = w_n;
+            if (i + kTfceThreads < b1) w_n = rows[i + kTfceThreads];
+            int p[8], r[9];
+            unsigned adj[8];
+            ring_unpack(w, p, adj);
+            prune8(p, adj, i, b0, r);
+            join_roots<false>(parent, size, r, i, b0);
+          }
+        } else if (use_np) {
           // rows of neighbour positions: a joined neighbour is one with a
           // smaller position (active before this level, or earlier in it)
           int4 lo_n = none4, hi_n = none4;

@@ -233,7 +233,9 @@ int tfce_adj_create_ex(const int32_t* row_ptr, const int32_t* col, int32_t V, in
           // there is one) and the adjacency of ring-consecutive neighbours
           // kept as one link byte.  Adjacencies the walk does not place next
           // to each other are left out, which only keeps more candidates.
-          std::vector<uint8_t> link(V, 0);
+          // Each row is then packed into 16 bytes: six 18-bit indices, the
+          // link byte from bit 108, the length from bit 114.
+          std::vector<uint32_t> packed((size_t)V * 4, 0);
           for (int32_t v = 0; v < V; ++v) {
             int32_t* row = ell.data() + (size_t)v * width;
             const int deg = row_ptr[v + 1] - row_ptr[v];
@@ -248,13 +250,21 @@ int tfce_adj_create_ex(const int32_t* row_ptr, const int32_t* col, int32_t V, in
             for (int j = 0; j + 1 < deg; ++j)
               if (adjacent(row[j], row[j + 1])) m |= (uint8_t)(1u << j);
             if (deg >= 3 && adjacent(row[deg - 1], row[0])) m |= (uint8_t)(1u << 5);
-            link[v] = m;
+            uint64_t f[kRingMaxDeg];
+            for (int j = 0; j < kRingMaxDeg; ++j) f[j] = j < deg ? (uint64_t)row[j] : (uint64_t)kRingNone;
+            const uint64_t lo = f[0] | f[1] << 18 | f[2] << 36 | f[3] << 54;
+            const uint64_t hi = f[3] >> 10 | f[4] << 8 | f[5] << 26 | (uint64_t)m << 44 | (uint64_t)deg << 50;
+            packed[(size_t)v * 4 + 0] = (uint32_t)lo;
+            packed[(size_t)v * 4 + 1] = (uint32_t)(lo >> 32);
+            packed[(size_t)v * 4 + 2] = (uint32_t)hi;
+            packed[(size_t)v * 4 + 3] = (uint32_t)(hi >> 32);
           }
           if ((e = hipMemcpy(a->d_ell, ell.data(), sizeof(int32_t) * ell.size(), hipMemcpyHostToDevice)) != hipSuccess)
             return cleanup(-(int)e, "upload ring-ordered ell");
-          if ((e = hipMalloc(&a->d_nadj, link.size())) != hipSuccess) return cleanup(-(int)e, "hipMalloc(links)");
-          if ((e = hipMemcpy(a->d_nadj, link.data(), link.size(), hipMemcpyHostToDevice)) != hipSuccess)
-            return cleanup(-(int)e, "upload links");
+          const size_t pbytes = packed.size() * sizeof(uint32_t);
+          if ((e = hipMalloc(&a->d_nadj, pbytes)) != hipSuccess) return cleanup(-(int)e, "hipMalloc(ring rows)");
+          if ((e = hipMemcpy(a->d_nadj, packed.data(), pbytes, hipMemcpyHostToDevice)) != hipSuccess)
+            return cleanup(-(int)e, "upload ring rows");
           a->flags |= kSurfRing;
         } else {
           std::vector<uint8_t> nadj((size_t)V * width, 0);

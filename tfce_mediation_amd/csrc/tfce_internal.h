@@ -26,12 +26,13 @@ constexpr int32_t kStatusTooManyLevels = -3;
 constexpr int32_t kSurfSymmetric = 1;     // every edge is listed from both ends
 constexpr int32_t kSurfZeroInLevels = 2;  // the full image holds zeros outside this element set
 constexpr int32_t kSurfZeroInScale = 4;   // the rescale max includes those zeros
-// Symmetric surface whose table rows hold at most kRingMaxDeg neighbours in
-// ring order (a closed triangle mesh's vertex link) and whose elements number
-// less than kRingNone: nadj then holds one byte per element, bit j (j < 5) set
-// iff neighbours j and j+1 are adjacent and bit 5 iff the last and the first
-// are, and a neighbour-position row packs the kRingMaxDeg positions in 18 bits
-// each, with that byte, into one 16-byte word (kRingRowWords).
+// Symmetric surface whose rows hold at most kRingMaxDeg neighbours, put in
+// ring order (a closed triangle mesh's vertex link), with fewer than kRingNone
+// elements.  nadj then holds one 16-byte packed row per element: the
+// neighbours' indices in 18 bits each (kRingNone past the row's end), a link
+// byte (bit j < 5 set iff neighbours j and j+1 are adjacent, bit 5 iff the
+// last and the first are) and the row length; a neighbour-position row packs
+// positions the same way (kRingRowWords words).
 constexpr int32_t kSurfRing = 8;
 constexpr int kRingMaxDeg = 6;
 constexpr int32_t kRingNone = 0x3ffff;  // 18-bit position of an inactive neighbour
@@ -43,8 +44,8 @@ struct SurfDev {
   const int32_t* ell;      // [V][ell_width] neighbour lists padded with -1, or null
   int32_t ell_width;       // largest list length (0: use the CSR)
   const uint8_t* nadj;     // [V][ell_width]: bit k of entry j set iff neighbours j and 8*(j/8)+k are
-                           // adjacent (groups of 8 entries); [V] ring links with kSurfRing; null if
-                           // not built
+                           // adjacent (groups of 8 entries); [V][16] packed rows with kSurfRing;
+                           // null if not built
   const double* sizepow;   // [V+1] (double)s^E, null when E == 1
   int64_t offset;          // first element of this surface inside a map row
   int32_t V;

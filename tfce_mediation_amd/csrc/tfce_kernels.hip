@@ -213,19 +213,18 @@ __device__ __forceinline__ void prune8(const int (&p)[8], const unsigned (&adj)[
   for (int j = 0; j < 8; ++j) r[j] = (keep >> j) & 1u ? p[j] : -1;
 }
 
-// Ring-packed neighbour-position row of one element: the positions of
-// entries 0..5 (a.x .. b.y; kNever: inactive) in 18 bits each from bit 0, the
-// link byte from bit 108 and the row length from bit 114.
-__device__ __forceinline__ int4 ring_pack(const int4 a, const int4 b, unsigned link, int deg) {
-  auto c = [](int q) -> uint64_t { return (uint64_t)(uint32_t)(q == kNever ? kRingNone : q); };
-  const uint64_t lo = c(a.x) | c(a.y) << 18 | c(a.z) << 36 | c(a.w) << 54;
-  const uint64_t hi = c(a.w) >> 10 | c(b.x) << 8 | c(b.y) << 26 | (uint64_t)link << 44 | (uint64_t)deg << 50;
+// Ring-packed row of one element: six 18-bit fields from bit 0 (neighbour
+// indices in the table, positions in the per-image rows; kRingNone: none or
+// inactive), then the 9-bit tail: link byte (bits 108-113), row length
+// (bits 114-116).
+__device__ __forceinline__ int4 ring_pack(const int (&q)[kRingMaxDeg], unsigned tail) {
+  auto c = [](int v) -> uint64_t { return (uint64_t)(uint32_t)(v == kNever ? kRingNone : v); };
+  const uint64_t lo = c(q[0]) | c(q[1]) << 18 | c(q[2]) << 36 | c(q[3]) << 54;
+  const uint64_t hi = c(q[3]) >> 10 | c(q[4]) << 8 | c(q[5]) << 26 | (uint64_t)tail << 44;
   return make_int4((int)(uint32_t)lo, (int)(uint32_t)(lo >> 32), (int)(uint32_t)hi, (int)(uint32_t)(hi >> 32));
 }
 
-// Positions (kRingNone: inactive or no entry) and the adjacency masks among
-// them of a ring-packed row.
-__device__ __forceinline__ void ring_unpack(const int4 w, int (&p)[8], unsigned (&adj)[8]) {
+__device__ __forceinline__ void ring_fields(const int4 w, int (&p)[kRingMaxDeg], unsigned& tail) {
   const uint64_t lo = (uint64_t)(uint32_t)w.x | (uint64_t)(uint32_t)w.y << 32;
   const uint64_t hi = (uint64_t)(uint32_t)w.z | (uint64_t)(uint32_t)w.w << 32;
   const uint64_t M = kRingNone;
@@ -235,9 +234,20 @@ __device__ __forceinline__ void ring_unpack(const int4 w, int (&p)[8], unsigned 
   p[3] = (int)(((lo >> 54) | (hi << 10)) & M);
   p[4] = (int)((hi >> 8) & M);
   p[5] = (int)((hi >> 26) & M);
+  tail = (unsigned)(hi >> 44) & 0x1ffu;
+}
+
+// Positions (kRingNone: inactive or no entry) and the adjacency masks among
+// them of a ring-packed row.
+__device__ __forceinline__ void ring_unpack(const int4 w, int (&p)[8], unsigned (&adj)[8]) {
+  int f[kRingMaxDeg];
+  unsigned tail;
+  ring_fields(w, f, tail);
+#pragma unroll
+  for (int j = 0; j < kRingMaxDeg; ++j) p[j] = f[j];
   p[6] = p[7] = kRingNone;
-  const unsigned link = (unsigned)(hi >> 44) & 63u;
-  const int deg = (int)(hi >> 50) & 7;
+  const unsigned link = tail & 63u;
+  const int deg = (int)(tail >> 6);
 #pragma unroll
   for (int j = 0; j < 8; ++j) adj[j] = 0;
 #pragma unroll
@@ -623,33 +633,24 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
           qv[j] = v < V ? pos[v] : kNever;
         }
         if (ring) {
-          // one 16-byte row per element: positions, links, length
-          int4 uu[8];
+          // one 16-byte row per element: the packed table row (neighbour
+          // indices, links, length) in, the same row of positions out
+          const int4* tab = reinterpret_cast<const int4*>(S.nadj);
+          int4 tw[4];
 #pragma unroll
-          for (int j = 0; j < 8; ++j)
-            uu[j] = qv[j >> 1] != kNever
-                        ? reinterpret_cast<const int4*>(ell + (size_t)(v0 + (j >> 1) * kTfceThreads) * 8)[j & 1]
-                        : make_int4(-1, -1, -1, -1);
-          unsigned link[4];
-#pragma unroll
-          for (int t = 0; t < 4; ++t) link[t] = qv[t] != kNever ? S.nadj[v0 + t * kTfceThreads] : 0u;
-          int4 o[8];
+          for (int t = 0; t < 4; ++t) tw[t] = qv[t] != kNever ? tab[v0 + t * kTfceThreads] : make_int4(-1, -1, -1, -1);
+          int q[4][kRingMaxDeg];
+          unsigned tail[4];
 #pragma unroll
           for (int t = 0; t < 4; ++t) {
-            o[2 * t].x = uu[2 * t].x >= 0 ? pos[uu[2 * t].x] : kNever;
-            o[2 * t].y = uu[2 * t].y >= 0 ? pos[uu[2 * t].y] : kNever;
-            o[2 * t].z = uu[2 * t].z >= 0 ? pos[uu[2 * t].z] : kNever;
-            o[2 * t].w = uu[2 * t].w >= 0 ? pos[uu[2 * t].w] : kNever;
-            o[2 * t + 1].x = uu[2 * t + 1].x >= 0 ? pos[uu[2 * t + 1].x] : kNever;
-            o[2 * t + 1].y = uu[2 * t + 1].y >= 0 ? pos[uu[2 * t + 1].y] : kNever;
+            int u[kRingMaxDeg];
+            ring_fields(tw[t], u, tail[t]);
+#pragma unroll
+            for (int j = 0; j < kRingMaxDeg; ++j) q[t][j] = u[j] != kRingNone ? pos[u[j]] : kNever;
           }
 #pragma unroll
-          for (int t = 0; t < 4; ++t) {
-            if (qv[t] == kNever) continue;
-            const int4 a0 = uu[2 * t], a1 = uu[2 * t + 1];
-            const int deg = (a0.x >= 0) + (a0.y >= 0) + (a0.z >= 0) + (a0.w >= 0) + (a1.x >= 0) + (a1.y >= 0);
-            reinterpret_cast<int4*>(npos)[qv[t]] = ring_pack(o[2 * t], o[2 * t + 1], link[t], deg);
-          }
+          for (int t = 0; t < 4; ++t)
+            if (qv[t] != kNever) reinterpret_cast<int4*>(npos)[qv[t]] = ring_pack(q[t], tail[t]);
           continue;
         }
         if (W == 8) {

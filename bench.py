@@ -9,7 +9,9 @@ Synthetic smoothed data with five effect patches, generated on the GPU.
 `--config c1|c2|c4|c5` measures the other workloads of the project's table the
 same way (c5: mediation, +z only).
 
-A step is one batch of P = 256 permutations through the whole path:
+A step is one batch of P permutations through the whole path (P = 256 at
+c3; by default the engine's batch size, which gives the TFCE launch four
+items per compute unit: 512 for c1, c2 and c5):
 permuted-design contraction on MFMA (split-bf16, 3 products) with the fused
 t epilogue, then one persistent TFCE launch over every (surface, permutation,
 sign) item, then the max over surfaces.  Inputs are resident in HBM before the
@@ -205,7 +207,8 @@ def main():
     ap.add_argument("--config", default="c3", choices=sorted(WORKLOADS))
     ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--batch", type=int, default=None,
+                    help="permutations per step (default: the engine's default_batch())")
     ap.add_argument("--precision", default="bf16x3")
     ap.add_argument("--plain-perms", type=int, default=16)
     ap.add_argument("--cpu-workers", type=int, default=16)
@@ -247,7 +250,7 @@ def main():
     torch.cuda.synchronize()
     log(f"[bench] {name}: data and engine ready in {time.time() - t_setup:.1f}s: V={cfg['V']} n={cfg['n']} "
         f"surfaces={len(cfg['surfaces'])}")
-    P = args.batch
+    P = args.batch if args.batch is not None else eng.default_batch()
     n = cfg["n"]
     perm_batches = [synth.permutations(n, P, 1000 + s, "cuda") for s in range(args.warmup + args.steps)]
 

@@ -86,3 +86,17 @@ def test_pipeline_matches_batch_by_batch():
             full = e.run(sum(int(b.shape[0]) for b in batches), batch=64, perms=torch.cat(batches).contiguous(),
                          pipelined=pipelined)
             np.testing.assert_array_equal(full, np.concatenate([o.cpu().numpy() for o in outs]))
+
+
+def test_default_batch_fills_the_compute_units():
+    """Four TFCE items per compute unit, in multiples of 64 permutations
+    between 256 and 1024."""
+    import torch
+    from tfce_mediation_amd.tfce import default_batch
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    print(f"[gpu] default_batch with {cus} compute units", flush=True)
+    for items in (1, 2, 3, 4, 50):
+        b = default_batch(items)
+        assert b % 64 == 0 and 256 <= b <= 1024
+        assert items * b >= min(4 * cus, items * 1024)
+        assert b == 256 or items * (b - 64) < 4 * cus

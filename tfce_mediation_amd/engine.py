@@ -16,7 +16,7 @@ import numpy as np
 from . import synth
 from .ols import PermutedOLS
 from .adjacency import locality_layout
-from .tfce import Adjacency, SurfaceSet, TFCE_ADJ_IMPLICIT_ZERO, TFCE_ADJ_ZERO_IN_SCALE
+from .tfce import Adjacency, SurfaceSet, TFCE_ADJ_IMPLICIT_ZERO, TFCE_ADJ_ZERO_IN_SCALE, default_batch
 
 
 class PermutationEngine:
@@ -158,14 +158,22 @@ class PermutationEngine:
         run.graph = g
         return run
 
-    def run(self, n_perm, batch=256, seed=0, perms=None, pipelined=False):
+    def default_batch(self, device=None):
+        """Permutations per batch (``tfce.default_batch`` for this engine's
+        surfaces and signs)."""
+        return default_batch(len(self.csrs) * (2 if self.sign_mask == 3 else 1), device)
+
+    def run(self, n_perm, batch=None, seed=0, perms=None, pipelined=False):
         """Null of n_perm permutations: [n_perm, n_signs] (host numpy), batch
-        after batch on the current stream, or through ``pipeline``.  Measured
+        after batch (``batch`` permutations each, default ``default_batch()``)
+        on the current stream, or through ``pipeline``.  Measured
         on MI355X at 327,684 vertices the one-stream order is the faster one:
         a contraction launched beside the sweep takes compute units the
         sweep's workgroups would occupy and delays them by more than the
         contraction takes alone."""
         import torch
+        if batch is None:
+            batch = self.default_batch()
         batches = []
         done = 0
         while done < n_perm:
@@ -228,5 +236,6 @@ class MediationEngine:
         return per if per_surface else per.max(0).values
 
     graph = PermutationEngine.graph
+    default_batch = PermutationEngine.default_batch
     run = PermutationEngine.run
     pipeline = PermutationEngine.pipeline

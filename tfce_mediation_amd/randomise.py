@@ -30,7 +30,7 @@ import numpy as np
 from . import synth
 from .adjacency import induced_subgraph, locality_layout, to_csr
 from .ols import PermutedOLS, total_ss
-from .tfce import Adjacency, SurfaceSet, TFCE_ADJ_IMPLICIT_ZERO, TFCE_ADJ_ZERO_IN_SCALE
+from .tfce import Adjacency, SurfaceSet, TFCE_ADJ_IMPLICIT_ZERO, TFCE_ADJ_ZERO_IN_SCALE, default_batch
 
 
 def permutation_indices(n, perm_numbers, seed=0, blocks=None):
@@ -78,8 +78,12 @@ def _in_layout(csrs, offsets, Y, vd=None):
     return csrs2, Y, vd
 
 
-def _batches(perms, batch):
+def _batches(perms, batch, items_per_perm):
+    """(first, device batch) over perms; ``batch`` None: tfce.default_batch
+    for ``items_per_perm`` (surface, sign) items of each TFCE launch."""
     import torch
+    if batch is None:
+        batch = default_batch(items_per_perm)
     for a in range(0, len(perms), batch):
         yield a, torch.as_tensor(np.ascontiguousarray(perms[a:a + batch], dtype=np.int32)).cuda()
 
@@ -91,7 +95,7 @@ def _write(path, values, fmt):
 
 
 def vertex_regress(ny, pred_x, adjac_lh, adjac_rh, bin_mask_lh, bin_mask_rh, optstfce, perms,
-                   vdensity_lh=1, vdensity_rh=1, specifyvars=None, out_dir=None, batch=256,
+                   vdensity_lh=1, vdensity_rh=1, specifyvars=None, out_dir=None, batch=None,
                    precision="bf16x3"):
     """Null rows [n_perm, 2] (+t, -t) per contrast, dict {tnum: array}; written
     to perm_tstat_con<tnum>_TFCE_maxVertex.csv when out_dir is given."""
@@ -118,7 +122,7 @@ def vertex_regress(ny, pred_x, adjac_lh, adjac_rh, bin_mask_lh, bin_mask_rh, opt
     surfs = SurfaceSet([Adjacency(lh, H, E, fl), Adjacency(rh, H, E, fr)], [0, nv_lh])
     model = PermutedOLS(ny, X, contrasts=contrasts, perm_columns=perm_cols, precision=precision)
     nulls = {c: [] for c in contrasts}
-    for _, pb in _batches(perms, batch):
+    for _, pb in _batches(perms, batch, 2 * 2):
         t = model.t(pb)
         for ci, c in enumerate(contrasts):
             nulls[c].append(surfs.batch(t[ci], 3, vdensity=vd).max(0).values.cpu().numpy())
@@ -129,7 +133,7 @@ def vertex_regress(ny, pred_x, adjac_lh, adjac_rh, bin_mask_lh, bin_mask_rh, opt
     return nulls
 
 
-def voxel_regress(ny, pred_x, adjac, optstfce, perms, specifyvars=None, out_dir=None, batch=256,
+def voxel_regress(ny, pred_x, adjac, optstfce, perms, specifyvars=None, out_dir=None, batch=None,
                   precision="bf16x3", ancova=False):
     """Voxel randomise: null rows [n_perm, 2] per contrast.  With ``ancova``
     (the -f branch, voxel_tfce_multiple_regression_randomise.py:79-89) the
@@ -151,14 +155,14 @@ def voxel_regress(ny, pred_x, adjac, optstfce, perms, specifyvars=None, out_dir=
     if ancova:
         tss = total_ss(ny)
         null = []
-        for _, pb in _batches(perms, batch):
+        for _, pb in _batches(perms, batch, 1):
             null.append(surfs.batch(model.fstat(pb, tss), 1)[0, :, 0].cpu().numpy())
         nulls = {"fstat": np.concatenate(null, 0)}
         if out_dir is not None:
             _write(os.path.join(out_dir, "perm_fstat_TFCE_maxVoxel.csv"), nulls["fstat"], "%1.4f")
         return nulls
     nulls = {c: [] for c in contrasts}
-    for _, pb in _batches(perms, batch):
+    for _, pb in _batches(perms, batch, 2):
         t = model.t(pb)
         t = t.nan_to_num(nan=0.0)  # perm_tvalues[np.isnan(perm_tvalues)] = 0 (randomise.py:109)
         for ci, c in enumerate(contrasts):
@@ -219,7 +223,7 @@ class MediationModel:
 
 
 def vertex_mediation(y, pred_x, depend_y, medtype, adjac_lh, adjac_rh, bin_mask_lh, bin_mask_rh, optstfce, perms,
-                     vdensity_lh=1, vdensity_rh=1, out_dir=None, batch=256, precision="bf16x3"):
+                     vdensity_lh=1, vdensity_rh=1, out_dir=None, batch=None, precision="bf16x3"):
     """Mediation randomise: null [n_perm] of the positive Sobel-z TFCE maxima."""
     import torch
     H, E = float(optstfce[0]), float(optstfce[1])
@@ -234,7 +238,7 @@ def vertex_mediation(y, pred_x, depend_y, medtype, adjac_lh, adjac_rh, bin_mask_
     surfs = SurfaceSet([Adjacency(lh, H, E, fl), Adjacency(rh, H, E, fr)], [0, nv_lh])
     med = MediationModel(y, pred_x, depend_y, medtype, precision=precision)
     out = []
-    for _, pb in _batches(perms, batch):
+    for _, pb in _batches(perms, batch, 2):
         z = med.z(pb)
         out.append(surfs.batch(z, 1, vdensity=vd).max(0).values[:, 0].cpu().numpy())
     null = np.concatenate(out)
@@ -244,7 +248,7 @@ def vertex_mediation(y, pred_x, depend_y, medtype, adjac_lh, adjac_rh, bin_mask_
 
 
 def mmr(merge_y, pred_x, surfaces, offsets, settings, perms, groups=None, merged=True, zero_in_image=None,
-        vdensity=None, out_dir=None, batch=256, precision="bf16x3"):
+        vdensity=None, out_dir=None, batch=None, precision="bf16x3"):
     """Multi-surface randomise.  surfaces: CSR per surface (in-mask elements),
     offsets: their first column in merge_y; settings: (H, E) per surface;
     groups: settings-group id per surface (default: one per distinct setting).
@@ -270,7 +274,7 @@ def mmr(merge_y, pred_x, surfaces, offsets, settings, perms, groups=None, merged
         sets[g] = (idx, SurfaceSet(adjs, [offsets[i] for i in idx]))
     zi = zero_in_image or {}
     res = {}
-    for _, pb in _batches(perms, batch):
+    for _, pb in _batches(perms, batch, 2 * min(len(idx) for idx, _ in sets.values())):
         t = model.t(pb)
         for ci, c in enumerate(contrasts):
             for g, (idx, ss) in sets.items():

@@ -110,6 +110,23 @@ class CreateAdjSet:
         enhn[:] = d_enh.cpu().numpy()
 
 
+def default_batch(items_per_perm, device=None):
+    """Permutations per batch for a TFCE launch with ``items_per_perm``
+    (surface, sign) items per permutation: the smallest multiple of 64, at
+    least 256 and at most 1024, that gives the launch four items per compute
+    unit, the number of sweep workgroups a compute unit holds at once.  With
+    fewer items (one surface with +/-t, or two with +z only, at 256
+    permutations) half of every compute unit idles: measured on MI355X, 512
+    permutations instead of 256 give 8% more permutations/s on a 232k-voxel
+    volume (one surface, +/-t) and 25% more for mediation on a 327,684-vertex
+    mesh (two surfaces, +z)."""
+    import torch
+    dev = device if device is not None else torch.cuda.current_device()
+    cus = torch.cuda.get_device_properties(dev).multi_processor_count
+    want = -(-4 * cus // max(1, int(items_per_perm)))
+    return int(min(1024, max(256, -(-want // 64) * 64)))
+
+
 class SurfaceSet:
     """Several adjacencies laid out along one map row (tfce_surfset_t).
 

@@ -277,6 +277,23 @@ __device__ __forceinline__ void prune_row(const int (&e)[8], int i, int b0, int 
   prune8(p, adj, i, b0, r);
 }
 
+// Candidates of element i from one chunk of 8 entries u[] (-1: none) of a
+// symmetric table row with adjacency bytes (wlo, whi): the neighbours'
+// positions, joined = smaller position, adjacent old ones pruned.
+__device__ __forceinline__ void chunk_cands(const int32_t* pos, const int (&u)[8], int i, int b0, unsigned wlo,
+                                            unsigned whi, int (&r)[9]) {
+  int q[8];
+  unsigned adj[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) q[j] = u[j] >= 0 ? pos[u[j]] : kNever;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    adj[j] = (wlo >> (8 * j)) & 0xffu;
+    adj[4 + j] = (whi >> (8 * j)) & 0xffu;
+  }
+  prune8(q, adj, i, b0, r);
+}
+
 // 3b for one element i (vertex v) and up to 8 of its neighbours u[] (-1: none):
 // gather their positions, find the roots of the joined ones and of i's set
 // (ri, an ancestor) as interleaved chains, then hook every other distinct root
@@ -326,6 +343,15 @@ __device__ __forceinline__ void cx_desc(int& a, int& b) {
   b = lo;
 }
 
+// r[0..7] sorted descending (Batcher's 19-comparator network).
+__device__ __forceinline__ void sort8_desc(int (&r)[9]) {
+  cx_desc(r[0], r[1]); cx_desc(r[2], r[3]); cx_desc(r[4], r[5]); cx_desc(r[6], r[7]);
+  cx_desc(r[0], r[2]); cx_desc(r[1], r[3]); cx_desc(r[4], r[6]); cx_desc(r[5], r[7]);
+  cx_desc(r[1], r[2]); cx_desc(r[5], r[6]); cx_desc(r[0], r[4]); cx_desc(r[3], r[7]);
+  cx_desc(r[1], r[5]); cx_desc(r[2], r[6]); cx_desc(r[1], r[4]); cx_desc(r[3], r[6]);
+  cx_desc(r[2], r[4]); cx_desc(r[3], r[5]); cx_desc(r[3], r[4]);
+}
+
 // The root-joining half of join_chunk: r[0..7] are positions to join with
 // i's set (ri, an ancestor of it), -1 for none.  The positions are sorted
 // descending first (Batcher's 19-comparator network), so the empty slots are
@@ -333,11 +359,7 @@ __device__ __forceinline__ void cx_desc(int& a, int& b) {
 // instruction in the chains and CASes below.
 template <bool SPLIT>
 __device__ __forceinline__ int join_roots(int32_t* parent, int32_t* size, int (&r)[9], int ri, int b0) {
-  cx_desc(r[0], r[1]); cx_desc(r[2], r[3]); cx_desc(r[4], r[5]); cx_desc(r[6], r[7]);
-  cx_desc(r[0], r[2]); cx_desc(r[1], r[3]); cx_desc(r[4], r[6]); cx_desc(r[5], r[7]);
-  cx_desc(r[1], r[2]); cx_desc(r[5], r[6]); cx_desc(r[0], r[4]); cx_desc(r[3], r[7]);
-  cx_desc(r[1], r[5]); cx_desc(r[2], r[6]); cx_desc(r[1], r[4]); cx_desc(r[3], r[6]);
-  cx_desc(r[2], r[4]); cx_desc(r[3], r[5]); cx_desc(r[3], r[4]);
+  sort8_desc(r);
   // i's own set enters as ri without a lookup: if ri is no longer a root its
   // CAS below fails and uf_link takes over, and if it is the smallest, the
   // other roots hooked under it still land in its set under a smaller id
@@ -489,7 +511,7 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_prepare_kernel(TfceBatchArg
 }
 
 template <bool FULL>
-__global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs a) {
+__global__ __launch_bounds__(kTfceThreads, 4) void tfce_batch_kernel(TfceBatchArgs a) {
   __shared__ float s_T[kMaxLevels];
   __shared__ float s_HH[kMaxLevels];
   __shared__ int s_start[kMaxLevels + 1];
@@ -814,7 +836,49 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_batch_kernel(TfceBatchArgs 
             v_nn = (i + 2 * kTfceThreads < b1) ? order[i + 2 * kTfceThreads] : 0;
             const float xv = symmetric ? 0.0f : sign * x[v];
             int ri = i;  // root (or an ancestor) of i's set so far
-            if (ell) {
+            if (ell && S.nadj && !ring) {
+              // symmetric rows with adjacency bytes: two chunks per join.
+              // Each chunk's pruned candidates sorted descending; the larger
+              // half of the bitonic split of the pair holds all of them
+              // when they are 8 or fewer, the smaller half any others.
+              const int4* row = reinterpret_cast<const int4*>(ell + (size_t)v * W);
+              int ua[8] = {lo0.x, lo0.y, lo0.z, lo0.w, hi0.x, hi0.y, hi0.z, hi0.w};
+              for (int e = 0;; e += 16) {
+                unsigned wlo = 0, whi = 0;
+                chunk_adj(S.nadj, v, W, e, wlo, whi);
+                int ra[9], rb[9];
+                chunk_cands(pos, ua, i, b0, wlo, whi, ra);
+                sort8_desc(ra);
+                const bool more = e + 8 < W && ua[7] >= 0;
+                int ub7 = -1;
+#pragma unroll
+                for (int j = 0; j < 9; ++j) rb[j] = -1;
+                if (more) {
+                  const int4 lo = row[(e + 8) >> 2];
+                  const int4 hi = (e + 12 < W) ? row[((e + 8) >> 2) + 1] : none4;
+                  const int ub[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+                  chunk_adj(S.nadj, v, W, e + 8, wlo, whi);
+                  chunk_cands(pos, ub, i, b0, wlo, whi, rb);
+                  ub7 = hi.w;
+                }
+                sort8_desc(rb);
+                int top[9], bot[9];
+                bool spill = false;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                  top[j] = max(ra[j], rb[7 - j]);
+                  bot[j] = min(ra[j], rb[7 - j]);
+                  spill |= bot[j] >= 0;
+                }
+                ri = join_roots<true>(parent, size, top, ri, b0);
+                if (__any(spill)) ri = join_roots<true>(parent, size, bot, ri, b0);
+                if (!more || e + 16 >= W || ub7 < 0) break;
+                const int4 lo = row[(e + 16) >> 2];
+                const int4 hi = (e + 20 < W) ? row[((e + 16) >> 2) + 1] : none4;
+                ua[0] = lo.x; ua[1] = lo.y; ua[2] = lo.z; ua[3] = lo.w;
+                ua[4] = hi.x; ua[5] = hi.y; ua[6] = hi.z; ua[7] = hi.w;
+              }
+            } else if (ell) {
               int u[8] = {lo0.x, lo0.y, lo0.z, lo0.w, hi0.x, hi0.y, hi0.z, hi0.w};
               unsigned wlo = 0, whi = 0;
               chunk_adj(S.nadj, v, W, 0, wlo, whi);

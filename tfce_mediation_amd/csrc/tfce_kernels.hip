@@ -511,7 +511,7 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_prepare_kernel(TfceBatchArg
 }
 
 template <bool FULL>
-__global__ __launch_bounds__(kTfceThreads, 4) void tfce_batch_kernel(TfceBatchArgs a) {
+__global__ __launch_bounds__(kTfceThreads, FULL ? 3 : 4) void tfce_batch_kernel(TfceBatchArgs a) {
   __shared__ float s_T[kMaxLevels];
   __shared__ float s_HH[kMaxLevels];
   __shared__ int s_start[kMaxLevels + 1];
@@ -837,46 +837,73 @@ __global__ __launch_bounds__(kTfceThreads, 4) void tfce_batch_kernel(TfceBatchAr
             const float xv = symmetric ? 0.0f : sign * x[v];
             int ri = i;  // root (or an ancestor) of i's set so far
             if (ell && S.nadj && !ring) {
-              // symmetric rows with adjacency bytes: two chunks per join.
+              // symmetric rows with adjacency bytes: four chunks per join.
               // Each chunk's pruned candidates sorted descending; the larger
-              // half of the bitonic split of the pair holds all of them
-              // when they are 8 or fewer, the smaller half any others.
+              // half of the bitonic split of two sorted lists holds all of
+              // their candidates when they are 8 or fewer, the smaller half
+              // any others (joined at once, rarely needed).  Two such
+              // halves, sorted, are split again.
               const int4* row = reinterpret_cast<const int4*>(ell + (size_t)v * W);
-              int ua[8] = {lo0.x, lo0.y, lo0.z, lo0.w, hi0.x, hi0.y, hi0.z, hi0.w};
-              for (int e = 0;; e += 16) {
-                unsigned wlo = 0, whi = 0;
-                chunk_adj(S.nadj, v, W, e, wlo, whi);
-                int ra[9], rb[9];
-                chunk_cands(pos, ua, i, b0, wlo, whi, ra);
-                sort8_desc(ra);
-                const bool more = e + 8 < W && ua[7] >= 0;
-                int ub7 = -1;
+              int u[8] = {lo0.x, lo0.y, lo0.z, lo0.w, hi0.x, hi0.y, hi0.z, hi0.w};
+              for (int e = 0;; e += 32) {
+                int half[2][9];
+                bool live = true;  // this chunk holds entries
 #pragma unroll
-                for (int j = 0; j < 9; ++j) rb[j] = -1;
-                if (more) {
-                  const int4 lo = row[(e + 8) >> 2];
-                  const int4 hi = (e + 12 < W) ? row[((e + 8) >> 2) + 1] : none4;
-                  const int ub[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
-                  chunk_adj(S.nadj, v, W, e + 8, wlo, whi);
-                  chunk_cands(pos, ub, i, b0, wlo, whi, rb);
-                  ub7 = hi.w;
+                for (int p2 = 0; p2 < 2; ++p2) {
+                  int ra[9], rb[9];
+#pragma unroll
+                  for (int j = 0; j < 9; ++j) ra[j] = rb[j] = -1;
+#pragma unroll
+                  for (int c = 0; c < 2; ++c) {
+                    const int ec = e + 16 * p2 + 8 * c;
+                    if (!(p2 == 0 && c == 0)) {
+                      live = live && ec < W && u[7] >= 0;
+                      if (live) {
+                        const int4 lo = row[ec >> 2];
+                        const int4 hi = (ec + 4 < W) ? row[(ec >> 2) + 1] : none4;
+                        u[0] = lo.x; u[1] = lo.y; u[2] = lo.z; u[3] = lo.w;
+                        u[4] = hi.x; u[5] = hi.y; u[6] = hi.z; u[7] = hi.w;
+                      }
+                    }
+                    if (live) {
+                      unsigned wlo = 0, whi = 0;
+                      chunk_adj(S.nadj, v, W, ec, wlo, whi);
+                      if (c == 0) {
+                        chunk_cands(pos, u, i, b0, wlo, whi, ra);
+                        sort8_desc(ra);
+                      } else {
+                        chunk_cands(pos, u, i, b0, wlo, whi, rb);
+                        sort8_desc(rb);
+                      }
+                    }
+                  }
+                  int bot[9];
+                  bool spill = false;
+#pragma unroll
+                  for (int j = 0; j < 8; ++j) {
+                    half[p2][j] = max(ra[j], rb[7 - j]);
+                    bot[j] = min(ra[j], rb[7 - j]);
+                    spill |= bot[j] >= 0;
+                  }
+                  if (__any(spill)) ri = join_roots<true>(parent, size, bot, ri, b0);
                 }
-                sort8_desc(rb);
+                sort8_desc(half[0]);
+                sort8_desc(half[1]);
                 int top[9], bot[9];
                 bool spill = false;
 #pragma unroll
                 for (int j = 0; j < 8; ++j) {
-                  top[j] = max(ra[j], rb[7 - j]);
-                  bot[j] = min(ra[j], rb[7 - j]);
+                  top[j] = max(half[0][j], half[1][7 - j]);
+                  bot[j] = min(half[0][j], half[1][7 - j]);
                   spill |= bot[j] >= 0;
                 }
                 ri = join_roots<true>(parent, size, top, ri, b0);
                 if (__any(spill)) ri = join_roots<true>(parent, size, bot, ri, b0);
-                if (!more || e + 16 >= W || ub7 < 0) break;
-                const int4 lo = row[(e + 16) >> 2];
-                const int4 hi = (e + 20 < W) ? row[((e + 16) >> 2) + 1] : none4;
-                ua[0] = lo.x; ua[1] = lo.y; ua[2] = lo.z; ua[3] = lo.w;
-                ua[4] = hi.x; ua[5] = hi.y; ua[6] = hi.z; ua[7] = hi.w;
+                if (!live || e + 32 >= W || u[7] < 0) break;
+                const int4 lo = row[(e + 32) >> 2];
+                const int4 hi = (e + 36 < W) ? row[((e + 32) >> 2) + 1] : none4;
+                u[0] = lo.x; u[1] = lo.y; u[2] = lo.z; u[3] = lo.w;
+                u[4] = hi.x; u[5] = hi.y; u[6] = hi.z; u[7] = hi.w;
               }
             } else if (ell) {
               int u[8] = {lo0.x, lo0.y, lo0.z, lo0.w, hi0.x, hi0.y, hi0.z, hi0.w};

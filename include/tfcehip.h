@@ -49,9 +49,12 @@ typedef struct tfce_adj tfce_adj_t;
  * activated end).  Blocking: uploads the CSR before returning, with a
  * fixed-width copy of the lists (longest list <= 32) and, for a symmetric
  * adjacency, which entries of each list are adjacent to each other (within
- * groups of 8 entries; host work O(nnz * 8 log degree)).  The sweep uses both
- * to read a vertex's neighbours in one row and to join each group of
- * already-connected neighbours once. */
+ * groups of 8 entries; host work O(nnz * 8 log degree)).  When every list
+ * holds at most 6 neighbours and V < 2^18 - 1 (closed triangle meshes such
+ * as fsaverage), the lists are put in ring order instead and packed with the
+ * adjacency of ring-consecutive neighbours into one 16-byte row per vertex.
+ * The sweep uses these to read a vertex's neighbours in one row and to join
+ * each group of already-connected neighbours once. */
 int tfce_adj_create(const int32_t *row_ptr, const int32_t *col, int32_t V, int64_t nnz,
                     float H, float E, tfce_adj_t **out);
 int tfce_adj_create_ex(const int32_t *row_ptr, const int32_t *col, int32_t V, int64_t nnz,

@@ -206,6 +206,9 @@ int tfce_adj_create_ex(const int32_t* row_ptr, const int32_t* col, int32_t V, in
     int32_t width = 0;
     for (int32_t v = 0; v < V; ++v) width = std::max(width, row_ptr[v + 1] - row_ptr[v]);
     width = (width + 3) & ~3;  // rows of whole 16-byte loads
+    // rows wider than 16 entries (26-connected voxels) are padded to 32: one
+    // 128-byte line per row, and 32-byte rows of adjacency bytes
+    if (width > 16 && width <= kEllMaxWidth) width = kEllMaxWidth;
     if (width > 0 && width <= kEllMaxWidth) {
       std::vector<int32_t> ell((size_t)V * width, -1);
       for (int32_t v = 0; v < V; ++v)

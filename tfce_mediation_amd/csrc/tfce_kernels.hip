@@ -845,6 +845,15 @@ __global__ __launch_bounds__(kTfceThreads, FULL ? 3 : 4) void tfce_batch_kernel(
               // halves, sorted, are split again.
               const int4* row = reinterpret_cast<const int4*>(ell + (size_t)v * W);
               int u[8] = {lo0.x, lo0.y, lo0.z, lo0.w, hi0.x, hi0.y, hi0.z, hi0.w};
+              // rows of 32 (the padded wide tables): the row's 32 adjacency
+              // bytes in two 16-byte loads
+              const bool w32 = W == kEllMaxWidth;
+              uint4 na[2] = {make_uint4(0u, 0u, 0u, 0u), make_uint4(0u, 0u, 0u, 0u)};
+              if (w32) {
+                const uint4* nr = reinterpret_cast<const uint4*>(S.nadj + (size_t)v * kEllMaxWidth);
+                na[0] = nr[0];
+                na[1] = nr[1];
+              }
               for (int e = 0;; e += 32) {
                 int half[2][9];
                 bool live = true;  // this chunk holds entries
@@ -867,7 +876,12 @@ __global__ __launch_bounds__(kTfceThreads, FULL ? 3 : 4) void tfce_batch_kernel(
                     }
                     if (live) {
                       unsigned wlo = 0, whi = 0;
-                      chunk_adj(S.nadj, v, W, ec, wlo, whi);
+                      if (w32) {
+                        wlo = c == 0 ? na[p2].x : na[p2].z;
+                        whi = c == 0 ? na[p2].y : na[p2].w;
+                      } else {
+                        chunk_adj(S.nadj, v, W, ec, wlo, whi);
+                      }
                       if (c == 0) {
                         chunk_cands(pos, u, i, b0, wlo, whi, ra);
                         sort8_desc(ra);

@@ -90,9 +90,14 @@ GRAPHS = {
     "ico4_shuffled": lambda: shuffled_rows(*synth.icosphere_csr(4), 7),
     "ico5_masked": lambda: adjacency.induced_subgraph(*synth.icosphere_csr(5),
                                                       np.random.default_rng(3).random(10242) < 0.85),
+    # a few vertices of degree 7 and 8: rows of width 8 that are not ring-packed
+    "ico4_w8": lambda: mesh_with_extra_edges(4, 60, 2, 4),
+    # in-plane 8-connected grid (width 8, not ring-packed; voxel 0 isolated)
+    "grid8": lambda: adjacency.voxel_csr(np.ones((1, 30, 40), bool), 26),
     # rows of width 12: two groups of 8 in the neighbour-position rows
     "ico4_w12": lambda: mesh_with_extra_edges(4, 3000, 2, 1),
-    # rows of width 20: the table path, pruning per group of 8
+    # rows of width 20, padded to 32: the table path, pruning per group of 8,
+    # four groups joined at once
     "ico4_w20": lambda: mesh_with_extra_edges(4, 9000, 2, 2),
     "ico5": lambda: synth.icosphere_csr(5),
     "vox26": lambda: adjacency.voxel_csr(synth.ellipsoid_mask((21, 25, 19), (10, 12, 9), (9, 11, 8.5)), 26),
@@ -155,7 +160,7 @@ def canon(labels):
     return out
 
 
-@pytest.mark.parametrize("gname", ["ico5", "ico5_masked", "vox26", "vox6_walls"])
+@pytest.mark.parametrize("gname", ["ico5", "ico5_masked", "ico4_w8", "vox26", "vox6_walls"])
 def test_partitions_equal_at_every_level(torch, gname):
     from tfce_mediation_amd.tfce import Adjacency
     row_ptr, col = GRAPHS[gname]()
@@ -177,7 +182,8 @@ def test_partitions_equal_at_every_level(torch, gname):
         np.testing.assert_array_equal(canon(lab.cpu().numpy()), ref)
 
 
-@pytest.mark.parametrize("gname,E", [("ico5", 1.0), ("ico5_masked", 1.0), ("vox26", 0.5)])
+@pytest.mark.parametrize("gname,E", [("ico5", 1.0), ("ico5_masked", 1.0), ("ico4_w8", 1.0), ("grid8", 0.5),
+                                     ("ico4_w20", 1.0), ("vox26", 0.5)])
 def test_batch_max_matches_oracle(torch, gname, E):
     from tfce_mediation_amd.tfce import Adjacency
     row_ptr, col = GRAPHS[gname]()

@@ -294,30 +294,19 @@ __device__ __forceinline__ void chunk_cands(const int32_t* pos, const int (&u)[8
   prune8(q, adj, i, b0, r);
 }
 
-// 3b for one element i (vertex v) and up to 8 of its neighbours u[] (-1: none):
-// gather their positions, find the roots of the joined ones and of i's set
-// (ri, an ancestor) as interleaved chains, then hook every other distinct root
-// under the smallest with independent CASes; a failed CAS (the root was hooked
-// meanwhile) falls back to uf_link.  Returns an ancestor of i's set.
+// 3b for one element i (vertex v) and up to 8 of its neighbours u[] (-1: none)
+// of a table or CSR row without adjacency bytes: gather their positions, find
+// the roots of the joined ones and of i's set (ri, an ancestor) as interleaved
+// chains, then hook every other distinct root under the smallest with
+// independent CASes; a failed CAS (the root was hooked meanwhile) falls back
+// to uf_link.  Returns an ancestor of i's set.
 __device__ __forceinline__ int join_chunk(int32_t* parent, int32_t* size, const int32_t* pos, const float* x,
                                           const int (&u)[8],
                                           int i, int v, int ri, int b0, int b1, bool symmetric, float sign,
-                                          float xv, bool prune, unsigned wlo, unsigned whi) {
+                                          float xv) {
   int q[8], r[9];
 #pragma unroll
   for (int j = 0; j < 8; ++j) q[j] = u[j] >= 0 ? pos[u[j]] : kNever;
-  if (prune) {
-    // symmetric with adjacency bytes (wlo, whi: the chunk's 8 bytes): joined
-    // = smaller position, old groups pruned
-    unsigned adj[8];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      adj[j] = (wlo >> (8 * j)) & 0xffu;
-      adj[4 + j] = (whi >> (8 * j)) & 0xffu;
-    }
-    prune8(q, adj, i, b0, r);
-    return join_roots<true>(parent, size, r, ri, b0);
-  }
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
     bool take = q[j] < b1 && q[j] != i;
@@ -920,10 +909,9 @@ __global__ __launch_bounds__(kTfceThreads, FULL ? 3 : 4) void tfce_batch_kernel(
                 u[4] = hi.x; u[5] = hi.y; u[6] = hi.z; u[7] = hi.w;
               }
             } else if (ell) {
+              // no adjacency bytes (asymmetric lists): chunk by chunk
               int u[8] = {lo0.x, lo0.y, lo0.z, lo0.w, hi0.x, hi0.y, hi0.z, hi0.w};
-              unsigned wlo = 0, whi = 0;
-              chunk_adj(S.nadj, v, W, 0, wlo, whi);
-              ri = join_chunk(parent, size, pos, x, u, i, v, ri, b0, b1, symmetric, sign, xv, S.nadj != nullptr, wlo, whi);
+              ri = join_chunk(parent, size, pos, x, u, i, v, ri, b0, b1, symmetric, sign, xv);
               // rows longer than 8: the table is -1 padded, so a chunk that ends
               // in padding is the last
               const int4* row = reinterpret_cast<const int4*>(ell + (size_t)v * W);
@@ -932,9 +920,7 @@ __global__ __launch_bounds__(kTfceThreads, FULL ? 3 : 4) void tfce_batch_kernel(
                 const int4 hi = (e + 4 < W) ? row[(e >> 2) + 1] : none4;
                 u[0] = lo.x; u[1] = lo.y; u[2] = lo.z; u[3] = lo.w;
                 u[4] = hi.x; u[5] = hi.y; u[6] = hi.z; u[7] = hi.w;
-                unsigned wlo = 0, whi = 0;
-                chunk_adj(S.nadj, v, W, e, wlo, whi);
-                ri = join_chunk(parent, size, pos, x, u, i, v, ri, b0, b1, symmetric, sign, xv, S.nadj != nullptr, wlo, whi);
+                ri = join_chunk(parent, size, pos, x, u, i, v, ri, b0, b1, symmetric, sign, xv);
               }
             } else {
               const int e0 = S.row_ptr[v], e1 = S.row_ptr[v + 1];
@@ -942,7 +928,7 @@ __global__ __launch_bounds__(kTfceThreads, FULL ? 3 : 4) void tfce_batch_kernel(
                 int u[8];
 #pragma unroll
                 for (int j = 0; j < 8; ++j) u[j] = (e + j < e1) ? S.col[e + j] : -1;
-                ri = join_chunk(parent, size, pos, x, u, i, v, ri, b0, b1, symmetric, sign, xv, false, 0u, 0u);
+                ri = join_chunk(parent, size, pos, x, u, i, v, ri, b0, b1, symmetric, sign, xv);
               }
             }
           }

@@ -788,21 +788,36 @@ __global__ __launch_bounds__(kTfceThreads, FULL ? 3 : 4) void tfce_batch_kernel(
 #pragma unroll
               for (int j = 0; j < 8; ++j) r[j] = r[j] < i ? r[j] : -1;
             }
-            int ri = join_roots<false>(parent, size, r, i, b0);
-            const int4* row = reinterpret_cast<const int4*>(npos + (size_t)i * W);
-            for (int e = 8; e < W; e += 8) {
-              const int4 l2 = row[e >> 2];
-              const int4 h2 = (e + 4 < W) ? row[(e >> 2) + 1] : make_int4(kNever, kNever, kNever, kNever);
-              int r2[9] = {l2.x, l2.y, l2.z, l2.w, h2.x, h2.y, h2.z, h2.w, 0};
-              if (S.nadj) {
-                const int e8[8] = {l2.x, l2.y, l2.z, l2.w, h2.x, h2.y, h2.z, h2.w};
-                prune_row(e8, i, b0, r2);
-              } else {
-#pragma unroll
-                for (int j = 0; j < 8; ++j) r2[j] = r2[j] < i ? r2[j] : -1;
-              }
-              ri = join_roots<false>(parent, size, r2, ri, b0);
+            if (W <= 8) {
+              join_roots<false>(parent, size, r, i, b0);
+              continue;
             }
+            // rows of 12 or 16: both chunks' candidates, sorted, split
+            // bitonically into the 8 largest (all of them when they are 8 or
+            // fewer) and the rest, joined only when a lane has any
+            const int4* row = reinterpret_cast<const int4*>(npos + (size_t)i * W);
+            const int4 l2 = row[2];
+            const int4 h2 = (W > 12) ? row[3] : make_int4(kNever, kNever, kNever, kNever);
+            int r2[9] = {l2.x, l2.y, l2.z, l2.w, h2.x, h2.y, h2.z, h2.w, 0};
+            if (S.nadj) {
+              const int e8[8] = {l2.x, l2.y, l2.z, l2.w, h2.x, h2.y, h2.z, h2.w};
+              prune_row(e8, i, b0, r2);
+            } else {
+#pragma unroll
+              for (int j = 0; j < 8; ++j) r2[j] = r2[j] < i ? r2[j] : -1;
+            }
+            sort8_desc(r);
+            sort8_desc(r2);
+            int top[9], bot[9];
+            bool spill = false;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+              top[j] = max(r[j], r2[7 - j]);
+              bot[j] = min(r[j], r2[7 - j]);
+              spill |= bot[j] >= 0;
+            }
+            const int ri = join_roots<false>(parent, size, top, i, b0);
+            if (__any(spill)) join_roots<false>(parent, size, bot, ri, b0);
           }
         } else {
           int v_nxt = (b0 + tid < b1) ? order[b0 + tid] : 0;

@@ -7,7 +7,7 @@ predicts the 95% and 99% critical values from the first ``m0`` null values, and
 the loop may stop early only when that prediction agrees with the running
 empirical critical values within ``delta`` (relative) for both levels; otherwise
 it runs to ``n_max`` values and reports the plain empirical result.  The plain
-run is the fallback and the judge of the model's error.
+run is the fallback and the measure of the model's error.
 
 * features: 32 quantiles of log(null[:m0]) at fixed probabilities, centred and
   scaled by that sample's mean / std of log values;

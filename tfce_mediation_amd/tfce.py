@@ -122,8 +122,12 @@ def default_batch(items_per_perm, device=None):
     mesh (two surfaces, +z)."""
     import torch
     dev = device if device is not None else torch.cuda.current_device()
-    cus = torch.cuda.get_device_properties(dev).multi_processor_count
-    want = -(-4 * cus // max(1, int(items_per_perm)))
+    return batch_for(torch.cuda.get_device_properties(dev).multi_processor_count, items_per_perm)
+
+
+def batch_for(compute_units, items_per_perm):
+    """``default_batch`` for a device of ``compute_units`` compute units."""
+    want = -(-4 * int(compute_units) // max(1, int(items_per_perm)))
     return int(min(1024, max(256, -(-want // 64) * 64)))
 
 

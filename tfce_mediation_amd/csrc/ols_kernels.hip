@@ -383,7 +383,16 @@ __device__ __forceinline__ void epi_run(const OlsGemmArgs& a, const float* ep, i
   const int ppt = rows / a.rpp;
   const int p0 = (int)(mrow0 / rows) * ppt;
   // the design shapes of the drivers (one fixed intercept column and up to
-  // seven permuted ones, rpp = m) get unrolled copies; others the general loop
+  // seven permuted ones, rpp = m; the mediation models' intercept and
+  // outcome with one or two permuted columns) get unrolled copies; others
+  // the general loop
+  if (a.f == 2 && a.rpp == a.m) {
+    switch (a.m) {
+      case 1: epi_loop<LDC, 2, 1>(a, ep, cl, col, p0, ppt, slot, step, gF, hF, r0, mu, yinv); return;
+      case 2: epi_loop<LDC, 2, 2>(a, ep, cl, col, p0, ppt, slot, step, gF, hF, r0, mu, yinv); return;
+      default: break;
+    }
+  }
   if (a.f == 1 && a.rpp == a.m) {
     switch (a.m) {
       case 1: epi_loop<LDC, 1, 1>(a, ep, cl, col, p0, ppt, slot, step, gF, hF, r0, mu, yinv); return;

@@ -19,4 +19,24 @@ Contents
                ``pyfunc.py:107-126``).
 * ``graphs``   adjacency builders restated from ``pyfunc.py:37-76`` and
                ``tm_func.py:502-562`` plus brute-force BFS components.
+
+Pins (each checked by a test that shares no code with the function)
+-------------------------------------------------------------------
+* ``tfce.tfce``          the reference header itself built in ``oracle/_ref``
+                         (bit for bit), brute-force BFS partitions and closed
+                         forms (``tests/test_oracle_tfce.py``).
+* ``ols.tval_int`` / ``perm_t`` / ``calc_beta_se`` / ``resid_covars``
+                         QR, ``scipy.stats.linregress``, lstsq, invariances.
+* ``ols.calcF``          F = t^2 of ``linregress`` at k = 2; F from the R^2 of
+                         ``np.linalg.lstsq``; the grand-mean TSS offset.
+* ``ols.calc_sobelz``    closed forms of the three z variants, symmetric path b.
+* ``fwe.find_nearest*`` / ``fwep_map``   hand-made nulls with literal answers.
+* ``fwe.apply_mfwer``    one surface: standardised-log nearest lookup; three
+                         surfaces with log-mask-size weights: per-permutation
+                         winners worked out by hand (``tests/test_oracle_fwe.py``).
+* ``graphs.create_adjac_vertex`` / ``create_adjac_voxel``   hand-derived lists
+                         (octahedron; 3x3x3 cube, 4x4x4 cube less one voxel,
+                         6- and 26-connectivity) and a coordinate brute force
+                         (``tests/test_adjacency.py``).
+* ``mlp``                central finite differences (``tests/test_oracle_mlp.py``).
 """

@@ -8,8 +8,8 @@ TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
 * ``find_nearest``          tm_func.py:566-573 / pyfunc.py:254-261.
 * ``apply_mfwer``           tm_func.py:403-491 with the per-element Python loops,
                             nulls passed in memory.
-* ``perm_max_vertex``       write_perm_maxTFCE_vertex scaling (pyfunc.py:107-119)
-  ``perm_max_voxel``        and write_perm_maxTFCE_voxel (:121-126) given TFCE maps.
+* ``perm_max_voxel``        write_perm_maxTFCE_voxel scaling (pyfunc.py:121-126)
+                            given a TFCE map.
 """
 import math
 
@@ -118,13 +118,6 @@ def apply_mfwer(image_array, num_contrasts, surface_range, num_perm, num_surf, n
     if mediation:
         return positive_data
     return positive_data, negative_data
-
-
-def perm_max_vertex(tfce_lh, t_lh, tfce_rh, t_rh, vd_lh=1, vd_rh=1):
-    """max over hemispheres of TFCE * (max(t)/100) * vdensity (float32 numpy)."""
-    a = tfce_lh[np.isfinite(tfce_lh)] * (t_lh[np.isfinite(t_lh)].max() / np.float32(100)) * vd_lh
-    b = tfce_rh[np.isfinite(tfce_rh)] * (t_rh[np.isfinite(t_rh)].max() / np.float32(100)) * vd_rh
-    return np.array([a.max(), b.max()]).max()
 
 
 def perm_max_voxel(tfce, t):

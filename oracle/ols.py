@@ -7,8 +7,9 @@ TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
                     float32, t = a / se.
 * ``calc_beta_se``  cynumstats.pyx:66-74 (X = [1, x]).
 * ``resid_covars``  cynumstats.pyx:54-57.
-* ``calcF``         cynumstats.pyx:31-36 (returns a scalar sum when y is 2-D,
-                    exactly as the reference's builtin ``sum`` over rows).
+* ``calcF``         cynumstats.pyx:31-36 (the builtin ``sum`` over the rows of a
+                    2-D y gives one RSS and one TSS per column; the TSS is taken
+                    about the grand mean of all of y, ``np.mean(y)``).
 * ``calc_sobelz``   pyfunc.py:130-162 for mediation types I, M (Y needs
                     scipy's linregress on two vectors).
 * ``perm_t``        tval_int on X[perm] for a batch of permutations, the

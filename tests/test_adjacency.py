@@ -119,3 +119,87 @@ def test_locality_order_is_a_band_reducing_permutation():
     adj = [[1], [0, 2], [1], [4], [3]]
     o = locality_order(*to_csr(adj))
     assert sorted(o[:3].tolist()) in ([0, 1, 2], [2, 3, 4])
+
+
+# ---------------------------------------------------------------------------
+# Hand-derived lists (pyfunc.py:48-76 read as rules, not as code): labels are
+# C-order ranks of the mask voxels; with 26-connectivity a voxel lists every
+# mask voxel at Chebyshev distance 1 except label 0; with 6-connectivity only a
+# voxel whose 3x3x3 window is not clipped by the volume lists anything (its
+# face neighbours except label 0); label 0 lists nothing.
+def _brute(mask, dirtype):
+    idx = np.argwhere(mask)  # C order = label order
+    out = []
+    dims = np.array(mask.shape)
+    for a, pa in enumerate(idx):
+        lst = []
+        if a != 0 and (dirtype == 26 or (np.all(pa > 0) and np.all(pa < dims - 1))):
+            for b, pb in enumerate(idx):
+                d = np.abs(pa - pb)
+                if b == 0 or b == a:
+                    continue
+                if (dirtype == 26 and d.max() == 1) or (dirtype == 6 and d.sum() == 1):
+                    lst.append(b)
+        out.append(lst)
+    return out
+
+
+def _both(mask, dirtype):
+    ref = graphs.create_adjac_voxel(mask, mask.astype(np.float32), int(mask.sum()), dirtype)
+    rp, col = adjacency.voxel_csr(mask, dirtype)
+    mine = [col[rp[i]:rp[i + 1]].tolist() for i in range(len(rp) - 1)]
+    return ref, mine
+
+
+def test_voxel_adjacency_hand_derived_cube3():
+    mask = np.ones((3, 3, 3), bool)
+    for dirtype in (26, 6):
+        ref, mine = _both(mask, dirtype)
+        assert ref == mine == _brute(mask, dirtype)
+    ref26, _ = _both(mask, 26)
+    assert ref26[0] == []
+    assert ref26[13] == [i for i in range(1, 27) if i != 13]  # centre: everyone but itself and 0
+    assert ref26[26] == [13, 14, 16, 17, 22, 23, 25]          # far corner (2,2,2)
+    assert ref26[1] == [2, 3, 4, 5, 9, 10, 11, 12, 13, 14]    # (0,0,1): 0 is never a neighbour
+    assert all(0 not in lst for lst in ref26)
+    ref6, _ = _both(mask, 6)
+    assert ref6[13] == [4, 10, 12, 14, 16, 22]
+    assert all(lst == [] for i, lst in enumerate(ref6) if i != 13)  # every other voxel touches a wall
+
+
+def test_voxel_adjacency_hand_derived_cube4_one_removed():
+    mask = np.ones((4, 4, 4), bool)
+    mask[1, 2, 1] = False  # C-order index 25: labels past it shift down by one
+
+    def lab(x, y, z):
+        i = 16 * x + 4 * y + z
+        assert i != 25
+        return i - (i > 25)
+    for dirtype in (26, 6):
+        ref, mine = _both(mask, dirtype)
+        assert len(ref) == 63
+        assert ref == mine == _brute(mask, dirtype)
+    ref26, _ = _both(mask, 26)
+    ref6, _ = _both(mask, 6)
+    assert ref26[0] == [] and ref6[0] == []
+    assert ref26[lab(0, 0, 1)] == [2, 4, 5, 6, 16, 17, 18, 20, 21, 22]
+    assert ref6[lab(1, 1, 1)] == sorted([lab(0, 1, 1), lab(2, 1, 1), lab(1, 0, 1), lab(1, 1, 0), lab(1, 1, 2)])
+    assert ref6[lab(2, 2, 2)] == sorted([lab(1, 2, 2), lab(3, 2, 2), lab(2, 1, 2), lab(2, 3, 2), lab(2, 2, 1),
+                                         lab(2, 2, 3)])
+    interior = {lab(x, y, z) for x in (1, 2) for y in (1, 2) for z in (1, 2) if (x, y, z) != (1, 2, 1)}
+    assert len(interior) == 7
+    assert all((lst != []) == (i in interior) for i, lst in enumerate(ref6))
+    # the removed voxel's neighbours lose it; the centre-ish voxel (2,2,2) lists 25 others
+    assert len(ref26[lab(2, 2, 2)]) == 25
+    assert all(0 not in lst for lst in ref26 + ref6)
+
+
+def test_vertex_adjacency_hand_derived_octahedron():
+    """pyfunc.py:37-46 on an octahedron: every vertex is joined to the four
+    vertices that are not its antipode (0-1, 2-3, 4-5 are the antipodal pairs)."""
+    v = np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1]], float)
+    f = np.array([[0, 2, 4], [2, 1, 4], [1, 3, 4], [3, 0, 4], [2, 0, 5], [1, 2, 5], [3, 1, 5], [0, 3, 5]])
+    want = [[2, 3, 4, 5], [2, 3, 4, 5], [0, 1, 4, 5], [0, 1, 4, 5], [0, 1, 2, 3], [0, 1, 2, 3]]
+    assert [sorted(s) for s in graphs.create_adjac_vertex(v, f)] == want
+    rp, col = adjacency.faces_to_csr(f, 6)
+    assert [col[rp[i]:rp[i + 1]].tolist() for i in range(6)] == want

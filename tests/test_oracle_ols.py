@@ -88,3 +88,42 @@ def test_sobel_forms():
     tb2 = _qr_t(np.column_stack([np.ones(n), m, x]), y[:, None])[1, 0]
     assert tb == pytest.approx(tb2, rel=1e-9)
     assert z[0] == pytest.approx(1 / np.sqrt(1 / tb ** 2 + 1 / ta ** 2 + 1 / (ta ** 2 * tb ** 2)), rel=1e-6)
+
+
+def test_calcF_two_columns_is_t_squared():
+    """k = 2: the F of [1, x] against the intercept-only model is the square
+    of the slope's t (scipy's linregress, no shared code)."""
+    rng = np.random.default_rng(11)
+    n = 57
+    x = rng.standard_normal(n)
+    y = 0.3 * x + rng.standard_normal(n)
+    X = np.column_stack([np.ones(n), x])
+    F = ols.calcF(X, y[:, None], n, 2)
+    lr = stats.linregress(x, y)
+    assert F.shape == (1,)
+    assert F[0] == pytest.approx((lr.slope / lr.stderr) ** 2, rel=1e-10)
+
+
+def test_calcF_matches_r_squared_of_lstsq():
+    """F = (R^2 / (k-1)) / ((1 - R^2) / (n-k)) with R^2 from np.linalg.lstsq,
+    for every column of a 2-D y whose columns share one mean (calcF takes
+    the TSS about the grand mean of all of y, which is then each column's)."""
+    rng = np.random.default_rng(12)
+    n, k, V = 80, 4, 30
+    X = np.column_stack([np.ones(n), rng.standard_normal((n, k - 1))])
+    Y = rng.standard_normal((n, V)) + 0.2 * X[:, 1:2] * rng.standard_normal(V)
+    Y = Y - Y.mean(0) + 1.5  # every column's mean is 1.5
+    F = ols.calcF(X, Y, n, k)
+    coef, rss, _, _ = np.linalg.lstsq(X, Y, rcond=None)
+    tss = ((Y - Y.mean(0)) ** 2).sum(0)
+    r2 = 1 - rss / tss
+    np.testing.assert_allclose(F, (r2 / (k - 1)) / ((1 - r2) / (n - k)), rtol=1e-9)
+    # a column with its own mean: the grand-mean TSS adds n * (mean - grand)^2
+    Y2 = Y.copy()
+    Y2[:, 0] += 2.0
+    F2 = ols.calcF(X, Y2, n, k)
+    g = Y2.mean()
+    tss0 = ((Y2[:, 0] - g) ** 2).sum()
+    rss0 = rss[0]
+    assert F2[0] == pytest.approx(((tss0 - rss0) / (k - 1)) / (rss0 / (n - k)), rel=1e-9)
+    assert tss0 == pytest.approx(tss[0] + n * (Y2[:, 0].mean() - g) ** 2, rel=1e-12)

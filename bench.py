@@ -11,18 +11,21 @@ same way (c5: mediation, +z only).
 
 A step is one batch of P permutations through the whole path (P = 256 at
 c3; by default the engine's batch size, which gives the TFCE launch four
-items per compute unit: 512 for c1, c2 and c5):
-permuted-design contraction on MFMA (split-bf16, 3 products) with the fused
-t epilogue, then one persistent TFCE launch over every (surface, permutation,
-sign) item, then the max over surfaces.  Inputs are resident in HBM before the
-timed region.
+items per compute unit: 512 for c1, c2 and c5): the permuted-design
+contraction on MFMA (split precision, 3 products: FP16x3 by default, the
+split scheme of the precision target that meets the 1e-5 t bar; BF16x3 and
+plain FP32 are timed beside it) with the fused t epilogue, then one
+persistent TFCE launch over every (surface, permutation, sign) item, then
+the max over surfaces.  Inputs are resident in HBM before the timed region.
 
 Printed: one JSON line with the metric, the plain FP32 per-permutation GPU path
-measured the same way, the same steps replayed from a captured hipGraph, the
-autotuner's launch shapes, a roofline entry for the dominant kernel (TFCE
-sweep) from HIP events inside the timed region, and the CPU baseline
-(reference TFCE build + the reference's numpy t-value code path) timed on this
-host's cores over a bounded sample.
+measured the same way, the same steps in the other precisions and replayed
+from a captured hipGraph, the autotuner's launch shapes, the statistic's error
+against FP64, a roofline entry for the dominant kernel (TFCE sweep: touch-once
+algorithmic bytes over the HIP-event time of the sweep inside the timed
+region, beside the counter-measured traffic), and the CPU baseline (reference
+TFCE build + the reference's numpy statistic code path) timed on this host's
+cores over a bounded sample, with its agreement with the GPU null.
 """
 import argparse
 import json
@@ -63,7 +66,7 @@ def log(*a):
 # CPU baseline worker (spawned process, single-threaded BLAS)
 # ----------------------------------------------------------------------------
 def _cpu_worker(args):
-    ypath, xpath, ppath, csr_path, perm_ids, settings, offsets, contrast = args
+    ypath, xpath, ppath, csr_path, perm_ids, settings, offsets, contrast, med = args
     import numpy as np
     from oracle import ols as ools
     from oracle import tfce as otfce
@@ -73,26 +76,44 @@ def _cpu_worker(args):
     z = np.load(csr_path)
     csrs = [(z["rp%d" % i], z["col%d" % i]) for i in range(len(settings))]
     use_ref = otfce.have_reference()
-    n, k = X.shape
+    n = X.shape[0]
+    V = Y.shape[1]
+
+    def beta_se(Xd):
+        # calc_beta_se as the reference runs it: float64 products on the
+        # float32 data and the per-element se_of_slope loop (cynumstats.pyx:47-74)
+        k = Xd.shape[1]
+        invXX = np.linalg.inv(np.dot(Xd.T, Xd))
+        a = ools.cy_lin_lstsqr_mat(Xd, Y)
+        sigma2 = np.sum((Y - np.dot(Xd, a)) ** 2, axis=0) / (n - k)
+        se = np.zeros((k, V), dtype=np.float32)
+        for j in range(V):
+            se[:, j] = np.sqrt(np.diag(sigma2[j] * invXX))
+        return a, se
+
     out = []
     t0 = time.time()
     for p in perm_ids:
-        nx = X[perms[p]]
-        invXX = np.linalg.inv(np.dot(nx.T, nx))
-        # tval_int as the reference runs it: float64 products on the float32
-        # data and the per-element se_of_slope loop (cynumstats.pyx:47-64)
-        a = ools.cy_lin_lstsqr_mat(nx, Y)
-        sigma2 = np.sum((Y - np.dot(nx, a)) ** 2, axis=0) / (n - k)
-        se = np.zeros((k, Y.shape[1]), dtype=np.float32)
-        for j in range(Y.shape[1]):
-            se[:, j] = np.sqrt(np.diag(sigma2[j] * invXX))
-        t = (a / se)[contrast]
+        if med:
+            # vertex_tfce_mediation_randomise.py:82-91, type M, Aroian:
+            # x permuted, y fixed; path A y_img ~ [1, x], path B y_img ~ [1, y, x]
+            x, yv = X[:, 0], X[:, 1]
+            xp = x[perms[p]]
+            a_a, se_a = beta_se(np.column_stack([np.ones(n), xp]))
+            a_b, se_b = beta_se(np.column_stack([np.ones(n), yv, xp]))
+            ta, tb = a_a[1] / se_a[1], a_b[1] / se_b[1]
+            stat = 1 / np.sqrt((1 / (tb ** 2)) + (1 / (ta ** 2)) + (1 / (ta ** 2 * tb ** 2)))
+            signs = (1.0,)
+        else:
+            a, se = beta_se(X[perms[p]])
+            stat = (a / se)[contrast]
+            signs = (1.0, -1.0)
         vals = []
-        for sgn in (1.0, -1.0):
+        for sgn in signs:
             best = None
             for (rp, col), (H, E), off in zip(csrs, settings, offsets):
-                V = len(rp) - 1
-                img = np.ascontiguousarray(sgn * t[off:off + V], dtype=np.float32)
+                Vs = len(rp) - 1
+                img = np.ascontiguousarray(sgn * stat[off:off + Vs], dtype=np.float32)
                 if use_ref:
                     enh = otfce.ref_tfce(rp, col, img, H, E)
                 else:
@@ -104,13 +125,32 @@ def _cpu_worker(args):
     return time.time() - t0, out, use_ref
 
 
-def cpu_baseline(cfg, perms_np, n_workers, per_worker, settings):
+def host_cores():
+    """(os.cpu_count(), CPUs this process may run on, CPUs of the cgroup quota or None)."""
+    nproc = os.cpu_count() or 1
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = nproc
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()[:2]
+        if q != "max":
+            quota = int(q) / int(period)
+    except (OSError, ValueError):
+        pass
+    return nproc, aff, quota
+
+
+def cpu_baseline(cfg, perms_np, n_workers, per_worker, settings, med=False):
     import numpy as np
     tmp = tempfile.mkdtemp(prefix="tfce_cpu_")
     ypath = os.path.join(tmp, "Y.npy")
     np.save(ypath, cfg["Y"].cpu().numpy())
     xpath = os.path.join(tmp, "X.npy")
-    np.save(xpath, cfg["X"])
+    # mediation: the columns are x and the outcome y
+    np.save(xpath, np.column_stack([cfg["x"], cfg["y"]]) if med else cfg["X"])
     ppath = os.path.join(tmp, "perms.npy")
     np.save(ppath, perms_np)
     csr_path = os.path.join(tmp, "csr.npz")
@@ -119,7 +159,7 @@ def cpu_baseline(cfg, perms_np, n_workers, per_worker, settings):
         arrays["rp%d" % i], arrays["col%d" % i] = rp, col
     np.savez(csr_path, **arrays)
     jobs = [(ypath, xpath, ppath, csr_path, list(range(w * per_worker, (w + 1) * per_worker)), settings,
-             list(cfg["offsets"]), cfg["contrast"]) for w in range(n_workers)]
+             list(cfg["offsets"]), cfg.get("contrast", 0), med) for w in range(n_workers)]
     env_keep = {k: os.environ.get(k) for k in ("OPENBLAS_NUM_THREADS", "OMP_NUM_THREADS", "MKL_NUM_THREADS")}
     for k in env_keep:
         os.environ[k] = "1"
@@ -200,6 +240,37 @@ def plain_engine(name, cfg, settings, precision="fp32"):
                              precision=precision, settings=settings)
 
 
+def active_counts(x):
+    """Active elements per row of the float32 device maps x [P, V] at the last
+    level of the reference's sequence (T0 = max, T -= fl(T0/100) while T >= 0,
+    active iff x > T): int64 numpy [P]."""
+    import numpy as np
+    import torch
+    T0 = x.max(1).values.cpu().numpy().astype(np.float32)
+    dT = (T0 / np.float32(100)).astype(np.float32)
+    last = np.full_like(T0, np.inf)
+    T = T0.copy()
+    live = (T0 >= 0) & (dT > 0)
+    while live.any():
+        last[live] = T[live]
+        T = np.where(live, (T - dT).astype(np.float32), T)
+        live &= T >= 0
+    return (x > torch.as_tensor(last, device=x.device)[:, None]).sum(1).cpu().numpy()
+
+
+def fp64_sobel(cfg, perm):
+    """Aroian z of the mediation workload (type M: x permuted, y fixed) from the
+    FP64 single-design path, in map columns."""
+    import numpy as np
+    from tfce_mediation_amd.cynumstats import _direct
+    n = cfg["n"]
+    xp = np.asarray(cfg["x"], dtype=np.float64)[perm]
+    one = np.ones(n)
+    ta = _direct(np.column_stack([one, xp]), cfg["Y"], t=True)["t"][1]
+    tb = _direct(np.column_stack([one, np.asarray(cfg["y"], dtype=np.float64), xp]), cfg["Y"], t=True)["t"][1]
+    return 1 / np.sqrt((1 / (tb ** 2)) + (1 / (ta ** 2)) + (1 / (ta ** 2 * tb ** 2)))
+
+
 # ----------------------------------------------------------------------------
 def main():
     ap = argparse.ArgumentParser()
@@ -209,10 +280,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=None,
                     help="permutations per step (default: the engine's default_batch())")
-    ap.add_argument("--precision", default="bf16x3")
+    ap.add_argument("--precision", default="fp16x3",
+                    help="contraction precision of the headline (fp16x3: hi/lo split-FP16, 3 products; "
+                         "bf16x3: hi/lo split-BF16, 3 products; fp32)")
     ap.add_argument("--plain-perms", type=int, default=16)
-    ap.add_argument("--cpu-workers", type=int, default=16)
-    ap.add_argument("--cpu-perms-per-worker", type=int, default=1)
+    ap.add_argument("--cpu-workers", type=int, default=None,
+                    help="CPU baseline processes (default: the CPUs this process may use, at most 16, "
+                         "the host share of one GPU)")
+    ap.add_argument("--cpu-perms-per-worker", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
@@ -331,41 +406,48 @@ def main():
             graph_rate = None
         del run
 
-    # ---- t against FP64, and the FP16 split scheme -----------------------------
+    # ---- t against FP64; the other split scheme and batched FP32 -------------
     # t of 4 permutations of this workload (engine layout) against the FP64
     # single-design path (tval_int semantics on X[perm], cynumstats) mapped
-    # back to the engine's columns; then the same steps with the contraction
-    # in FP16x3 (hi/lo fp16 of power-of-two-scaled operands, 3 products).
+    # back to the engine's columns (c5: the Sobel z against the FP64 path
+    # model); then the same timed steps with the contraction in the other
+    # split scheme and in plain FP32 (same engine, same sweep).
     t_err = {}
-    fp16_rate = None
-    if name != "c5" and not args.no_precision_check:
+    alt_rate = {}
+    if not args.no_precision_check:
         from tfce_mediation_amd.cynumstats import _direct
-        X = np.asarray(cfg["X"], dtype=np.float64)
         perms_chk = perm_batches[args.warmup][:4]
         cols = eng.cols if eng.cols is not None else np.arange(eng.V)
-        ref = np.stack([_direct(X[p], cfg["Y"], t=True)["t"][eng.contrast][cols]
-                        for p in perms_chk.cpu().numpy()])
+        if name == "c5":
+            ref = np.stack([fp64_sobel(cfg, p)[cols] for p in perms_chk.cpu().numpy()])
+        else:
+            X = np.asarray(cfg["X"], dtype=np.float64)
+            ref = np.stack([_direct(X[p], cfg["Y"], t=True)["t"][eng.contrast][cols]
+                            for p in perms_chk.cpu().numpy()])
 
         def terr(tg):
             tg = tg.astype(np.float64)
             big = np.abs(ref) >= 1
             return {"norm": float(np.linalg.norm(tg - ref) / np.linalg.norm(ref)),
-                    "point_abs_t_ge_1": float(np.max(np.abs(tg - ref)[big] / np.abs(ref[big])))}
+                    "point_abs_ge_1": float(np.max(np.abs(tg - ref)[big] / np.abs(ref[big])))}
         t_err[args.precision] = terr(eng.tvalues(perms_chk).cpu().numpy())
-        if args.precision != "fp16x3":
-            _, eng16, _, _, _ = make_workload(name, "fp16x3")
-            t_err["fp16x3"] = terr(eng16.tvalues(perms_chk).cpu().numpy())
+        for prec in ("fp16x3", "bf16x3", "fp32"):
+            if prec == args.precision:
+                continue
+            _, eng2, _, _, _ = make_workload(name, prec)
+            if prec != "fp32":
+                t_err[prec] = terr(eng2.tvalues(perms_chk).cpu().numpy())
             for s in range(args.warmup):
-                eng16.null_batch(perm_batches[s])
+                eng2.null_batch(perm_batches[s])
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             for perms in timed:
-                eng16.null_batch(perms)
+                eng2.null_batch(perms)
             torch.cuda.synchronize()
-            fp16_rate = args.steps * P / (time.perf_counter() - t0)
-            del eng16
-        log(f"[bench] t vs FP64 (4 permutations): {t_err}; fp16x3 steps: "
-            f"{'-' if fp16_rate is None else f'{fp16_rate:.1f} perms/s'}")
+            alt_rate[prec] = args.steps * P / (time.perf_counter() - t0)
+            del eng2
+        log(f"[bench] error vs FP64 (4 permutations): {t_err}; batched steps in other precisions: "
+            + ", ".join(f"{k} {v:.1f} perms/s" for k, v in alt_rate.items()))
 
     # ---- plain FP32 per-permutation path ------------------------------------
     # the reference's call structure: one t map per permutation, then one
@@ -403,37 +485,65 @@ def main():
         log(f"[bench] plain fp32 per-permutation path: {plain:.2f} perms/s")
         del eng32
 
-    # ---- CPU baseline (GLM workloads) -----------------------------------------
-    cpu = None
-    nw = max(1, min(args.cpu_workers, os.cpu_count() or 1))
+    # ---- CPU baseline ---------------------------------------------------------
+    nproc, n_aff, quota = host_cores()
+    share = min(n_aff, 16) if quota is None else max(1, min(n_aff, int(quota), 16))
+    nw = args.cpu_workers if args.cpu_workers is not None else share
     if name == "c4":
         nw = min(nw, 4)  # each process holds FP64 temporaries of the 3.5 M-element data (about 10 GB)
-    if not args.no_cpu and name != "c5":
+    cpu = None
+    if not args.no_cpu:
         perms_np = perm_batches[args.warmup].cpu().numpy()
-        log(f"[bench] CPU baseline: {nw} single-threaded workers x {args.cpu_perms_per_worker} permutation(s)")
-        cpu = cpu_baseline(cfg, perms_np, nw, args.cpu_perms_per_worker, settings)
+        log(f"[bench] CPU baseline: {nw} single-threaded workers x {args.cpu_perms_per_worker} permutation(s) "
+            f"(os.cpu_count() {nproc}, affinity {n_aff}, cgroup quota {quota})")
+        cpu = cpu_baseline(cfg, perms_np, nw, args.cpu_perms_per_worker, settings, med=(name == "c5"))
         log(f"[bench] CPU: aggregate {cpu['aggregate_rate']:.3f} perms/s, per core {cpu['per_core_rate']:.3f}")
         # the same permutations through the GPU path: agreement of the null values
         gpu_vals = eng.null_batch(perm_batches[args.warmup][: cpu["perms"]]).cpu().numpy()
         cpu_vals = np.array(cpu["values"])
         rel = np.abs(gpu_vals - cpu_vals) / np.maximum(np.abs(cpu_vals), 1e-12)
         cpu["max_rel_diff_null"] = float(rel.max())
-        log(f"[bench] GPU vs CPU null values: max rel diff {rel.max():.2e}")
+        cpu["median_rel_diff_null"] = float(np.median(rel))
+        log(f"[bench] GPU vs CPU null values: max rel diff {rel.max():.2e}, median {np.median(rel):.2e}")
 
     # ---- roofline of the dominant kernel (TFCE sweep) -----------------------
-    bytes_per_sign = 0
-    for rp, col in cfg["surfaces"]:
+    # Touch-once algorithmic bytes of the incremental sweep, per image (one
+    # surface, one permutation, one sign) with V elements, A of them active at
+    # the last level and d = nnz / V neighbours on average:
+    #   read the map 4V; write and read each element's activation position 8V;
+    #   read each active element's neighbour list 4dA and gather its
+    #   neighbours' positions 4dA; write and read one union-find node
+    #   {parent, size, value} per active element 24A.
+    #   B = 12 V + (8 d + 24) A
+    # A is counted on the maps of the first timed step with the reference's
+    # float32 level sequence (the last threshold >= 0).
+    maps = eng.tvalues(perm_batches[args.warmup])
+    signs = (1.0, -1.0) if n_signs == 2 else (1.0,)
+    touch_bytes = 0.0
+    level_model_bytes = 0.0
+    images = 0
+    for (rp, col), off in zip(eng.csrs, cfg["offsets"]):
         V_s, nnz_s = len(rp) - 1, len(col)
-        bytes_per_sign += N_THR * (4 * (V_s + 1) + 4 * nnz_s + 8 * V_s) + 8 * V_s
-    units = P * n_signs
-    achieved = bytes_per_sign * units / (tfce_ms * 1e-3) / 1e9
-    traffic, traffic_src = _measured_traffic("tfce_batch_kernel<false>", name, units * len(cfg["surfaces"]))
+        d = nnz_s / V_s
+        for sgn in signs:
+            x = sgn * maps[:, int(off):int(off) + V_s]
+            A = active_counts(x)
+            touch_bytes += float((12.0 * V_s + (8.0 * d + 24.0) * A).sum())
+            level_model_bytes += float(P * (N_THR * (4 * (V_s + 1) + 4 * nnz_s + 8 * V_s) + 8 * V_s))
+            images += P
+    del maps
+    achieved = touch_bytes / (tfce_ms * 1e-3) / 1e9
+    traffic, traffic_src = _measured_traffic("tfce_batch_kernel<false>", name, images)
     gemm_flops = flops_per_perm * P
     workload, data = WORKLOADS[name]
     if name == "c3":
         metric = "permutations/s at 327,684 vertices, n=500: batched split-BF16 vs plain FP32"
     else:
-        metric = f"permutations/s ({name}): batched split-BF16 vs plain FP32"
+        metric = f"permutations/s ({name}): batched split-precision vs plain FP32"
+    prec_text = {"fp16x3": "fp16x3 (hi/lo split-FP16 of power-of-two scaled operands, 3 MFMA products, fp32 "
+                           "accumulate; fp64 SSE/t epilogue; fp32 TFCE sums, int32 union-find)",
+                 "bf16x3": "bf16x3 (hi/lo split-BF16, 3 MFMA products, fp32 accumulate; fp64 SSE/t epilogue; "
+                           "fp32 TFCE sums, int32 union-find)"}
     out = {
         "metric": metric,
         "value": round(value, 2),
@@ -445,18 +555,22 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16x3 (fp32 accumulate; fp64 SSE/t epilogue; fp32 TFCE sums, int32 union-find)"
-        if args.precision == "bf16x3" else args.precision,
+        "dtype": prec_text.get(args.precision, args.precision),
         "data": data,
         "config": {
             "workload": workload,
             "permutations_per_step": P,
             "precision": args.precision,
             "plain_fp32_perms_per_s": None if plain is None else round(plain, 3),
+            "plain_path": "per permutation, the reference's call structure: one FP32 t map, then one "
+                          "single-image TFCE launch (one workgroup) per surface and sign",
             "batched_over_plain": None if plain is None else round(value / plain, 2),
+            "batched_fp32_perms_per_s": None if "fp32" not in alt_rate else round(alt_rate["fp32"], 1),
+            "batched_over_batched_fp32": None if "fp32" not in alt_rate else round(value / alt_rate["fp32"], 3),
+            "bf16x3_perms_per_s": None if "bf16x3" not in alt_rate else round(alt_rate["bf16x3"], 1),
+            "fp16x3_perms_per_s": None if "fp16x3" not in alt_rate else round(alt_rate["fp16x3"], 1),
             "hipgraph_replay_perms_per_s": None if graph_rate is None else round(graph_rate, 1),
-            "t_rel_err_vs_fp64": t_err or None,
-            "fp16x3_perms_per_s": None if fp16_rate is None else round(fp16_rate, 1),
+            "stat_rel_err_vs_fp64": t_err or None,
             "autotuned": None if tuned is None else {"gemm_tile": ["256x128", "512x64", "512x128"][tuned["gemm_tile"]],
                                                      "tfce_wgs_per_cu": tuned["tfce_wgs_per_cu"]},
             "schedule": mode + (" (contraction of step s+1 on a second stream during the sweep of step s)"
@@ -464,7 +578,7 @@ def main():
             "contraction_ms_per_step": round(gemm_ms, 3),
             "tfce_ms_per_step": round(tfce_ms, 3),
             "contraction_tflops": round(gemm_flops / (gemm_ms * 1e-3) / 1e12, 2),
-            "contraction_frac_of_bf16_peak": round(gemm_flops / (gemm_ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS, 4),
+            "contraction_frac_of_dense_peak": round(gemm_flops / (gemm_ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS, 4),
         },
         "roofline": {
             "kernel": "tfce_batch_kernel<false> (persistent TFCE sweep)",
@@ -475,24 +589,32 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
             "traffic_source": traffic_src,
+            "traffic_over_algorithmic": None if traffic is None else round(traffic / touch_bytes, 3),
             "traffic_gbs": None if traffic is None else round(traffic / (tfce_ms * 1e-3) / 1e9, 1),
             "traffic_frac_of_peak": None if traffic is None else round(traffic / (tfce_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-            "algorithmic_bytes_per_unit": bytes_per_sign,
-            "units_per_launch": units,
-            "note": "level-sweep byte model (100 thresholds x (CSR + map + labels) per sign); the kernel "
-                    "visits each vertex once per level only while it is active, so it can exceed 1",
+            "algorithmic_bytes_per_launch": int(touch_bytes),
+            "images_per_launch": images,
+            "byte_model": "touch-once: B = 12 V + (8 d + 24) A per image (V elements, A active at the last "
+                          "level, d neighbours per element)",
+            "level_sweep_model_frac": round(level_model_bytes / (tfce_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 3),
         },
         "cpu_baseline": None if cpu is None else {
             "value": round(cpu["aggregate_rate"], 4),
             "unit": "permutations/s",
             "cores": nw,
+            "host_nproc": nproc,
+            "host_affinity_cpus": n_aff,
+            "host_cgroup_cpus": quota,
             "kind": "reference" if cpu["used_reference"] else "port",
             "per_core": round(cpu["per_core_rate"], 4),
-            "sample": f"{cpu['perms']} permutations of the same workload, one per single-threaded "
-                      f"process: the reference's numpy tval_int code path (float64, per-element "
-                      f"se_of_slope loop) + its own fast_tfce.hpp (oracle/_ref) for +/-t on every "
-                      f"surface + write_perm_maxTFCE scaling; shell echo excluded",
+            "sample": f"{cpu['perms']} permutations of the same workload, {args.cpu_perms_per_worker} per "
+                      f"single-threaded process: " + ("the reference's numpy calc_beta_se code path for both "
+                      "mediation paths (float64, per-element se_of_slope loop) and the Aroian z"
+                      if name == "c5" else "the reference's numpy tval_int code path (float64, per-element "
+                      "se_of_slope loop)") + " + its own fast_tfce.hpp (oracle/_ref) for every sign and "
+                      "surface + write_perm_maxTFCE scaling; shell echo excluded",
             "max_rel_diff_vs_gpu_null": cpu.get("max_rel_diff_null"),
+            "median_rel_diff_vs_gpu_null": cpu.get("median_rel_diff_null"),
         },
     }
     print(json.dumps(out), flush=True)

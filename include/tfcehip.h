@@ -10,7 +10,14 @@
  *   - each call enqueues on the given stream and returns without a host
  *     synchronisation, unless its comment says otherwise;
  *   - handles are immutable after creation, so one handle may serve calls on
- *     several streams at once (scratch memory is kept per (device, stream)).
+ *     several streams at once (scratch memory is kept per (device, stream));
+ *   - graph capture: a call made while its stream captures a graph allocates
+ *     nothing and fails with TFCE_EINVAL unless the same call ran eagerly on
+ *     that stream before (the stream's scratch is large enough); the graph
+ *     then holds that scratch, which the library never frees while a later
+ *     call on the stream grows it.  Replays share the capture stream's
+ *     scratch with each other and with later calls on that stream: run them
+ *     in order, not concurrently with such calls.
  */
 #ifndef TFCEHIP_H
 #define TFCEHIP_H
@@ -166,6 +173,12 @@ int tfce_ols_perm_t(const tfce_ols_model_t *model, const int32_t *d_perm, int32_
  * not used. */
 int tfce_ols_perm_f(const tfce_ols_model_t *model, const int32_t *d_perm, int32_t P, const double *d_tss,
                     float *d_f, int64_t ldf, hipStream_t stream);
+
+/* The total sum of squares calcF takes (cynumstats.pyx:34, TSS = sum((y -
+ * np.mean(y))**2) summed over rows): d_tss[v] = sum_i (Y[i][v] - m)^2 in FP64,
+ * m the grand mean of all n x V values of Y (row stride ldy).  The input to
+ * tfce_ols_perm_f.  Blocking. */
+int tfce_total_ss(const float *d_Y, int32_t n, int64_t V, int64_t ldy, double *d_tss, hipStream_t stream);
 
 /* Workgroup tile of the split-precision contraction (process-wide; the
  * autotuner's knob): 0 = 256 rows x 128 elements (default), 1 = 512 x 64,

@@ -100,3 +100,33 @@ def test_default_batch_fills_the_compute_units():
         assert b % 64 == 0 and 256 <= b <= 1024
         assert items * b >= min(4 * cus, items * 1024)
         assert b == 256 or items * (b - 64) < 4 * cus
+
+
+def test_graph_replay_after_scratch_grows_on_capture_stream():
+    """A graph holds the scratch of the stream it was captured on.  A larger
+    eager batch on that same stream grows the scratch afterwards; the
+    library keeps the captured buffers alive, so later replays still give
+    the eager null."""
+    import torch
+    from tfce_mediation_amd import synth
+    from tfce_mediation_amd.engine import PermutationEngine
+    print("[gpu] engine hipGraph replay after the capture stream's scratch grows", flush=True)
+    cfg = synth.config1(seed=19, n=60, order=4, device="cuda")
+    eng = PermutationEngine(cfg["Y"], cfg["X"], cfg["surfaces"], cfg["offsets"], cfg["H"], cfg["E"],
+                            contrast=cfg["contrast"])
+    P = 40
+    run = eng.graph(P)
+    big = synth.config1(seed=20, n=60, order=6, device="cuda")
+    eng_big = PermutationEngine(big["Y"], big["X"], big["surfaces"], big["offsets"], big["H"], big["E"],
+                                contrast=big["contrast"])
+    with torch.cuda.stream(run.stream):
+        grown = eng_big.null_batch(synth.permutations(big["n"], 300, 3, "cuda"))
+    run.stream.synchronize()
+    assert grown.shape == (300, 2)
+    for rep in range(3):
+        perms = synth.permutations(cfg["n"], P, 200 + rep, "cuda")
+        got = run(perms).clone()
+        torch.cuda.synchronize()
+        ref = eng.null_batch(perms)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(got.cpu().numpy(), ref.cpu().numpy())

@@ -10,12 +10,19 @@ struct Workspace {
   void* ptr = nullptr;
   size_t bytes = 0;
   int32_t* queue = nullptr;  // 256 bytes: work counters
+  bool captured = false;     // a graph captured on this stream holds ptr
 };
 
 int set_error(int code, const std::string& msg);
 int current_device();
 int cu_count(int device);
 int workspace_get(hipStream_t stream, size_t bytes, Workspace** out);
+// True while stream is capturing a graph.
+bool stream_capturing(hipStream_t stream);
+// Give up a scratch buffer of `stream`: freed at once (after the stream has
+// drained) unless a captured graph may still use it, in which case it stays
+// allocated until the process ends.
+void scratch_release(void* ptr, bool captured, hipStream_t stream);
 int tfce_grid(int64_t n_items);
 
 }  // namespace tfcehip

@@ -53,18 +53,29 @@ std::mutex g_ols_mutex;
 struct OlsWs {
   void* ptr = nullptr;
   size_t bytes = 0;
+  bool captured = false;  // a graph captured on this stream holds ptr
 };
 std::map<std::pair<int, hipStream_t>, OlsWs> g_ols_ws;
 
+// Grow-only scratch of the contraction per (device, stream); the capture rule
+// of workspace_get (capi_tfce.cpp) applies.
 int ols_ws(hipStream_t stream, size_t bytes, void** out) {
   std::lock_guard<std::mutex> g(g_ols_mutex);
   OlsWs& w = g_ols_ws[std::make_pair(current_device(), stream)];
+  if (stream_capturing(stream)) {
+    if (w.bytes < bytes)
+      return set_error(TFCE_EINVAL, "contraction scratch of this stream is too small to capture this call: "
+                                    "run it once eagerly on the capture stream first");
+    w.captured = true;
+    *out = w.ptr;
+    return 0;
+  }
   if (w.bytes < bytes) {
     if (w.ptr) {
-      (void)hipStreamSynchronize(stream);
-      (void)hipFree(w.ptr);
+      scratch_release(w.ptr, w.captured, stream);
       w.ptr = nullptr;
       w.bytes = 0;
+      w.captured = false;
     }
     hipError_t e = hipMalloc(&w.ptr, bytes);
     if (e != hipSuccess) return set_error(-(int)e, std::string("hipMalloc(ols scratch): ") + hipGetErrorString(e));
@@ -585,6 +596,19 @@ int tfce_se_of_slope(const double* d_sigma2, const double* invXX, int32_t k, int
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
   (void)hipFree(dd);
   if (e != hipSuccess) return set_error(-(int)e, "se_of_slope kernel");
+  return 0;
+}
+
+int tfce_total_ss(const float* d_Y, int32_t n, int64_t V, int64_t ldy, double* d_tss, hipStream_t stream) {
+  if (!d_Y || !d_tss) return set_error(TFCE_EINVAL, "NULL argument");
+  if (n < 1 || V < 1 || ldy < V) return set_error(TFCE_EINVAL, "need n >= 1, V >= 1 and ldy >= V");
+  double* dm = nullptr;
+  int rc = dalloc(&dm, 1, "grand mean");
+  if (rc) return rc;
+  hipError_t e = launch_total_ss(d_Y, n, V, ldy, d_tss, dm, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  (void)hipFree(dm);
+  if (e != hipSuccess) return set_error(-(int)e, "total_ss kernels");
   return 0;
 }
 

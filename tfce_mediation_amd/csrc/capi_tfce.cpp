@@ -83,22 +83,54 @@ size_t device_memory(int device) {
   return bytes;
 }
 
+bool stream_capturing(hipStream_t stream) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(stream, &cs) != hipSuccess) return false;
+  return cs == hipStreamCaptureStatusActive;
+}
+
+static std::mutex g_retained_mutex;
+static std::vector<void*> g_retained;  // superseded buffers a captured graph may use
+
+void scratch_release(void* ptr, bool captured, hipStream_t stream) {
+  if (!ptr) return;
+  if (captured) {
+    std::lock_guard<std::mutex> g(g_retained_mutex);
+    g_retained.push_back(ptr);
+    return;
+  }
+  (void)hipStreamSynchronize(stream);
+  (void)hipFree(ptr);
+}
+
 // Grow-only scratch for (device, stream).  Growing synchronises the stream so
-// no kernel of that stream still uses the old buffer.
+// no kernel of that stream still uses the old buffer.  A call made while the
+// stream captures a graph never allocates (the scratch must have been grown
+// by an eager call first) and marks the buffer as held by a graph: when a
+// later call grows it, the old buffer is kept allocated, since a replay of
+// that graph, on any stream, still writes it.
 int workspace_get(hipStream_t stream, size_t bytes, Workspace** out) {
   const int dev = current_device();
   std::lock_guard<std::mutex> g(g_ws_mutex);
   Workspace& w = g_ws[std::make_pair(dev, stream)];
+  if (stream_capturing(stream)) {
+    if (!w.queue || w.bytes < bytes)
+      return set_error(TFCE_EINVAL, "TFCE scratch of this stream is too small to capture this call: run it "
+                                    "once eagerly on the capture stream first");
+    w.captured = true;
+    *out = &w;
+    return 0;
+  }
   if (!w.queue) {
     hipError_t e = hipMalloc(&w.queue, 256);
     if (e != hipSuccess) return set_error(-(int)e, std::string("hipMalloc(queue): ") + hipGetErrorString(e));
   }
   if (w.bytes < bytes) {
     if (w.ptr) {
-      (void)hipStreamSynchronize(stream);
-      (void)hipFree(w.ptr);
+      scratch_release(w.ptr, w.captured, stream);
       w.ptr = nullptr;
       w.bytes = 0;
+      w.captured = false;
     }
     size_t want = std::max(bytes, (size_t)(w.bytes * 5 / 4));
     hipError_t e = hipMalloc(&w.ptr, want);

@@ -127,6 +127,8 @@ struct OlsDirectArgs {
 hipError_t launch_ols_direct(const OlsDirectArgs& a, hipStream_t s);
 hipError_t launch_se_of_slope(const double* sigma2, const double* diag, int k, int64_t V, float* se,
                               hipStream_t s);
+hipError_t launch_total_ss(const float* Y, int32_t n, int64_t V, int64_t ldy, double* tss, double* mean,
+                           hipStream_t s);
 hipError_t launch_sobel(const float* ta, const float* tb, int64_t n, int alg, float ta_const, float* z,
                         hipStream_t s);
 hipError_t launch_ols_prep(const OlsPrepArgs& a, hipStream_t s);

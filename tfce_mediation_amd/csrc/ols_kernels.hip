@@ -788,6 +788,52 @@ __global__ void se_of_slope_kernel(const double* sigma2, const double* diag, int
   for (int j = 0; j < k; ++j) se[(int64_t)j * V + v] = (float)sqrt(sigma2[v] * diag[j]);
 }
 }  // namespace
+// Total sum of squares of calcF (cynumstats.pyx:34): per element, about the
+// grand mean of all of Y, in FP64.  Three passes: column sums, the grand mean
+// (one workgroup, fixed reduction order), then sum_i (y_iv - mean)^2.
+__global__ __launch_bounds__(256) void colsum_kernel(const float* Y, int32_t n, int64_t V, int64_t ldy, double* out) {
+  const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (v >= V) return;
+  double s = 0.0;
+  for (int32_t i = 0; i < n; ++i) s += (double)Y[i * ldy + v];
+  out[v] = s;
+}
+
+__global__ __launch_bounds__(1024) void grand_mean_kernel(const double* sums, int64_t V, int32_t n, double* mean) {
+  __shared__ double red[1024];
+  double s = 0.0;
+  for (int64_t v = threadIdx.x; v < V; v += 1024) s += sums[v];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) mean[0] = red[0] / ((double)n * (double)V);
+}
+
+__global__ __launch_bounds__(256) void tss_kernel(const float* Y, int32_t n, int64_t V, int64_t ldy, const double* mean,
+                                                 double* tss) {
+  const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (v >= V) return;
+  const double m = mean[0];
+  double s = 0.0;
+  for (int32_t i = 0; i < n; ++i) {
+    const double d = (double)Y[i * ldy + v] - m;
+    s += d * d;
+  }
+  tss[v] = s;
+}
+
+hipError_t launch_total_ss(const float* Y, int32_t n, int64_t V, int64_t ldy, double* tss, double* mean,
+                           hipStream_t s) {
+  const unsigned blocks = (unsigned)((V + 255) / 256);
+  hipLaunchKernelGGL(colsum_kernel, dim3(blocks), dim3(256), 0, s, Y, n, V, ldy, tss);
+  hipLaunchKernelGGL(grand_mean_kernel, dim3(1), dim3(1024), 0, s, tss, V, n, mean);
+  hipLaunchKernelGGL(tss_kernel, dim3(blocks), dim3(256), 0, s, Y, n, V, ldy, mean, tss);
+  return hipGetLastError();
+}
+
 hipError_t launch_se_of_slope(const double* sigma2, const double* diag, int k, int64_t V, float* se,
                               hipStream_t s) {
   if (V <= 0) return hipSuccess;

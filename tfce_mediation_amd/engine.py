@@ -20,7 +20,7 @@ from .tfce import Adjacency, SurfaceSet, TFCE_ADJ_IMPLICIT_ZERO, TFCE_ADJ_ZERO_I
 
 
 class PermutationEngine:
-    def __init__(self, Y, X, surfaces, offsets, H=2.0, E=1.0, contrast=None, precision="bf16x3",
+    def __init__(self, Y, X, surfaces, offsets, H=2.0, E=1.0, contrast=None, precision="fp16x3",
                  perm_columns=None, vdensity=None, implicit_zero=None, settings=None, locality=True):
         """surfaces: list of CSR (row_ptr, col); offsets: first element of each
         surface in a map row; settings: optional per-surface (H, E).
@@ -137,7 +137,10 @@ class PermutationEngine:
         overwritten by the next replay).  The library's scratch is grown on
         the capture stream by two eager warm-up batches first, so nothing is
         allocated while capturing; every launch of the batch is a kernel (no
-        memset nodes)."""
+        memset nodes).  The graph holds the capture stream's scratch
+        (``run.stream``): later calls that grow it leave the captured buffers
+        allocated, and replays must not overlap other work on that stream's
+        scratch (replay on the current stream, in order)."""
         import torch
         s = torch.cuda.Stream()
         static_perm = torch.arange(self.n, dtype=torch.int32, device="cuda").repeat(P, 1).contiguous()
@@ -156,6 +159,7 @@ class PermutationEngine:
             g.replay()
             return static_out
         run.graph = g
+        run.stream = s
         return run
 
     def default_batch(self, device=None):
@@ -198,7 +202,7 @@ class MediationEngine:
     PermutationEngine."""
 
     def __init__(self, M, x, y, surfaces, offsets, H=2.0, E=1.0, medtype="M", alg="aroian",
-                 precision="bf16x3", settings=None, locality=True):
+                 precision="fp16x3", settings=None, locality=True):
         import torch
         from .randomise import MediationModel
         self.n = len(x)

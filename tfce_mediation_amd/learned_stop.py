@@ -179,7 +179,7 @@ class LearnedStop:
         return self.n_max, self.plain(values), "plain"
 
 
-def generate_nulls(make_config, seeds, n_values, batch=256, precision="bf16x3"):
+def generate_nulls(make_config, seeds, n_values, batch=256, precision="fp16x3"):
     """Long max-TFCE nulls (pooled +/- values in permutation order) from the
     engine, one per seed; make_config(seed) returns a synth config dict."""
     from . import synth
@@ -209,7 +209,7 @@ class WorkloadNull:
     per permutation (vertex_tfce_mediation_randomise.py)."""
 
     def __init__(self, kind, Y, X=None, surfaces=None, offsets=None, settings=None, x=None, y=None,
-                 precision="bf16x3"):
+                 precision="fp16x3"):
         from .engine import PermutationEngine
         self.kind = kind
         if kind == "glm":
@@ -244,7 +244,7 @@ class WorkloadNull:
         return np.concatenate(out).astype(np.float64)
 
 
-def random_workload(rng, kind=None, precision="bf16x3", log=None):
+def random_workload(rng, kind=None, precision="fp16x3", log=None):
     """A small random workload: vertex GLM (icosphere order 4-6, one or two
     hemispheres), voxel GLM (ellipsoid mask, 26-connectivity, E = 0.5) or
     vertex mediation; n, smoothness and covariates drawn from rng."""
@@ -285,7 +285,7 @@ def random_workload(rng, kind=None, precision="bf16x3", log=None):
     return WorkloadNull("glm", Y, X=X, surfaces=surfaces, offsets=offsets, settings=settings, precision=precision)
 
 
-def c3_workload(seed, precision="bf16x3"):
+def c3_workload(seed, precision="fp16x3"):
     """Two order-7 icospheres (327,684 vertices), n = 500, k = 6."""
     from . import synth
     cfg = synth.config3(seed=seed, device="cuda")
@@ -293,7 +293,7 @@ def c3_workload(seed, precision="bf16x3"):
                         settings=[(cfg["H"], cfg["E"])] * len(cfg["surfaces"]), precision=precision)
 
 
-def c5_workload(seed, precision="bf16x3"):
+def c5_workload(seed, precision="fp16x3"):
     """Mediation on two order-7 icospheres, n = 300, three covariates
     regressed out of the mediator images first."""
     import torch

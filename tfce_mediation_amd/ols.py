@@ -18,7 +18,7 @@ from . import _lib
 
 
 class PermutedOLS:
-    def __init__(self, Y, X, contrasts=None, perm_columns=None, precision="bf16x3", center=None,
+    def __init__(self, Y, X, contrasts=None, perm_columns=None, precision="fp16x3", center=None,
                  stream=None):
         import torch
         lib = _lib.lib()
@@ -105,8 +105,16 @@ class PermutedOLS:
 
 def total_ss(Y):
     """sum_i (y_iv - mean(Y))^2 per element [V] (float64, device): the TSS of
-    calcF (cynumstats.pyx:34), about the grand mean of every element of Y."""
+    calcF (cynumstats.pyx:34), about the grand mean of every element of Y,
+    computed in FP64 by ``tfce_total_ss``."""
     import torch
     Yd = Y if isinstance(Y, torch.Tensor) else torch.as_tensor(np.ascontiguousarray(Y, dtype=np.float32))
-    Yd = Yd.to(device="cuda", dtype=torch.float64)
-    return ((Yd - Yd.mean()) ** 2).sum(0).contiguous()
+    Yd = Yd.to(device="cuda", dtype=torch.float32)
+    if Yd.dim() == 1:
+        Yd = Yd[:, None]
+    if Yd.stride(1) != 1:
+        Yd = Yd.contiguous()
+    n, V = int(Yd.shape[0]), int(Yd.shape[1])
+    out = torch.empty(V, dtype=torch.float64, device=Yd.device)
+    _lib.check(_lib.lib().tfce_total_ss(_lib.ptr(Yd), n, V, Yd.stride(0), _lib.ptr(out), _lib.stream_ptr()))
+    return out

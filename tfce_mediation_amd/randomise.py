@@ -96,7 +96,7 @@ def _write(path, values, fmt):
 
 def vertex_regress(ny, pred_x, adjac_lh, adjac_rh, bin_mask_lh, bin_mask_rh, optstfce, perms,
                    vdensity_lh=1, vdensity_rh=1, specifyvars=None, out_dir=None, batch=None,
-                   precision="bf16x3"):
+                   precision="fp16x3"):
     """Null rows [n_perm, 2] (+t, -t) per contrast, dict {tnum: array}; written
     to perm_tstat_con<tnum>_TFCE_maxVertex.csv when out_dir is given."""
     import torch
@@ -134,7 +134,7 @@ def vertex_regress(ny, pred_x, adjac_lh, adjac_rh, bin_mask_lh, bin_mask_rh, opt
 
 
 def voxel_regress(ny, pred_x, adjac, optstfce, perms, specifyvars=None, out_dir=None, batch=None,
-                  precision="bf16x3", ancova=False):
+                  precision="fp16x3", ancova=False):
     """Voxel randomise: null rows [n_perm, 2] per contrast.  With ``ancova``
     (the -f branch, voxel_tfce_multiple_regression_randomise.py:79-89) the
     image is sqrt(max(F, 0)) of the whole model of X[perm] and the null is
@@ -181,7 +181,7 @@ class MediationModel:
     scalar x -> y regression.  Types M and I permute x only, type Y permutes
     x and y together (vertex_tfce_mediation_randomise.py:82-90)."""
 
-    def __init__(self, M, pred_x, depend_y, medtype="M", alg="aroian", precision="bf16x3"):
+    def __init__(self, M, pred_x, depend_y, medtype="M", alg="aroian", precision="fp16x3"):
         import torch
         x = np.asarray(pred_x, dtype=np.float64)
         y = np.asarray(depend_y, dtype=np.float64)
@@ -223,7 +223,7 @@ class MediationModel:
 
 
 def vertex_mediation(y, pred_x, depend_y, medtype, adjac_lh, adjac_rh, bin_mask_lh, bin_mask_rh, optstfce, perms,
-                     vdensity_lh=1, vdensity_rh=1, out_dir=None, batch=None, precision="bf16x3"):
+                     vdensity_lh=1, vdensity_rh=1, out_dir=None, batch=None, precision="fp16x3"):
     """Mediation randomise: null [n_perm] of the positive Sobel-z TFCE maxima."""
     import torch
     H, E = float(optstfce[0]), float(optstfce[1])
@@ -248,7 +248,7 @@ def vertex_mediation(y, pred_x, depend_y, medtype, adjac_lh, adjac_rh, bin_mask_
 
 
 def mmr(merge_y, pred_x, surfaces, offsets, settings, perms, groups=None, merged=True, zero_in_image=None,
-        vdensity=None, out_dir=None, batch=None, precision="bf16x3"):
+        vdensity=None, out_dir=None, batch=None, precision="fp16x3"):
     """Multi-surface randomise.  surfaces: CSR per surface (in-mask elements),
     offsets: their first column in merge_y; settings: (H, E) per surface;
     groups: settings-group id per surface (default: one per distinct setting).

@@ -101,6 +101,21 @@ int tfce_run_ex(tfce_adj_t *adj, const float *d_image, float *d_enhn, float min_
 int tfce_batch_max(tfce_adj_t *adj, const float *d_maps, int P, int sign_mask,
                    const float *d_vdensity, float *d_max, hipStream_t stream);
 
+/* tfce_batch_max that allocates nothing.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The scratch
+ * is one device array of TFCE_NOALLOC_SCRATCH_BYTES per GPU, held from module
+ * load until the process ends; its address is looked up on the first call on
+ * each device.  A call fails with TFCE_EINVAL, before any launch, when the
+ * per-item records of 2*P items plus one union-find slot of the adjacency do
+ * not fit (about 8.2 KB per item, and 4*(8 + w)*V bytes per slot with w = 0, 4,
+ * 8 or 16 words of neighbour rows); within the limit fewer workgroups run
+ * when fewer slots fit.  Every call on one GPU shares the array: calls that
+ * overlap on one GPU race.  Same outputs as tfce_batch_max, bit for bit. */
+#define TFCE_NOALLOC_SCRATCH_BYTES (256u << 20)
+int tfce_batch_max_noalloc(tfce_adj_t *adj, const float *d_maps, int P, int sign_mask,
+                           const float *d_vdensity, float *d_max, hipStream_t stream);
+
 /* ---------------------------------------------------------- surface sets */
 /* Several adjacencies laid out along one map row (lh|rh hemispheres, or the
  * merged image of tm_func.py:54-123 / create_position_array :556-562). */

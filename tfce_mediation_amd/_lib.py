@@ -30,6 +30,7 @@ SIGNATURES = {
     "tfce_run": (c_int, [c_p, c_p, c_p, c_f32, c_int, c_p]),
     "tfce_run_ex": (c_int, [c_p, c_p, c_p, c_f32, c_f32, c_int, c_i32, c_p, c_p, c_p]),
     "tfce_batch_max": (c_int, [c_p, c_p, c_int, c_int, c_p, c_p, c_p]),
+    "tfce_batch_max_noalloc": (c_int, [c_p, c_p, c_int, c_int, c_p, c_p, c_p]),
     "tfce_surfset_create": (c_int, [c_p, c_i32, c_p, c_p]),
     "tfce_surfset_destroy": (c_int, [c_p]),
     "tfce_surfset_order": (c_int, [c_p, c_p]),

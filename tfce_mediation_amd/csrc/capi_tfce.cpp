@@ -456,25 +456,46 @@ static int run_batch(const SurfDev* d_surfs, int32_t n_surf, int32_t v_max, int3
                      const float* d_vdensity, bool full, float* d_enhn, float min_t, float dh,
                      int32_t labels_level, int32_t* d_labels, float* d_scaled, float* d_raw,
                      float* d_own, int32_t* d_status, hipStream_t stream, int out_mul = 1,
-                     int out_add = 0, uint32_t batch_flags = 0) {
+                     int out_add = 0, uint32_t batch_flags = 0, const ExtScratch* ext = nullptr) {
   if (P <= 0) return 0;
   if (sign_mask < 1 || sign_mask > 3) return set_error(TFCE_EINVAL, "sign_mask must be 1, 2 or 3");
   const int n_signs = (sign_mask == 3) ? 2 : 1;
   const int64_t n_items = (int64_t)n_surf * P * n_signs;
-  Workspace* ws = nullptr;
   const int64_t slot_words = tfce_slot_words(v_max, np_w);
+  const size_t slot_bytes = (size_t)slot_words * sizeof(int32_t);
   // resident workgroups, each with its own slot; the slots stay within a
   // quarter of the device's memory (fewer workgroups for very large surfaces)
-  const int64_t mem_cap = (int64_t)(device_memory(current_device()) / 4 / ((size_t)slot_words * sizeof(int32_t)));
-  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(tfce_grid(n_items), mem_cap));
-  const size_t slots_bytes = (size_t)grid * slot_words * sizeof(int32_t);
+  const int64_t mem_cap = (int64_t)(device_memory(current_device()) / 4 / slot_bytes);
+  int grid = (int)std::max<int64_t>(1, std::min<int64_t>(tfce_grid(n_items), mem_cap));
   const size_t prep_bytes = (size_t)n_items * kPrepWords * sizeof(int32_t);
   // raw_max is mandatory for the kernel: park it in scratch when not wanted
   const size_t raw_bytes = (size_t)n_items * sizeof(float);
   const size_t t0_bytes = (batch_flags & TFCE_BATCH_SHARED_THRESHOLDS) ? (size_t)n_items * sizeof(float) : 0;
-  int rc = workspace_get(stream, slots_bytes + prep_bytes + raw_bytes + t0_bytes + 512, &ws);
-  if (rc) return rc;
-  char* wbase = reinterpret_cast<char*>(ws->ptr);
+  char* slots = nullptr;  // grid slots
+  char* rest = nullptr;   // prep records, raw maxima, shared thresholds
+  int32_t* queue = nullptr;
+  if (ext) {
+    // caller-given scratch: [queue 256 B][prep][raw, 256-aligned][T0][slots, 256-aligned]
+    const size_t fixed =
+        256 + ((prep_bytes + 255) / 256) * 256 + ((raw_bytes + 255) / 256) * 256 + ((t0_bytes + 255) / 256) * 256;
+    if (fixed + slot_bytes > ext->bytes)
+      return set_error(TFCE_EINVAL, "the batch needs " + std::to_string(fixed + slot_bytes) +
+                                        " bytes of scratch (records of every item and one slot of the largest "
+                                        "surface); the no-allocation scratch holds " + std::to_string(ext->bytes));
+    grid = (int)std::min<int64_t>(grid, (int64_t)((ext->bytes - fixed) / slot_bytes));
+    char* base = reinterpret_cast<char*>(ext->base);
+    queue = reinterpret_cast<int32_t*>(base);
+    rest = base + 256;
+    slots = base + fixed;
+  } else {
+    Workspace* ws = nullptr;
+    const size_t slots_bytes = (size_t)grid * slot_bytes;
+    int rc = workspace_get(stream, slots_bytes + prep_bytes + raw_bytes + t0_bytes + 512, &ws);
+    if (rc) return rc;
+    slots = reinterpret_cast<char*>(ws->ptr);
+    rest = slots + slots_bytes;
+    queue = ws->queue;
+  }
   TfceBatchArgs a{};
   a.surfs = d_surfs;
   a.n_surf = n_surf;
@@ -496,16 +517,18 @@ static int run_batch(const SurfDev* d_surfs, int32_t n_surf, int32_t v_max, int3
   a.enhn = d_enhn;
   a.labels_level = labels_level;
   a.labels = d_labels;
-  a.scratch = reinterpret_cast<int32_t*>(ws->ptr);
+  a.scratch = reinterpret_cast<int32_t*>(slots);
   a.slot_stride = slot_words;
   a.v_max = v_max;
   a.np_w = np_w;
-  a.queue = ws->queue;
+  a.queue = queue;
   a.n_items = n_items;
-  a.prep = reinterpret_cast<int32_t*>(wbase + slots_bytes);
-  a.raw_max = d_raw ? d_raw : reinterpret_cast<float*>(wbase + slots_bytes + prep_bytes);
+  a.prep = reinterpret_cast<int32_t*>(rest);
+  a.raw_max = d_raw ? d_raw : reinterpret_cast<float*>(rest + prep_bytes);
   if (batch_flags & TFCE_BATCH_SHARED_THRESHOLDS) {
-    float* T0 = reinterpret_cast<float*>(wbase + slots_bytes + prep_bytes + ((raw_bytes + 255) / 256) * 256);
+    float* T0 = reinterpret_cast<float*>(rest + prep_bytes + ((raw_bytes + 255) / 256) * 256);
+    // (inside the prep + raw + T0 extent of either layout: the stream's
+    // scratch adds 512 bytes, the caller's layout rounds each part to 256)
     hipError_t e0 = launch_shared_T0(a, T0, (batch_flags & TFCE_BATCH_ZERO_IN_IMAGE) ? 1 : 0, stream);
     if (e0 != hipSuccess) return set_error(-(int)e0, "shared threshold launch");
     a.ext_T0 = T0;
@@ -514,6 +537,29 @@ static int run_batch(const SurfDev* d_surfs, int32_t n_surf, int32_t v_max, int3
   if (e != hipSuccess) return set_error(-(int)e, std::string("tfce launch: ") + hipGetErrorString(e));
   return 0;
 }
+
+}  // extern "C"
+
+namespace tfcehip {
+
+int batch_max_in(tfce_adj_t* adj, const float* d_maps, int P, int sign_mask, const float* d_vdensity,
+                 float* d_max, const ExtScratch* ext, hipStream_t stream) {
+  if (!adj || !d_maps || !d_max) return set_error(TFCE_EINVAL, "NULL argument");
+  if (P <= 0) return 0;
+  if (sign_mask < 1 || sign_mask > 3) return set_error(TFCE_EINVAL, "sign_mask must be 1, 2 or 3");
+  if (sign_mask != 3) {
+    // the unrequested sign reads 0; the requested one lands at 2p + (sign_mask == 2)
+    hipError_t e = launch_fill_f32(d_max, (int64_t)P * 2, 0.0f, stream);
+    if (e != hipSuccess) return set_error(-(int)e, "fill max");
+  }
+  return run_batch(adj->d_surf, 1, adj->V, np_width(adj), d_maps, adj->V, P, sign_mask, nullptr, d_vdensity,
+                   d_vdensity != nullptr, nullptr, 0.0f, 0.0f, -1, nullptr, d_max, nullptr, nullptr, nullptr,
+                   stream, sign_mask == 3 ? 1 : 2, sign_mask == 2 ? 1 : 0, 0u, ext);
+}
+
+}  // namespace tfcehip
+
+extern "C" {
 
 int tfce_run_ex(tfce_adj_t* adj, const float* d_image, float* d_enhn, float min_t, float dh, int accumulate,
                 int32_t labels_level, int32_t* d_labels, int32_t* d_status, hipStream_t stream) {
@@ -534,17 +580,7 @@ int tfce_run(tfce_adj_t* adj, const float* d_image, float* d_enhn, float dh, int
 
 int tfce_batch_max(tfce_adj_t* adj, const float* d_maps, int P, int sign_mask, const float* d_vdensity,
                    float* d_max, hipStream_t stream) {
-  if (!adj || !d_maps || !d_max) return set_error(TFCE_EINVAL, "NULL argument");
-  if (P <= 0) return 0;
-  if (sign_mask < 1 || sign_mask > 3) return set_error(TFCE_EINVAL, "sign_mask must be 1, 2 or 3");
-  if (sign_mask != 3) {
-    // the unrequested sign reads 0; the requested one lands at 2p + (sign_mask == 2)
-    hipError_t e = launch_fill_f32(d_max, (int64_t)P * 2, 0.0f, stream);
-    if (e != hipSuccess) return set_error(-(int)e, "fill max");
-  }
-  return run_batch(adj->d_surf, 1, adj->V, np_width(adj), d_maps, adj->V, P, sign_mask, nullptr, d_vdensity,
-                   d_vdensity != nullptr, nullptr, 0.0f, 0.0f, -1, nullptr, d_max, nullptr, nullptr, nullptr,
-                   stream, sign_mask == 3 ? 1 : 2, sign_mask == 2 ? 1 : 0);
+  return tfcehip::batch_max_in(adj, d_maps, P, sign_mask, d_vdensity, d_max, nullptr, stream);
 }
 
 int tfce_surfset_create(const tfce_adj_t* const* adjs, int32_t n, const int64_t* offsets, tfce_surfset_t** out) {

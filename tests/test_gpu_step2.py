@@ -1,0 +1,142 @@
+"""Step 2 from step-1 hand-off directories laid out as the reference's step 1
+writes them (np.save of the same variables, ragged adjacency as object
+arrays; written by this test, so they may be unpickled): the files appended
+where the reference appends them, equal to the batched drivers called
+directly, and the voxel mediation driver against the oracle loop."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import ols as ools
+from oracle import tfce as otfce
+from tfce_mediation_amd import adjacency, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def ragged(row_ptr, col):
+    lists = [col[row_ptr[i]:row_ptr[i + 1]].tolist() for i in range(len(row_ptr) - 1)]
+    out = np.empty(len(lists), dtype=object)
+    for i, l in enumerate(lists):
+        out[i] = l
+    return out
+
+
+def save(d, **arrays):
+    os.makedirs(d, exist_ok=True)
+    for k, v in arrays.items():
+        np.save(os.path.join(d, k), v)
+
+
+def test_vertex_regression_from_step1_directory(tmp_path):
+    import torch
+    from tfce_mediation_amd import randomise, step2
+    say("[gpu] step2.vertex_regression from python_temp_<surface>")
+    rp, col = synth.icosphere_csr(3)
+    V = len(rp) - 1
+    rng = np.random.default_rng(1)
+    m_lh, m_rh = rng.random(V) < 0.9, rng.random(V) < 0.85
+    n = 30
+    merge_y = rng.standard_normal((n, int(m_lh.sum() + m_rh.sum()))).astype(np.float32)
+    pred_x = np.column_stack([rng.standard_normal(n), (rng.random(n) < 0.5).astype(float)])
+    vd = adjacency.vertex_density(rp)
+    save(os.path.join(tmp_path, "python_temp_area"), pred_x=pred_x, num_subjects=n, all_vertex=V,
+         num_vertex=merge_y.shape[1], num_vertex_lh=int(m_lh.sum()), num_vertex_rh=int(m_rh.sum()),
+         bin_mask_lh=m_lh, bin_mask_rh=m_rh, adjac_lh=ragged(rp, col), adjac_rh=ragged(rp, col),
+         merge_y=merge_y, optstfce=np.array([2.0, 1.0]), vdensity_lh=vd, vdensity_rh=vd)
+    with pytest.raises(ValueError, match="allow_pickle"):
+        step2.vertex_regression("area", 1, 8, root=str(tmp_path))
+    got = step2.vertex_regression("area", 1, 8, root=str(tmp_path), seed=3, allow_pickle=True)
+    perms = randomise.permutation_indices(n, range(1, 9), seed=3)
+    ref = randomise.vertex_regress(merge_y, pred_x, ragged(rp, col), ragged(rp, col), m_lh, m_rh, [2.0, 1.0],
+                                   perms, vdensity_lh=vd, vdensity_rh=vd)
+    for c in (1, 2):
+        np.testing.assert_array_equal(got[c], ref[c])
+        vals = np.loadtxt(os.path.join(tmp_path, "output_area", "perm_Tstat_area",
+                                       "perm_tstat_con%d_TFCE_maxVertex.csv" % c))
+        np.testing.assert_allclose(vals, got[c].ravel(), atol=5e-5)
+    torch.cuda.synchronize()
+
+
+def test_voxel_regression_and_ancova_from_step1_directory(tmp_path):
+    from tfce_mediation_amd import randomise, step2
+    say("[gpu] step2.voxel_regression from python_temp (t and F branches)")
+    mask = synth.ellipsoid_mask((12, 13, 11), (5.5, 6, 5), (5, 5.5, 4.5))
+    adj = adjacency.create_adjac_voxel(mask, mask, int(mask.sum()), 26)
+    rng = np.random.default_rng(2)
+    n = 28
+    y = rng.standard_normal((int(mask.sum()), n)).astype(np.float32)
+    pred_x = np.column_stack([rng.standard_normal(n), rng.standard_normal(n)])
+    obj = np.empty(len(adj), dtype=object)
+    for i, l in enumerate(adj):
+        obj[i] = l
+    for ancova in (0, 1):
+        root = os.path.join(tmp_path, "a%d" % ancova)
+        save(os.path.join(root, "python_temp"), num_voxel=y.shape[0], num_subjects=n, raw_nonzero_corr=y,
+             pred_x=pred_x, adjac=obj, ancova=ancova, optstfce=np.array([2.0, 0.5]))
+        got = step2.voxel_regression(2, 5, root=root, seed=9, allow_pickle=True)
+        numbers = range(2, 11) if ancova else range(2, 6)
+        perms = randomise.permutation_indices(n, numbers, seed=9)
+        ref = randomise.voxel_regress(y.T, pred_x, adj, [2.0, 0.5], perms, ancova=bool(ancova))
+        for key in ref:
+            np.testing.assert_array_equal(got[key], ref[key])
+        name = "perm_fstat_TFCE_maxVoxel.csv" if ancova else "perm_tstat_con1_TFCE_maxVoxel.csv"
+        assert os.path.exists(os.path.join(root, "output", "perm_Tstat", name))
+
+
+def test_voxel_mediation_driver_and_directory(tmp_path):
+    from tfce_mediation_amd import randomise, step2
+    say("[gpu] randomise.voxel_mediation / step2.voxel_mediation")
+    mask = synth.ellipsoid_mask((12, 12, 11), (5.5, 5.5, 5), (5, 5, 4.5))
+    rp, col = adjacency.voxel_csr(mask, 26)
+    V = int(mask.sum())
+    rng = np.random.default_rng(4)
+    n = 40
+    x = rng.standard_normal(n)
+    dy = rng.standard_normal(n)
+    M = (rng.standard_normal((n, V)) + 0.5 * x[:, None] * (rng.random(V) < 0.3)).astype(np.float32)
+    perms = randomise.permutation_indices(n, range(0, 6), seed=1)
+    for mt in ("M", "I"):
+        null = randomise.voxel_mediation(M, x, dy, mt, (rp, col), [2.0, 1.0], perms, precision="fp32")
+        for p, perm in enumerate(perms):
+            z = ools.calc_sobelz(mt, x[perm], dy, M.astype(np.float64), n, V).astype(np.float32)
+            enh, _, _ = otfce.tfce(rp, col, z, 2.0, 1.0, exact_pow=True)
+            exp = enh.max() * (z.max() / np.float32(100))
+            assert abs(null[p] - exp) <= 1e-4 * abs(exp), (mt, p, null[p], exp)
+    obj = np.empty(V, dtype=object)
+    for i in range(V):
+        obj[i] = col[rp[i]:rp[i + 1]].tolist()
+    save(os.path.join(tmp_path, "python_temp"), num_voxel=V, num_subjects=n, raw_nonzero_corr=M.T.copy(),
+         pred_x=x, depend_y=dy, adjac=obj, medtype="M", optstfce=np.array([2.0, 1.0]))
+    got = step2.voxel_mediation("M", 0, 5, root=str(tmp_path), seed=1, allow_pickle=True)
+    ref = randomise.voxel_mediation(M, x, dy, "M", (rp, col), [2.0, 1.0], perms)
+    np.testing.assert_array_equal(got, ref)
+    vals = np.loadtxt(os.path.join(tmp_path, "output_med_M", "perm_SobelZ", "perm_Zstat_M_TFCE_maxVoxel.csv"))
+    np.testing.assert_allclose(vals, got, atol=5e-5)
+
+
+def test_vertex_mediation_from_step1_directory(tmp_path):
+    from tfce_mediation_amd import randomise, step2
+    say("[gpu] step2.vertex_mediation from python_temp_med_<surface>")
+    rp, col = synth.icosphere_csr(3)
+    V = len(rp) - 1
+    rng = np.random.default_rng(6)
+    m = rng.random(V) < 0.9
+    n = 36
+    merge_y = rng.standard_normal((n, 2 * int(m.sum()))).astype(np.float32)
+    x, dy = rng.standard_normal(n), rng.standard_normal(n)
+    save(os.path.join(tmp_path, "python_temp_med_area"), pred_x=x, depend_y=dy, num_subjects=n,
+         num_vertex=merge_y.shape[1], num_vertex_lh=int(m.sum()), num_vertex_rh=int(m.sum()), all_vertex=V,
+         bin_mask_lh=m, bin_mask_rh=m, adjac_lh=ragged(rp, col), adjac_rh=ragged(rp, col),
+         optstfce=np.array([2.0, 1.0]), vdensity_lh=1, vdensity_rh=1, merge_y=merge_y)
+    got = step2.vertex_mediation("area", "I", 0, 4, root=str(tmp_path), seed=2, allow_pickle=True)
+    perms = randomise.permutation_indices(n, range(0, 5), seed=2)
+    ref = randomise.vertex_mediation(merge_y, x, dy, "I", ragged(rp, col), ragged(rp, col), m, m, [2.0, 1.0], perms)
+    np.testing.assert_array_equal(got, ref)
+    assert os.path.exists(os.path.join(tmp_path, "output_med_area", "perm_SobelZ_I",
+                                       "perm_Zstat_I_TFCE_maxVertex.csv"))

@@ -11,6 +11,9 @@ the GPU (permuted t on MFMA, one persistent TFCE launch per batch):
   (``perm_tstat_con<j>_TFCE_maxVoxel.csv`` in "%1.4f");
 * ``vertex_mediation`` vertex_tfce_mediation_randomise.py:48-92 (Sobel z,
   positive pass only, ``perm_Zstat_<type>_TFCE_maxVertex.csv``);
+* ``voxel_mediation``  voxel_tfce_mediation_randomise.py:42-76 (Sobel z of a
+  voxel mask, positive pass only, ``perm_Zstat_<type>_TFCE_maxVoxel.csv`` in
+  "%1.4f");
 * ``mmr``              tm_func.calculate_tfce in a loop (the mmr randomise of
   tm_multimodality_multisurface_regression.py:523-572): per-surface nulls
   ``perm_maxTFCE_surf<s>_tcon<c>.csv`` in "%f"; ``merged=True`` shares the
@@ -244,6 +247,24 @@ def vertex_mediation(y, pred_x, depend_y, medtype, adjac_lh, adjac_rh, bin_mask_
     null = np.concatenate(out)
     if out_dir is not None:
         _write(os.path.join(out_dir, "perm_Zstat_%s_TFCE_maxVertex.csv" % medtype), null, "%.4f")
+    return null
+
+
+def voxel_mediation(ny, pred_x, depend_y, medtype, adjac, optstfce, perms, out_dir=None, batch=None,
+                    precision="fp16x3"):
+    """Voxel mediation randomise: null [n_perm] of max(TFCE(+z)) * max(z)/100
+    (write_perm_maxTFCE_voxel, pyfunc.py:121-126) for the Sobel z of every
+    permutation (x permuted for types M and I, x and y together for type Y,
+    voxel_tfce_mediation_randomise.py:63-75); ny is [n, V]."""
+    (csr,), ny, _ = _in_layout([to_csr(adjac)], [0], ny)
+    surfs = SurfaceSet([Adjacency(csr, float(optstfce[0]), float(optstfce[1]))], [0])
+    med = MediationModel(ny, pred_x, depend_y, medtype, precision=precision)
+    out = []
+    for _, pb in _batches(perms, batch, 1):
+        out.append(surfs.batch(med.z(pb), 1)[0, :, 0].cpu().numpy())
+    null = np.concatenate(out)
+    if out_dir is not None:
+        _write(os.path.join(out_dir, "perm_Zstat_%s_TFCE_maxVoxel.csv" % medtype), null, "%1.4f")
     return null
 
 

@@ -1,0 +1,119 @@
+"""Step 2 from the step-1 hand-off directories: the ``run(opts)`` bodies of
+the reference's randomise scripts, reading what step 1 saved and writing the
+null CSVs where the reference writes them.
+
+* ``vertex_regression``  vertex_tfce_multiple_regression_randomise.py:56-118
+  reads ``python_temp_<surface>/`` (written by
+  STEP_1_vertex_tfce_multiple_regression.py:248-266) and appends to
+  ``output_<surface>/perm_Tstat_<surface>/perm_tstat_con<j>_TFCE_maxVertex.csv``;
+* ``voxel_regression``   voxel_tfce_multiple_regression_randomise.py:51-120
+  reads ``python_temp/`` (STEP_1_voxel_tfce_multiple_regression.py:112-116)
+  and appends to ``output/perm_Tstat/``; with ``ancova`` saved as 1 it runs
+  the F branch over twice the permutation numbers (:79-80);
+* ``vertex_mediation``   vertex_tfce_mediation_randomise.py:48-92 reads
+  ``python_temp_med_<surface>/`` (STEP_1_vertex_tfce_mediation.py:244-277)
+  and appends to ``output_med_<surface>/perm_SobelZ_<medtype>/``;
+* ``voxel_mediation``    voxel_tfce_mediation_randomise.py:42-76 reads
+  ``python_temp/`` (STEP_1_voxel_tfce_mediation.py:90-95) and appends to
+  ``output_med_<medtype>/perm_SobelZ/``.
+
+The permutations of permutation numbers ``start..stop`` come from
+``randomise.permutation_indices`` (numpy's legacy RNG seeded with number +
+seed): the reference seeds from the clock (e.g.
+vertex_tfce_multiple_regression_randomise.py:91), which cannot be reproduced.
+
+Step 1 saves ragged adjacency lists with ``np.save`` as object arrays, which
+numpy can only read by unpickling, and unpickling runs code from the file.
+``allow_pickle`` must therefore be set explicitly, and only for directories
+that the caller's own step-1 run wrote.
+"""
+import os
+
+import numpy as np
+
+from . import randomise
+
+
+def load_temp(directory, names, allow_pickle=False):
+    """{name: array} of ``directory/<name>.npy``."""
+    out = {}
+    for name in names:
+        path = os.path.join(directory, name + ".npy")
+        try:
+            out[name] = np.load(path, allow_pickle=allow_pickle)
+        except ValueError as e:
+            raise ValueError("%s holds Python objects (ragged adjacency lists); pass allow_pickle=True only "
+                             "for a directory your own step-1 run wrote" % path) from e
+    return out
+
+
+def _out_dir(root, *parts):
+    path = os.path.join(root, *parts)
+    os.makedirs(path, exist_ok=True)
+    return path
+
+
+def _perms(n, numbers, seed, blocks=None):
+    return randomise.permutation_indices(int(n), list(numbers), seed=seed, blocks=blocks)
+
+
+def vertex_regression(surface, start, stop, root=".", specifyvars=None, exchangeblock=None, seed=0,
+                      allow_pickle=False, batch=None, precision="fp16x3"):
+    """Permutation numbers start..stop (inclusive) of the vertex GLM of
+    ``surface``; returns the driver's nulls {contrast: [n_perm, 2]}."""
+    tmp = os.path.join(root, "python_temp_%s" % surface)
+    v = load_temp(tmp, ["merge_y", "num_subjects", "pred_x", "adjac_lh", "adjac_rh", "bin_mask_lh",
+                        "bin_mask_rh", "optstfce", "vdensity_lh", "vdensity_rh"], allow_pickle)
+    blocks = None
+    if exchangeblock is not None:
+        blocks = np.genfromtxt(exchangeblock, dtype=str)
+    perms = _perms(v["num_subjects"], range(int(start), int(stop) + 1), seed, blocks)
+    out = _out_dir(root, "output_%s" % surface, "perm_Tstat_%s" % surface)
+    return randomise.vertex_regress(v["merge_y"], v["pred_x"], v["adjac_lh"], v["adjac_rh"], v["bin_mask_lh"],
+                                    v["bin_mask_rh"], v["optstfce"], perms, vdensity_lh=v["vdensity_lh"],
+                                    vdensity_rh=v["vdensity_rh"], specifyvars=specifyvars, out_dir=out,
+                                    batch=batch, precision=precision)
+
+
+def voxel_regression(start, stop, root=".", specifyvars=None, exchangeblock=None, seed=0, allow_pickle=False,
+                     batch=None, precision="fp16x3"):
+    """Permutation numbers start..stop of the voxel GLM saved in
+    ``python_temp/`` (start..2*stop in the F branch)."""
+    tmp = os.path.join(root, "python_temp")
+    v = load_temp(tmp, ["num_subjects", "raw_nonzero_corr", "pred_x", "adjac", "ancova", "optstfce"],
+                  allow_pickle)
+    ancova = int(v["ancova"]) == 1
+    numbers = range(int(start), 2 * int(stop) + 1) if ancova else range(int(start), int(stop) + 1)
+    blocks = None
+    if exchangeblock is not None and not ancova:
+        blocks = np.genfromtxt(exchangeblock, dtype=str)
+    perms = _perms(v["num_subjects"], numbers, seed, blocks)
+    out = _out_dir(root, "output", "perm_Tstat")
+    return randomise.voxel_regress(v["raw_nonzero_corr"].T, v["pred_x"], v["adjac"], v["optstfce"], perms,
+                                   specifyvars=specifyvars, out_dir=out, batch=batch, precision=precision,
+                                   ancova=ancova)
+
+
+def vertex_mediation(surface, medtype, start, stop, root=".", seed=0, allow_pickle=False, batch=None,
+                     precision="fp16x3"):
+    """Permutation numbers start..stop of the vertex mediation of ``surface``."""
+    tmp = os.path.join(root, "python_temp_med_%s" % surface)
+    v = load_temp(tmp, ["merge_y", "num_subjects", "pred_x", "depend_y", "adjac_lh", "adjac_rh", "bin_mask_lh",
+                        "bin_mask_rh", "optstfce", "vdensity_lh", "vdensity_rh"], allow_pickle)
+    perms = _perms(v["num_subjects"], range(int(start), int(stop) + 1), seed)
+    out = _out_dir(root, "output_med_%s" % surface, "perm_SobelZ_%s" % medtype)
+    return randomise.vertex_mediation(v["merge_y"], v["pred_x"], v["depend_y"], medtype, v["adjac_lh"],
+                                      v["adjac_rh"], v["bin_mask_lh"], v["bin_mask_rh"], v["optstfce"], perms,
+                                      vdensity_lh=v["vdensity_lh"], vdensity_rh=v["vdensity_rh"], out_dir=out,
+                                      batch=batch, precision=precision)
+
+
+def voxel_mediation(medtype, start, stop, root=".", seed=0, allow_pickle=False, batch=None, precision="fp16x3"):
+    """Permutation numbers start..stop of the voxel mediation saved in ``python_temp/``."""
+    tmp = os.path.join(root, "python_temp")
+    v = load_temp(tmp, ["num_subjects", "raw_nonzero_corr", "pred_x", "depend_y", "adjac", "optstfce"],
+                  allow_pickle)
+    perms = _perms(v["num_subjects"], range(int(start), int(stop) + 1), seed)
+    out = _out_dir(root, "output_med_%s" % medtype, "perm_SobelZ")
+    return randomise.voxel_mediation(v["raw_nonzero_corr"].T, v["pred_x"], v["depend_y"], medtype, v["adjac"],
+                                     v["optstfce"], perms, out_dir=out, batch=batch, precision=precision)

@@ -296,6 +296,9 @@ def main():
                     help="contraction tile (0: 256x128, 1: 512x64, 2: 512x128) instead of autotuning")
     ap.add_argument("--wgs-per-cu", type=int, default=None,
                     help="sweep workgroups per CU instead of autotuning")
+    ap.add_argument("--lds-sweep", type=int, default=None, choices=(0, 1),
+                    help="surfaces of at most 12,544 elements in the LDS-resident sweep (1) or the global one (0) "
+                         "instead of autotuning")
     ap.add_argument("--pipeline", action="store_true",
                     help="run the contraction of step s+1 on a second stream during the sweep of step s "
                          "(engine.pipeline) instead of one after the other on one stream")
@@ -331,10 +334,11 @@ def main():
 
     # ---- launch shapes picked on this workload (not cached: measured here) ----
     tuned = None
-    if args.tile is not None or args.wgs_per_cu is not None:
+    if args.tile is not None or args.wgs_per_cu is not None or args.lds_sweep is not None:
         from tfce_mediation_amd import autotune
         tuned = {"gemm_tile": 0 if args.tile is None else args.tile,
-                 "tfce_wgs_per_cu": 4 if args.wgs_per_cu is None else args.wgs_per_cu}
+                 "tfce_wgs_per_cu": 4 if args.wgs_per_cu is None else args.wgs_per_cu,
+                 "lds_sweep": 1 if args.lds_sweep is None else args.lds_sweep}
         autotune.apply(tuned)
     elif not args.no_autotune:
         from tfce_mediation_amd import autotune
@@ -572,7 +576,8 @@ def main():
             "hipgraph_replay_perms_per_s": None if graph_rate is None else round(graph_rate, 1),
             "stat_rel_err_vs_fp64": t_err or None,
             "autotuned": None if tuned is None else {"gemm_tile": ["256x128", "512x64", "512x128"][tuned["gemm_tile"]],
-                                                     "tfce_wgs_per_cu": tuned["tfce_wgs_per_cu"]},
+                                                     "tfce_wgs_per_cu": tuned["tfce_wgs_per_cu"],
+                                                     "lds_sweep": tuned.get("lds_sweep", 1)},
             "schedule": mode + (" (contraction of step s+1 on a second stream during the sweep of step s)"
                                 if args.pipeline else " (contraction then sweep, one stream)"),
             "contraction_ms_per_step": round(gemm_ms, 3),

@@ -123,8 +123,9 @@ typedef struct tfce_surfset tfce_surfset_t;
 int tfce_surfset_create(const tfce_adj_t *const *adjs, int32_t n, const int64_t *offsets,
                         tfce_surfset_t **out);
 int tfce_surfset_destroy(tfce_surfset_t *set);
-/* Launch order of a set (largest surface first): user_index[k] = caller's
- * index of the k-th surface; outputs of tfce_batch follow this order. */
+/* Launch order of a set (largest surface first; surfaces the LDS-resident
+ * sweep takes after all others): user_index[k] = caller's index of the k-th
+ * surface; outputs of tfce_batch follow this order. */
 int tfce_surfset_order(const tfce_surfset_t *set, int32_t *user_index);
 
 /* One launch over every (surface, map, sign) item, largest surface first.
@@ -269,6 +270,13 @@ int tfce_batch_ex(const tfce_surfset_t *set, const float *d_maps, int64_t map_st
 
 /* Number of resident workgroups per CU used by the TFCE sweep (autotuner knob). */
 int tfce_set_occupancy(int32_t wgs_per_cu);
+
+/* Batched maxima (tfce_batch*, tfce_batch_max*) of symmetric surfaces of at
+ * most 12,544 elements with a neighbour table run in an LDS-resident sweep
+ * (the whole union-find of an image in one compute unit's LDS) unless this
+ * process-wide knob is 0; the maxima are the same bit for bit either way.
+ * No counterpart in the reference. */
+int tfce_set_lds_sweep(int32_t enable);
 
 /* ---------------------------------------------------- learned early stop */
 /* No counterpart in the reference (nearest: the parametric null fits of

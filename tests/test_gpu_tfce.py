@@ -226,3 +226,85 @@ def test_large_hemisphere_maps(torch):
             x = (sgn * maps[p]).astype(np.float32)
             enh, _, _ = otfce.tfce(row_ptr, col, x, 2.0, 1.0, exact_pow=True)
             assert out[p, k] == np.float32(enh.max() * np.float32(x.max() / np.float32(100)))
+
+
+@pytest.mark.parametrize("gname,E", [("ico3", 1.0), ("ico4_shuffled", 1.0), ("ico5", 1.0), ("ico5_masked", 1.0),
+                                     ("ico4_w8", 1.0), ("ico4_w12", 1.0), ("ico4_w20", 0.5), ("grid8", 0.5),
+                                     ("vox26", 0.5), ("vox6_walls", 1.0)])
+def test_lds_sweep_matches_global_sweep(torch, gname, E):
+    """Surfaces of at most 12,544 elements run in the LDS-resident sweep; the
+    global-memory sweep (knob off) must give the same maxima bit for bit, and
+    both match the oracle (asymmetric lists always take the global sweep)."""
+    from tfce_mediation_amd import _lib
+    from tfce_mediation_amd.tfce import Adjacency
+    row_ptr, col = GRAPHS[gname]()
+    V = len(row_ptr) - 1
+    P = 40
+    say(f"[gpu] LDS vs global sweep {gname} V={V} P={P} E={E}")
+    maps = np.stack([smooth_map(row_ptr, col, 500 + p, shift=0.0) for p in range(P)])
+    if gname == "vox6_walls":
+        maps = maps + np.float32(1e-4) * np.arange(V, dtype=np.float32) / V
+    adj = Adjacency((row_ptr, col), 2.0, E)
+    d = torch.from_numpy(np.ascontiguousarray(maps, np.float32)).cuda()
+    lib = _lib.lib()
+    try:
+        _lib.check(lib.tfce_set_lds_sweep(1))
+        lds = adj.batch_max(d, 3).cpu().numpy()
+        one = adj.batch_max(d, 2).cpu().numpy()
+        _lib.check(lib.tfce_set_lds_sweep(0))
+        glob = adj.batch_max(d, 3).cpu().numpy()
+    finally:
+        _lib.check(lib.tfce_set_lds_sweep(1))
+    np.testing.assert_array_equal(lds, glob)
+    assert np.all(one[:, 0] == 0) and np.array_equal(one[:, 1], lds[:, 1])
+    for p in range(0, P, 5):
+        for k, sgn in enumerate((1, -1)):
+            x = (sgn * maps[p]).astype(np.float32)
+            enh, _, _ = otfce.tfce(row_ptr, col, x, 2.0, E, exact_pow=True)
+            exp = np.float32(enh.max() * np.float32(x.max() / np.float32(100)))
+            if E == 1.0:
+                assert lds[p, k] == exp, (p, k, lds[p, k], exp)
+            else:
+                assert ulp_diff(np.float32([lds[p, k]]), np.float32([exp])).max() <= 4
+
+
+def test_mixed_set_splits_between_sweeps(torch):
+    """A set with surfaces above and below the LDS limit: the large ones in
+    the global sweep, the small ones in the LDS sweep, per-surface maxima
+    equal to the all-global launch and to the oracle."""
+    from tfce_mediation_amd import _lib
+    from tfce_mediation_amd.tfce import Adjacency, SurfaceSet
+    graphs = [synth.icosphere_csr(6), synth.icosphere_csr(3), GRAPHS["vox26"](), synth.icosphere_csr(5),
+              GRAPHS["vox6_walls"]()]
+    sizes = [len(g[0]) - 1 for g in graphs]
+    offs = np.cumsum([0] + sizes[:-1])
+    width = int(sum(sizes))
+    P = 12
+    say(f"[gpu] mixed LDS/global set sizes={sizes} P={P}")
+    rng = np.random.default_rng(3)
+    maps = np.zeros((P, width), np.float32)
+    for i, (g, o) in enumerate(zip(graphs, offs)):
+        for p in range(P):
+            maps[p, o:o + sizes[i]] = smooth_map(g[0], g[1], 700 + 13 * p + i, shift=0.0)
+    Es = [1.0, 1.0, 0.5, 1.0, 1.0]
+    ss = SurfaceSet([Adjacency(g, 2.0, e) for g, e in zip(graphs, Es)], offs)
+    d = torch.from_numpy(maps).cuda()
+    lib = _lib.lib()
+    try:
+        _lib.check(lib.tfce_set_lds_sweep(1))
+        got = ss.batch(d, 3).cpu().numpy()
+        _lib.check(lib.tfce_set_lds_sweep(0))
+        ref = ss.batch(d, 3).cpu().numpy()
+    finally:
+        _lib.check(lib.tfce_set_lds_sweep(1))
+    np.testing.assert_array_equal(got, ref)
+    for i, (g, o) in enumerate(zip(graphs, offs)):
+        for p in range(0, P, 4):
+            for k, sgn in enumerate((1, -1)):
+                x = (sgn * maps[p, o:o + sizes[i]]).astype(np.float32)
+                enh, _, _ = otfce.tfce(g[0], g[1], x, 2.0, Es[i], exact_pow=True)
+                exp = np.float32(enh.max() * np.float32(x.max() / np.float32(100)))
+                if Es[i] == 1.0:
+                    assert got[i, p, k] == exp, (i, p, k)
+                else:
+                    assert ulp_diff(np.float32([got[i, p, k]]), np.float32([exp])).max() <= 4

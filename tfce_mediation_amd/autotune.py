@@ -1,10 +1,14 @@
-"""Autotuner for the two launch-shape knobs of the batched path.
+"""Autotuner for the launch-shape knobs of the batched path.
 
 * the workgroup tile of the split-precision contraction
   (``tfce_ols_set_tile``: 256 x 128, 512 x 64 or 512 x 128 rows x elements);
 * the number of resident sweep workgroups per compute unit
   (``tfce_set_occupancy``): more hide more latency, fewer keep more of each
-  slot's scratch in cache.
+  slot's scratch in cache;
+* whether surfaces of at most 12,544 elements take the LDS-resident sweep
+  (``tfce_set_lds_sweep``; one workgroup per compute unit holding both signs
+  of a map) or the global-memory sweep; both must give the same maxima bit
+  for bit, and the faster is kept.
 
 Each candidate is timed with HIP events on the real workload (one batch of P
 permutations of an engine), the contraction and the sweep separately since
@@ -52,11 +56,13 @@ def apply(cfg):
     lib = _lib.lib()
     _lib.check(lib.tfce_ols_set_tile(int(cfg["gemm_tile"])))
     _lib.check(lib.tfce_set_occupancy(int(cfg["tfce_wgs_per_cu"])))
+    _lib.check(lib.tfce_set_lds_sweep(int(cfg.get("lds_sweep", 1))))
 
 
 def tune(engine, P=256, reps=3, tiles=TILES, occupancies=OCCUPANCIES, cache=DEFAULT_CACHE, seed=0, log=None):
-    """Pick and apply the fastest tile and occupancy for engine's batches of P
-    permutations; returns {"gemm_tile", "tfce_wgs_per_cu", "timings_ms"}."""
+    """Pick and apply the fastest tile, occupancy and sweep kind for engine's
+    batches of P permutations; returns {"gemm_tile", "tfce_wgs_per_cu",
+    "lds_sweep", "timings_ms"}."""
     from . import synth
     key = workload_key(engine, P)
     if cache and os.path.exists(cache):
@@ -86,15 +92,30 @@ def tune(engine, P=256, reps=3, tiles=TILES, occupancies=OCCUPANCIES, cache=DEFA
     best_tile = min(timings["gemm"], key=timings["gemm"].get)
     _lib.check(lib.tfce_ols_set_tile(best_tile))
     t = engine.tvalues(perms)
+    signs = getattr(engine, "sign_mask", 3)
+
+    def sweep():
+        return engine.surfaces.batch(t, signs, vdensity=engine.vdensity)
+    _lib.check(lib.tfce_set_lds_sweep(0))
     for occ in occupancies:
         _lib.check(lib.tfce_set_occupancy(occ))
-        timings["tfce"][occ] = _time(lambda: engine.surfaces.batch(t, 3, vdensity=engine.vdensity), reps)
+        timings["tfce"][occ] = _time(sweep, reps)
     best_occ = min(timings["tfce"], key=timings["tfce"].get)
-    cfg = {"gemm_tile": int(best_tile), "tfce_wgs_per_cu": int(best_occ),
+    _lib.check(lib.tfce_set_occupancy(best_occ))
+    ref_null = sweep().cpu().numpy()
+    _lib.check(lib.tfce_set_lds_sweep(1))
+    lds = 0
+    if np.array_equal(sweep().cpu().numpy(), ref_null, equal_nan=True):
+        timings["tfce"]["lds"] = _time(sweep, reps)
+        lds = 1 if timings["tfce"]["lds"] < timings["tfce"][best_occ] else 0
+    elif log:
+        log("[autotune] the LDS sweep changes the maxima; dropped")
+    cfg = {"gemm_tile": int(best_tile), "tfce_wgs_per_cu": int(best_occ), "lds_sweep": lds,
            "timings_ms": {k: {str(a): round(b, 3) for a, b in v.items()} for k, v in timings.items()}}
     apply(cfg)
     if log:
-        log(f"[autotune] {key}: tile {best_tile}, {best_occ} sweep workgroups per CU; {cfg['timings_ms']}")
+        log(f"[autotune] {key}: tile {best_tile}, {best_occ} sweep workgroups per CU, LDS sweep {lds}; "
+            f"{cfg['timings_ms']}")
     if cache:
         try:
             os.makedirs(os.path.dirname(cache), exist_ok=True)

@@ -36,6 +36,8 @@ struct tfce_surfset {
   int32_t np_w = 0;  // widest neighbour-position table among the surfaces
   std::vector<SurfDev> host;  // in launch order (largest first)
   std::vector<int32_t> user_index;  // launch position -> caller's surface index
+  bool all_ring = true;             // every surface ring-packed
+  std::vector<uint8_t> lds_ok;      // per launch position: the LDS-resident sweep may take it
   SurfDev* d_surfs = nullptr;
 };
 
@@ -51,6 +53,7 @@ int set_error(int code, const std::string& msg) {
 static std::mutex g_ws_mutex;
 static std::map<std::pair<int, hipStream_t>, Workspace> g_ws;
 static int g_wgs_per_cu = 4;
+static bool g_lds_sweep = true;  // small surfaces in the LDS-resident sweep
 
 int current_device() {
   int d = 0;
@@ -164,6 +167,11 @@ extern "C" {
 
 const char* tfce_last_error(void) { return g_last_error.c_str(); }
 const char* tfce_version(void) { return "tfcehip 0.1 gfx950"; }
+
+int tfce_set_lds_sweep(int32_t enable) {
+  g_lds_sweep = enable != 0;
+  return 0;
+}
 
 int tfce_set_occupancy(int32_t wgs_per_cu) {
   if (wgs_per_cu < 1 || wgs_per_cu > 16) return set_error(TFCE_EINVAL, "occupancy must be in [1, 16]");
@@ -451,12 +459,19 @@ static int32_t np_width(const tfce_adj* a) {
   return ((a->flags & kSurfSymmetric) && a->ell_width > 0 && a->ell_width <= kNposMaxWidth) ? a->ell_width : 0;
 }
 
+// Whether the LDS-resident sweep takes a surface: symmetric, with a neighbour
+// table, at most kLdsMaxV elements.
+static bool lds_eligible(const tfce_adj* a) {
+  return (a->flags & kSurfSymmetric) && a->d_ell != nullptr && a->V <= kLdsMaxV && g_lds_sweep;
+}
+
 static int run_batch(const SurfDev* d_surfs, int32_t n_surf, int32_t v_max, int32_t np_w, const float* d_maps,
                      int64_t map_stride, int P, int sign_mask, const float* d_T0,
                      const float* d_vdensity, bool full, float* d_enhn, float min_t, float dh,
                      int32_t labels_level, int32_t* d_labels, float* d_scaled, float* d_raw,
                      float* d_own, int32_t* d_status, hipStream_t stream, int out_mul = 1,
-                     int out_add = 0, uint32_t batch_flags = 0, const ExtScratch* ext = nullptr) {
+                     int out_add = 0, uint32_t batch_flags = 0, const ExtScratch* ext = nullptr,
+                     bool ring_only = false, int32_t n_lds_surf = 0) {
   if (P <= 0) return 0;
   if (sign_mask < 1 || sign_mask > 3) return set_error(TFCE_EINVAL, "sign_mask must be 1, 2 or 3");
   const int n_signs = (sign_mask == 3) ? 2 : 1;
@@ -466,7 +481,11 @@ static int run_batch(const SurfDev* d_surfs, int32_t n_surf, int32_t v_max, int3
   // resident workgroups, each with its own slot; the slots stay within a
   // quarter of the device's memory (fewer workgroups for very large surfaces)
   const int64_t mem_cap = (int64_t)(device_memory(current_device()) / 4 / slot_bytes);
-  int grid = (int)std::max<int64_t>(1, std::min<int64_t>(tfce_grid(n_items), mem_cap));
+  // the trailing n_lds_surf surfaces (the smallest) run in the LDS-resident
+  // sweep, one 1,024-thread workgroup per compute unit, after the others
+  const int64_t n_lds_items = full ? 0 : (int64_t)std::min(n_lds_surf, n_surf) * P * n_signs;
+  const int64_t n_global = n_items - n_lds_items;
+  int grid = (int)std::max<int64_t>(1, std::min<int64_t>(tfce_grid(std::max<int64_t>(n_global, 1)), mem_cap));
   const size_t prep_bytes = (size_t)n_items * kPrepWords * sizeof(int32_t);
   // raw_max is mandatory for the kernel: park it in scratch when not wanted
   const size_t raw_bytes = (size_t)n_items * sizeof(float);
@@ -533,8 +552,14 @@ static int run_batch(const SurfDev* d_surfs, int32_t n_surf, int32_t v_max, int3
     if (e0 != hipSuccess) return set_error(-(int)e0, "shared threshold launch");
     a.ext_T0 = T0;
   }
-  hipError_t e = launch_tfce_batch(a, full, grid, stream);
+  hipError_t e = launch_tfce_batch(a, full, ring_only, grid, n_global, stream);
   if (e != hipSuccess) return set_error(-(int)e, std::string("tfce launch: ") + hipGetErrorString(e));
+  if (n_lds_items > 0) {
+    // one workgroup per compute unit, each taking both signs of a map at a time
+    const int lgrid = (int)std::max<int64_t>(1, std::min<int64_t>(n_lds_items / n_signs, cu_count(current_device())));
+    e = launch_tfce_lds(a, n_global, lgrid, stream);
+    if (e != hipSuccess) return set_error(-(int)e, std::string("tfce LDS launch: ") + hipGetErrorString(e));
+  }
   return 0;
 }
 
@@ -554,7 +579,8 @@ int batch_max_in(tfce_adj_t* adj, const float* d_maps, int P, int sign_mask, con
   }
   return run_batch(adj->d_surf, 1, adj->V, np_width(adj), d_maps, adj->V, P, sign_mask, nullptr, d_vdensity,
                    d_vdensity != nullptr, nullptr, 0.0f, 0.0f, -1, nullptr, d_max, nullptr, nullptr, nullptr,
-                   stream, sign_mask == 3 ? 1 : 2, sign_mask == 2 ? 1 : 0, 0u, ext);
+                   stream, sign_mask == 3 ? 1 : 2, sign_mask == 2 ? 1 : 0, 0u, ext, (adj->flags & kSurfRing) != 0,
+                   lds_eligible(adj) ? 1 : 0);
 }
 
 }  // namespace tfcehip
@@ -570,7 +596,8 @@ int tfce_run_ex(tfce_adj_t* adj, const float* d_image, float* d_enhn, float min_
     if (e != hipSuccess) return set_error(-(int)e, "fill enhn");
   }
   return run_batch(adj->d_surf, 1, adj->V, np_width(adj), d_image, adj->V, 1, 1, nullptr, nullptr, true, d_enhn, min_t,
-                   dh, labels_level, d_labels, nullptr, nullptr, nullptr, d_status, stream);
+                   dh, labels_level, d_labels, nullptr, nullptr, nullptr, d_status, stream, 1, 0, 0u, nullptr,
+                   (adj->flags & kSurfRing) != 0);
 }
 
 int tfce_run(tfce_adj_t* adj, const float* d_image, float* d_enhn, float dh, int accumulate,
@@ -583,6 +610,26 @@ int tfce_batch_max(tfce_adj_t* adj, const float* d_maps, int P, int sign_mask, c
   return tfcehip::batch_max_in(adj, d_maps, P, sign_mask, d_vdensity, d_max, nullptr, stream);
 }
 
+}  // extern "C"
+
+// Number of trailing surfaces (smallest first from the end) the LDS-resident
+// sweep takes, and whether the others are all ring-packed.
+static int32_t lds_tail(const tfce_surfset* s) {
+  if (!g_lds_sweep) return 0;
+  int32_t k = 0;
+  while (k < s->n && s->lds_ok[s->n - 1 - k]) ++k;
+  return k;
+}
+
+static bool big_all_ring(const tfce_surfset* s) {
+  const int32_t n_big = s->n - lds_tail(s);
+  for (int32_t k = 0; k < n_big; ++k)
+    if (!(s->host[k].flags & kSurfRing)) return false;
+  return true;
+}
+
+extern "C" {
+
 int tfce_surfset_create(const tfce_adj_t* const* adjs, int32_t n, const int64_t* offsets, tfce_surfset_t** out) {
   if (!out || !adjs || !offsets || n <= 0) return set_error(TFCE_EINVAL, "bad surface set arguments");
   *out = nullptr;
@@ -594,7 +641,17 @@ int tfce_surfset_create(const tfce_adj_t* const* adjs, int32_t n, const int64_t*
     if (!adjs[i]) { delete s; return set_error(TFCE_EINVAL, "NULL adjacency in set"); }
     idx[i] = i;
   }
-  std::stable_sort(idx.begin(), idx.end(), [&](int32_t a, int32_t b) { return adjs[a]->V > adjs[b]->V; });
+  // launch order: the surfaces the LDS-resident sweep can take last (they form
+  // its item range), largest first within each group (work balance of the
+  // persistent queues)
+  auto small = [&](int32_t i) {
+    const tfce_adj* a = adjs[i];
+    return (a->flags & kSurfSymmetric) && a->d_ell != nullptr && a->V <= kLdsMaxV;
+  };
+  std::stable_sort(idx.begin(), idx.end(), [&](int32_t a, int32_t b) {
+    if (small(a) != small(b)) return !small(a);
+    return adjs[a]->V > adjs[b]->V;
+  });
   for (int32_t k = 0; k < n; ++k) {
     const tfce_adj* a = adjs[idx[k]];
     SurfDev sd;
@@ -613,6 +670,8 @@ int tfce_surfset_create(const tfce_adj_t* const* adjs, int32_t n, const int64_t*
     s->user_index.push_back(idx[k]);
     s->v_max = std::max(s->v_max, a->V);
     s->np_w = std::max(s->np_w, np_width(a));
+    s->all_ring = s->all_ring && (a->flags & kSurfRing) != 0;
+    s->lds_ok.push_back(small(idx[k]) ? 1 : 0);
   }
   hipError_t e = hipMalloc(&s->d_surfs, sizeof(SurfDev) * n);
   if (e == hipSuccess) e = hipMemcpy(s->d_surfs, s->host.data(), sizeof(SurfDev) * n, hipMemcpyHostToDevice);
@@ -640,7 +699,7 @@ int tfce_batch_ex(const tfce_surfset_t* s, const float* d_maps, int64_t map_stri
     return set_error(TFCE_EINVAL, "unknown batch flag");
   return run_batch(s->d_surfs, s->n, s->v_max, s->np_w, d_maps, map_stride, P, sign_mask, nullptr, d_vdensity,
                    d_vdensity != nullptr, nullptr, 0.0f, 0.0f, -1, nullptr, d_scaled, d_raw, d_own, d_status,
-                   stream, 1, 0, flags);
+                   stream, 1, 0, flags, nullptr, big_all_ring(s), lds_tail(s));
 }
 
 int tfce_surfset_order(const tfce_surfset_t* s, int32_t* user_index) {
@@ -655,7 +714,7 @@ int tfce_batch(const tfce_surfset_t* s, const float* d_maps, int64_t map_stride,
   if (!s || !d_maps || !d_scaled) return set_error(TFCE_EINVAL, "NULL argument");
   return run_batch(s->d_surfs, s->n, s->v_max, s->np_w, d_maps, map_stride, P, sign_mask, d_T0, d_vdensity,
                    d_vdensity != nullptr, nullptr, 0.0f, 0.0f, -1, nullptr, d_scaled, d_raw, d_own, d_status,
-                   stream);
+                   stream, 1, 0, 0u, nullptr, big_all_ring(s), lds_tail(s));
 }
 
 }  // extern "C"

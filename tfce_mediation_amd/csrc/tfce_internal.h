@@ -37,6 +37,11 @@ constexpr int32_t kSurfRing = 8;
 constexpr int kRingMaxDeg = 6;
 constexpr int32_t kRingNone = 0x3ffff;  // 18-bit position of an inactive neighbour
 constexpr int32_t kRingRowWords = 4;
+// Largest surface the LDS-resident sweep holds (12 bytes per element plus the
+// level tables in 160 KB of LDS), and the most levels it takes (a step of
+// max/100 gives at most 102).
+constexpr int32_t kLdsMaxV = 13248;
+constexpr int kLdsMaxLevels = 128;
 
 struct SurfDev {
   const int32_t* row_ptr;  // V+1
@@ -93,7 +98,18 @@ constexpr int kPrepDT = 3;    // step (float bits)
 constexpr int kPrepT = 4;     // kMaxLevels thresholds, then kMaxLevels values of T^H
 constexpr int kPrepWords = kPrepT + 2 * kMaxLevels;
 
-hipError_t launch_tfce_batch(const TfceBatchArgs& a, bool full, int grid, hipStream_t stream);
+// Clears the work counters, writes every item's record (a.n_items) and runs
+// the global-memory sweep over items [0, n_global) (none when n_global <= 0).
+hipError_t launch_tfce_batch(const TfceBatchArgs& a, bool full, bool ring_only, int grid, int64_t n_global,
+                             hipStream_t stream);
+// The LDS-resident MAX sweep over items [item0, a.n_items) (symmetric surfaces
+// of at most kLdsMaxV elements with a neighbour table; item0 a multiple of
+// a.n_signs: one workgroup takes both signs of a map); its work counter is the
+// second 64-bit word of a.queue.  Run after launch_tfce_batch, which writes
+// the items' records and clears the counters.  Only for the step max/100
+// (dh == 0): an item with more than kLdsMaxLevels levels gets NaN and
+// status kStatusTooManyLevels.
+hipError_t launch_tfce_lds(const TfceBatchArgs& a, int64_t item0, int grid, hipStream_t stream);
 hipError_t launch_sizepow(double* out, int32_t n, float E, hipStream_t stream);
 hipError_t launch_shared_T0(const TfceBatchArgs& a, float* T0, int zero_in_image, hipStream_t stream);
 hipError_t launch_fill_f32(float* p, int64_t n, float v, hipStream_t stream);

@@ -499,7 +499,10 @@ __global__ __launch_bounds__(kTfceThreads) void tfce_prepare_kernel(TfceBatchArg
   rec[kPrepDT] = __float_as_int(dT);
 }
 
-template <bool FULL>
+// RING: every surface of the launch is ring-packed (closed triangle meshes of
+// at most six neighbours per vertex); only that path is compiled, which keeps
+// the register allocation of the other table layouts out of the kernel.
+template <bool FULL, bool RING>
 __global__ __launch_bounds__(kTfceThreads, FULL ? 3 : 4) void tfce_batch_kernel(TfceBatchArgs a) {
   __shared__ float s_T[kMaxLevels];
   __shared__ float s_HH[kMaxLevels];
@@ -537,7 +540,7 @@ __global__ __launch_bounds__(kTfceThreads, FULL ? 3 : 4) void tfce_batch_kernel(
     const float* x = a.maps + (int64_t)itr.p * a.map_stride + S.offset;
     float* enhn = (FULL && a.enhn) ? a.enhn + (int64_t)itr.p * a.map_stride + S.offset : nullptr;
     const float* vd = (FULL && a.vdensity) ? a.vdensity + S.offset : nullptr;
-    const bool symmetric = (S.flags & kSurfSymmetric) != 0;
+    const bool symmetric = RING || (S.flags & kSurfSymmetric) != 0;
     const int32_t* ell = S.ell;
     const int W = S.ell_width;
     const int32_t* rec = a.prep + item * kPrepWords;
@@ -594,8 +597,8 @@ __global__ __launch_bounds__(kTfceThreads, FULL ? 3 : 4) void tfce_batch_kernel(
     // ---- phase 2: level order ----------------------------------------------
     // With neighbour-position rows the MAX sweep never reads the level order
     // (vertex of a position): only the positions are written then.
-    const bool ring = (S.flags & kSurfRing) != 0;
-    const bool use_np = symmetric && ell != nullptr && (ring ? kRingRowWords : W) <= a.np_w;
+    const bool ring = RING || (S.flags & kSurfRing) != 0;
+    const bool use_np = RING || (symmetric && ell != nullptr && (ring ? kRingRowWords : W) <= a.np_w);
     const bool need_order = FULL || !use_np;
     if (L > 0) {
       // the levels again from the map (cheaper than storing and re-reading
@@ -1088,6 +1091,400 @@ __global__ __launch_bounds__(kTfceThreads, FULL ? 3 : 4) void tfce_batch_kernel(
   }
 }
 
+// ---------------------------------------------------------------------------
+// LDS-resident sweep (MAX mode) for symmetric surfaces of at most kLdsMaxV
+// elements.  One 1,024-thread workgroup per compute unit takes one map of a
+// surface with both of its signs: the elements active in +map and in -map
+// are disjoint (x > T >= 0 against -x > T >= 0), so the two images share one
+// set of per-element arrays in LDS - activation position (the + image's
+// elements first, then the - image's) and level order (16 bits each), one
+// 32-bit union-find word per element (parent in the low 16 bits, set size in
+// the high 16) and the value per element - and the two threshold sweeps run
+// level by level through the same barriers.  Finds, hooks and size merges are
+// LDS operations; only the map and the neighbour rows (shared by every image
+// of the surface, cache resident) come from global memory.  Per level:
+//   union     new elements join their active neighbours of the same image
+//             (hook the larger root under the smaller by CAS on the parent
+//             bits; sizes do not change in this phase, so a CAS fails only
+//             when its root was hooked meanwhile);
+//   merge     every element hooked at this level passes the size it held
+//             (1 for a new element, the whole set for an old root) and its
+//             value (atomicMax) to its final root, then points straight at it
+//             with size 0 (so it is never counted twice); meanwhile each
+//             thread gathers and prunes the neighbour candidates of its element
+//             of the next level (they depend only on the positions);
+//   increment every root adds (float)(size^E * T^H); the next level's
+//             elements are initialised in the same pass.
+// Same arithmetic as tfce_batch_kernel<false>: identical maxima, bit for bit.
+constexpr int kLdsThreads = 1024;
+
+struct LdsSweepShared {
+  float T[2][kLdsMaxLevels];
+  float HH[2][kLdsMaxLevels];
+  int start[2][kLdsMaxLevels + 2];  // level l of image g: positions [base[g] + start[g][l], base[g] + start[g][l+1])
+  int cur[2][kLdsMaxLevels];
+  int base[2];
+  float red[2][kLdsThreads / 64];
+  int unit_lo, unit_hi;
+  uint16_t pos[kLdsMaxV];    // element -> position in level order, 0xffff: never active
+  uint16_t order[kLdsMaxV];  // position -> element
+  uint32_t word[kLdsMaxV];   // parent | size << 16
+  float val[kLdsMaxV];       // MAX value of the set a root heads
+};
+
+__device__ __forceinline__ uint32_t lds_ld(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_st(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Root of x.  SPLIT re-points a visited node at its grandparent, keeping its
+// size bits: only safe while no size changes (the union phase).
+template <bool SPLIT>
+__device__ __forceinline__ int lds_find(uint32_t* w, int x) {
+  uint32_t wx = lds_ld(w + x);
+  int p = (int)(wx & 0xffffu);
+  while (p != x) {
+    const uint32_t wp = lds_ld(w + p);
+    const int gp = (int)(wp & 0xffffu);
+    if (SPLIT && gp != p) lds_st(w + x, (wx & 0xffff0000u) | (uint32_t)gp);
+    x = p;
+    wx = wp;
+    p = gp;
+  }
+  return x;
+}
+
+// Join the sets of a and b (element positions or ancestors); returns the
+// joined root (an ancestor of both afterwards).
+__device__ __forceinline__ int lds_link(uint32_t* w, int a, int b) {
+  while (true) {
+    a = lds_find<true>(w, a);
+    b = lds_find<true>(w, b);
+    if (a == b) return a;
+    const int hi = max(a, b), lo = min(a, b);
+    uint32_t old = lds_ld(w + hi);
+    if ((int)(old & 0xffffu) != hi) continue;  // hooked meanwhile
+    if (__hip_atomic_compare_exchange_strong(w + hi, &old, (old & 0xffff0000u) | (uint32_t)lo, __ATOMIC_RELAXED,
+                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
+      return lo;
+  }
+}
+
+// Position of element u in the image whose positions start at lo, kNever when
+// u is not active in that image.
+__device__ __forceinline__ int lds_pos(const uint16_t* pos, int u, int lo) {
+  const int q = pos[u];
+  return (q == 0xffff || q < lo) ? kNever : q;
+}
+
+// Candidates of element position i (image positions from lo, level from b0)
+// of a ring-packed surface, from its packed row: one neighbour per group of
+// mutually adjacent old neighbours, every joined neighbour of this level
+// (prune8).
+__device__ __forceinline__ void lds_ring_cands_row(const LdsSweepShared& sh, const int4 row, int i, int lo, int b0,
+                                                   int (&r)[9]) {
+  int u[kRingMaxDeg];
+  unsigned tail;
+  ring_fields(row, u, tail);
+  int q[8];
+#pragma unroll
+  for (int j = 0; j < kRingMaxDeg; ++j) q[j] = u[j] != kRingNone ? lds_pos(sh.pos, u[j], lo) : kNever;
+  q[6] = q[7] = kNever;
+  unsigned adj[8];
+  const unsigned link = tail & 63u;
+  const int deg = (int)(tail >> 6);
+#pragma unroll
+  for (int j = 0; j < 8; ++j) adj[j] = 0;
+#pragma unroll
+  for (int j = 0; j < 5; ++j)
+    if ((link >> j) & 1u) {
+      adj[j] |= 2u << j;
+      adj[j + 1] |= 1u << j;
+    }
+  if (link & 32u) {
+#pragma unroll
+    for (int j = 2; j < kRingMaxDeg; ++j)
+      if (j == deg - 1) {
+        adj[j] |= 1u;
+        adj[0] |= 1u << j;
+      }
+  }
+  prune8(q, adj, i, b0, r);
+}
+
+__device__ __forceinline__ void lds_ring_cands(const LdsSweepShared& sh, const SurfDev& S, int i, int lo, int b0,
+                                               int (&r)[9]) {
+  lds_ring_cands_row(sh, reinterpret_cast<const int4*>(S.nadj)[sh.order[i]], i, lo, b0, r);
+}
+
+// Union step of element position i (image positions from lo, level from b0)
+// of a surface with a fixed-width table (pruned per group of 8 when adjacency
+// bytes exist) or with the CSR only.
+__device__ __forceinline__ void lds_join_table(LdsSweepShared& sh, const SurfDev& S, int i, int lo, int b0) {
+  const int v = sh.order[i];
+  int ri = i;
+  if (S.ell) {
+    const int W = S.ell_width;
+    const int4* row = reinterpret_cast<const int4*>(S.ell + (size_t)v * W);
+    for (int e = 0; e < W; e += 8) {
+      const int4 l4 = row[e >> 2];
+      const int4 h4 = (e + 4 < W) ? row[(e >> 2) + 1] : make_int4(-1, -1, -1, -1);
+      const int uu[8] = {l4.x, l4.y, l4.z, l4.w, h4.x, h4.y, h4.z, h4.w};
+      int q[8], r[9];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) q[j] = uu[j] >= 0 ? lds_pos(sh.pos, uu[j], lo) : kNever;
+      if (S.nadj) {
+        unsigned wlo = 0, whi = 0, adj[8];
+        chunk_adj(S.nadj, v, W, e, wlo, whi);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          adj[j] = (wlo >> (8 * j)) & 0xffu;
+          adj[4 + j] = (whi >> (8 * j)) & 0xffu;
+        }
+        prune8(q, adj, i, b0, r);
+      } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) r[j] = q[j] < i ? q[j] : -1;
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (r[j] >= 0) ri = lds_link(sh.word, ri, r[j]);
+      if (uu[7] < 0) break;  // -1 padding: the row has ended
+    }
+  } else {
+    for (int e = S.row_ptr[v]; e < S.row_ptr[v + 1]; ++e) {
+      const int q = lds_pos(sh.pos, S.col[e], lo);
+      if (q < i) ri = lds_link(sh.word, ri, q);
+    }
+  }
+}
+
+// The k-th new element of level l over both images: its position, the first
+// position of its image and of its level.  n0 = new elements of image 0.
+struct LdsNew {
+  int i, lo, b0;
+};
+__device__ __forceinline__ LdsNew lds_new(const LdsSweepShared& sh, int k, int n0, int l) {
+  LdsNew e;
+  const int g = k < n0 ? 0 : 1;
+  e.lo = sh.base[g];
+  e.b0 = e.lo + sh.start[g][l];
+  e.i = e.b0 + (g ? k - n0 : k);
+  return e;
+}
+
+__global__ __launch_bounds__(kLdsThreads, 1) void tfce_lds_kernel(TfceBatchArgs a, int64_t unit0) {
+  __shared__ LdsSweepShared sh;
+  const int tid = threadIdx.x;
+  const int ng = a.n_signs;
+  unsigned long long* queue = reinterpret_cast<unsigned long long*>(a.queue) + 1;
+  const int64_t n_units = a.n_items / ng;
+  while (true) {
+    if (tid == 0) {
+      const unsigned long long u = atomicAdd(queue, 1ull) + (unsigned long long)unit0;
+      sh.unit_lo = (int)(u & 0x7fffffff);
+      sh.unit_hi = (int)(u >> 31);
+    }
+    __syncthreads();
+    const int64_t unit = ((int64_t)sh.unit_hi << 31) | sh.unit_lo;
+    if (unit >= n_units) break;  // uniform across the workgroup
+    const int64_t item0 = unit * ng;
+    const ItemRef itr = decode_item(a, item0);
+    const SurfDev S = a.surfs[itr.s];
+    const int V = S.V;
+    const float* x = a.maps + (int64_t)itr.p * a.map_stride + S.offset;
+    const bool ring = (S.flags & kSurfRing) != 0;
+    int Lg[2] = {0, 0};
+    float T0g[2] = {0.0f, 0.0f}, dTg[2] = {1.0f, 1.0f}, sgn[2] = {1.0f, -1.0f};
+    bool bad = false;
+    for (int g = 0; g < ng; ++g) {
+      const int32_t* rec = a.prep + (item0 + g) * kPrepWords;
+      const int st = rec[kPrepL];
+      Lg[g] = st > 0 ? st : 0;
+      if (Lg[g] > kLdsMaxLevels) {
+        bad = true;
+        Lg[g] = 0;
+      }
+      T0g[g] = __int_as_float(rec[kPrepT0]);
+      dTg[g] = __int_as_float(rec[kPrepDT]);
+      sgn[g] = decode_item(a, item0 + g).sign;
+      for (int l = tid; l < Lg[g]; l += kLdsThreads) {
+        sh.T[g][l] = __int_as_float(rec[kPrepT + l]);
+        sh.HH[g][l] = __int_as_float(rec[kPrepT + kMaxLevels + l]);
+        sh.cur[g][l] = 0;
+      }
+    }
+    const int Lmax = max(Lg[0], Lg[1]);
+    __syncthreads();
+    // activation levels: one histogram per image
+    for (int v = tid; v < V; v += kLdsThreads) {
+      const float xv = x[v];
+      for (int g = 0; g < ng; ++g) {
+        if (Lg[g] == 0) continue;
+        const int lev = level_of(sgn[g] * xv, sh.T[g], Lg[g], T0g[g], dTg[g]);
+        if (lev != kNever) {
+          atomicAdd(&sh.cur[g][lev], 1);
+          break;
+        }
+      }
+    }
+    __syncthreads();
+    if (tid < 64 * ng) {
+      const int g = tid >> 6, lane = tid & 63;
+      const int L = Lg[g];
+      const int per = (L + 63) / 64;
+      const int b = lane * per, e = min(L, b + per);
+      int sum = 0;
+      for (int l = b; l < e; ++l) sum += sh.cur[g][l];
+      const int incl = wave_incl_scan(sum);
+      int run = incl - sum;
+      for (int l = b; l < e; ++l) {
+        const int c = sh.cur[g][l];
+        sh.start[g][l] = run;
+        sh.cur[g][l] = run;
+        run += c;
+      }
+      if (lane == 63) {
+        sh.start[g][L] = incl;
+        sh.start[g][L + 1] = incl;
+      }
+    }
+    if (tid == 0) {
+      sh.base[0] = 0;
+      if (ng == 1) {
+        sh.start[1][0] = 0;
+        sh.start[1][1] = 0;
+      }
+    }
+    __syncthreads();
+    if (tid == 0) sh.base[1] = sh.start[0][Lg[0]];
+    __syncthreads();
+    for (int v = tid; v < V; v += kLdsThreads) {
+      const float xv = x[v];
+      int q = 0xffff;
+      for (int g = 0; g < ng; ++g) {
+        if (Lg[g] == 0) continue;
+        const int lev = level_of(sgn[g] * xv, sh.T[g], Lg[g], T0g[g], dTg[g]);
+        if (lev != kNever) {
+          q = sh.base[g] + atomicAdd(&sh.cur[g][lev], 1);
+          sh.order[q] = (uint16_t)v;
+          break;
+        }
+      }
+      sh.pos[v] = (uint16_t)q;
+    }
+    // the first level's elements (level 0 holds none: nothing exceeds the maximum)
+    for (int g = 0; g < ng; ++g)
+      for (int i = sh.base[g] + tid; i < sh.base[g] + (Lg[g] > 0 ? sh.start[g][1] : 0); i += kLdsThreads) {
+        sh.word[i] = (1u << 16) | (uint32_t)i;
+        sh.val[i] = 0.0f;
+      }
+    __syncthreads();
+
+    // candidates of this thread's first new element of the next level, gathered
+    // during the merge phase (they depend only on the positions; ring surfaces)
+    int pre[9];
+    bool have_pre = false;
+    auto gather_next = [&](int l) {
+      have_pre = false;
+      if (!ring || l >= Lmax) return;
+      const int n0 = l < Lg[0] ? sh.start[0][l + 1] - sh.start[0][l] : 0;
+      const int n1 = l < Lg[1] ? sh.start[1][l + 1] - sh.start[1][l] : 0;
+      if (tid < n0 + n1) {
+        const LdsNew e = lds_new(sh, tid, n0, l);
+        lds_ring_cands(sh, S, e.i, e.lo, e.b0, pre);
+        have_pre = true;
+      }
+    };
+    gather_next(0);
+    for (int l = 0; l < Lmax; ++l) {
+      const int n0 = l < Lg[0] ? sh.start[0][l + 1] - sh.start[0][l] : 0;
+      const int n1 = l < Lg[1] ? sh.start[1][l + 1] - sh.start[1][l] : 0;
+      // union
+      for (int k = tid; k < n0 + n1; k += kLdsThreads) {
+        const LdsNew e = lds_new(sh, k, n0, l);
+        if (ring) {
+          int r[9];
+          if (k == tid && have_pre) {
+#pragma unroll
+            for (int j = 0; j < 9; ++j) r[j] = pre[j];
+          } else {
+            lds_ring_cands(sh, S, e.i, e.lo, e.b0, r);
+          }
+          int ri = e.i;
+#pragma unroll
+          for (int j = 0; j < 8; ++j)
+            if (r[j] >= 0) ri = lds_link(sh.word, ri, r[j]);
+        } else {
+          lds_join_table(sh, S, e.i, e.lo, e.b0);
+        }
+      }
+      __syncthreads();
+      // merge the sets hooked at this level into their roots (images still sweeping)
+      const int m0 = l < Lg[0] ? sh.start[0][l + 1] : 0;
+      const int m1 = l < Lg[1] ? sh.start[1][l + 1] : 0;
+      const int b1base = sh.base[1];
+      for (int k = tid; k < m0 + m1; k += kLdsThreads) {
+        const int c = k < m0 ? k : b1base + (k - m0);
+        const uint32_t wc = lds_ld(sh.word + c);
+        if ((int)(wc & 0xffffu) == c || (wc >> 16) == 0u) continue;
+        const int r = lds_find<false>(sh.word, (int)(wc & 0xffffu));
+        __hip_atomic_fetch_add(sh.word + r, wc & 0xffff0000u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_max(reinterpret_cast<int32_t*>(sh.val) + r, __float_as_int(sh.val[c]), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+        lds_st(sh.word + c, (uint32_t)r);
+      }
+      gather_next(l + 1);
+      __syncthreads();
+      // increment every root of the images still sweeping; initialise the next level
+      for (int k = tid; k < m0 + m1; k += kLdsThreads) {
+        const int g = k < m0 ? 0 : 1;
+        const int c = g ? b1base + (k - m0) : k;
+        const uint32_t wc = sh.word[c];
+        if ((int)(wc & 0xffffu) == c)
+          sh.val[c] = sh.val[c] + tfce_inc((int)(wc >> 16), sh.HH[g][l], S.E, S.sizepow);
+      }
+      for (int g = 0; g < ng; ++g) {
+        if (l + 1 >= Lg[g]) continue;
+        const int lo = sh.base[g] + sh.start[g][l + 1], hi = sh.base[g] + sh.start[g][l + 2];
+        for (int i = lo + tid; i < hi; i += kLdsThreads) {
+          sh.word[i] = (1u << 16) | (uint32_t)i;
+          sh.val[i] = 0.0f;
+        }
+      }
+      __syncthreads();
+    }
+
+    // the largest value of each image (a root holds its set's largest)
+    for (int g = 0; g < ng; ++g) {
+      const int lo = sh.base[g], hi = lo + (Lg[g] > 0 ? sh.start[g][Lg[g]] : 0);
+      float best = 0.0f;
+      for (int c = lo + tid; c < hi; c += kLdsThreads) best = fmaxf(best, sh.val[c]);
+      best = wave_max(best);
+      if ((tid & 63) == 0) sh.red[g][tid >> 6] = best;
+    }
+    __syncthreads();
+    if (tid < ng) {
+      const int g = tid;
+      float best = sh.red[g][0];
+      for (int k = 1; k < kLdsThreads / 64; ++k) best = fmaxf(best, sh.red[g][k]);
+      const int64_t item = item0 + g;
+      const int32_t* rec = a.prep + item * kPrepWords;
+      const float own = __int_as_float(rec[kPrepOwn]);
+      const float smax = (S.flags & kSurfZeroInScale) ? fmaxf(own, 0.0f) : own;
+      const float sc = __fdiv_rn(smax, 100.0f);
+      const float raw = bad ? __int_as_float(0x7fc00000) : best;
+      a.raw_max[item] = raw;
+      if (a.scaled_max) a.scaled_max[item * a.out_mul + a.out_add] = __fmul_rn(raw, sc);
+      if (a.own_max) a.own_max[item] = own;
+      if (a.status) a.status[item] = bad ? kStatusTooManyLevels : rec[kPrepL];
+    }
+    __syncthreads();
+  }
+}
+
 // Shared first threshold per (map, sign): the maximum over every surface of the
 // set, as when the surfaces are TFCE'd as one merged image (tm_func.py:54-123,
 // fast_tfce.hpp:34-35 then sees the global maximum).  One workgroup per
@@ -1121,17 +1518,35 @@ __global__ void fill_f32_kernel(float* p, int64_t n, float v) {
     p[i] = v;
 }
 
-__global__ void zero_queue_kernel(int32_t* q) { q[0] = 0; q[1] = 0; }
+__global__ void zero_queue_kernel(int32_t* q) {
+  q[0] = 0;
+  q[1] = 0;
+  q[2] = 0;
+  q[3] = 0;
+}
 
 }  // namespace
 
-hipError_t launch_tfce_batch(const TfceBatchArgs& a, bool full, int grid, hipStream_t stream) {
+hipError_t launch_tfce_batch(const TfceBatchArgs& a, bool full, bool ring_only, int grid, int64_t n_global,
+                             hipStream_t stream) {
   hipLaunchKernelGGL(zero_queue_kernel, dim3(1), dim3(1), 0, stream, a.queue);
   hipLaunchKernelGGL(tfce_prepare_kernel, dim3((unsigned)a.n_items), dim3(kTfceThreads), 0, stream, a);
-  if (full)
-    hipLaunchKernelGGL(tfce_batch_kernel<true>, dim3(grid), dim3(kTfceThreads), 0, stream, a);
+  if (n_global <= 0) return hipGetLastError();
+  TfceBatchArgs g = a;
+  g.n_items = n_global;
+  if (full && ring_only)
+    hipLaunchKernelGGL((tfce_batch_kernel<true, true>), dim3(grid), dim3(kTfceThreads), 0, stream, g);
+  else if (full)
+    hipLaunchKernelGGL((tfce_batch_kernel<true, false>), dim3(grid), dim3(kTfceThreads), 0, stream, g);
+  else if (ring_only)
+    hipLaunchKernelGGL((tfce_batch_kernel<false, true>), dim3(grid), dim3(kTfceThreads), 0, stream, g);
   else
-    hipLaunchKernelGGL(tfce_batch_kernel<false>, dim3(grid), dim3(kTfceThreads), 0, stream, a);
+    hipLaunchKernelGGL((tfce_batch_kernel<false, false>), dim3(grid), dim3(kTfceThreads), 0, stream, g);
+  return hipGetLastError();
+}
+
+hipError_t launch_tfce_lds(const TfceBatchArgs& a, int64_t item0, int grid, hipStream_t stream) {
+  hipLaunchKernelGGL(tfce_lds_kernel, dim3(grid), dim3(kLdsThreads), 0, stream, a, item0 / a.n_signs);
   return hipGetLastError();
 }
 

@@ -66,6 +66,24 @@ class PermutationEngine:
         map column cols[j] when a locality layout is in use)."""
         return self.ols.t(perms)[0]
 
+    def rebind(self, Y, X, contrast=None, perm_columns=None):
+        """New data Y [n, width] (map column order) and design X on the same
+        surfaces: the adjacencies, their tables and the locality layout are
+        kept, only the prepared data and the design model are rebuilt."""
+        import torch
+        X = np.asarray(X, dtype=np.float64)
+        if self.cols is not None:
+            if isinstance(Y, torch.Tensor):
+                Y = Y.index_select(1, torch.as_tensor(self.cols, device=Y.device))
+            else:
+                Y = np.asarray(Y)[:, self.cols]
+        self.n, self.k = X.shape
+        self.contrast = int(self.k - 1 if contrast is None else contrast)
+        precision = self.ols.precision
+        self.ols = None
+        self.ols = PermutedOLS(Y, X, contrasts=[self.contrast], perm_columns=perm_columns, precision=precision)
+        assert self.ols.V == self.V
+
     def null_batch(self, perms, per_surface=False):
         """[P, 2] maxima (max over surfaces) of +t and -t, or [n_surf, P, 2]."""
         t = self.tvalues(perms)

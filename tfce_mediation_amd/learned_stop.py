@@ -98,7 +98,14 @@ class TailMLP:
                                                _lib.stream_ptr()))
         return Y.cpu().numpy()
 
-    def train(self, X, Y, steps=3000, batch=128, lr=2e-3, weight_decay=1e-5, seed=0, want_grad0=False):
+    def train(self, X, Y, steps=3000, batch=128, lr=2e-3, weight_decay=1e-5, seed=0, want_grad0=False,
+              chunk=500, log=None):
+        """``steps`` Adam steps on minibatches drawn from (X, Y) with ``seed``,
+        in launches of ``chunk`` steps (the kernel continues the Adam step
+        count, so the parameters equal those of one launch bit for bit);
+        ``log`` receives a progress line per launch.  Returns the per-step
+        losses (and the first step's gradient and the minibatch indices with
+        ``want_grad0``)."""
         import torch
         rng = np.random.default_rng(seed)
         Xd = torch.as_tensor(np.ascontiguousarray(X, dtype=np.float32)).cuda()
@@ -109,11 +116,18 @@ class TailMLP:
         idx_d = torch.as_tensor(idx).cuda()
         loss = torch.empty(steps, dtype=torch.float32, device="cuda")
         g0 = torch.empty(self.n_params, dtype=torch.float32, device="cuda") if want_grad0 else None
-        _lib.check(_lib.lib().tfce_mlp_train(_lib.ptr(self.params), _lib.ptr(self.m), _lib.ptr(self.v),
-                                             _lib.ptr(Xd), _lib.ptr(Yd), n, _lib.ptr(idx_d), steps, b, self.t,
-                                             float(lr), float(weight_decay), _lib.ptr(loss), _lib.ptr(g0),
-                                             _lib.stream_ptr()))
-        self.t += steps
+        lib = _lib.lib()
+        chunk = max(1, int(chunk))
+        for a in range(0, steps, chunk):
+            c = min(chunk, steps - a)
+            _lib.check(lib.tfce_mlp_train(_lib.ptr(self.params), _lib.ptr(self.m), _lib.ptr(self.v), _lib.ptr(Xd),
+                                          _lib.ptr(Yd), n, _lib.ptr(idx_d[a:]), c, b, self.t, float(lr),
+                                          float(weight_decay), _lib.ptr(loss[a:]),
+                                          _lib.ptr(g0) if a == 0 else None, _lib.stream_ptr()))
+            self.t += c
+            if log is not None:
+                torch.cuda.synchronize()
+                log(f"[train] {a + c}/{steps} Adam steps, minibatch loss {float(loss[a + c - 1]):.5f}")
         out = loss.cpu().numpy()
         return (out, g0.cpu().numpy(), idx) if want_grad0 else out
 
@@ -194,6 +208,34 @@ def generate_nulls(make_config, seeds, n_values, batch=256, precision="fp16x3"):
         null = eng.run(n_perm, batch=batch, seed=int(s) * 7919).ravel()[:n_values]
         out.append(null)
         del eng
+    return out
+
+
+def family_nulls(config, seeds, n_perm, batch=512, precision="fp16x3", log=None):
+    """Long max-TFCE nulls (pooled +/- values in permutation order, 2 * n_perm
+    each) of one synthetic workload family, one per seed: ``config(seed)``
+    returns a synth config dict whose surfaces do not depend on the seed, so
+    one engine serves every seed (``PermutationEngine.rebind``)."""
+    import time
+    from .engine import PermutationEngine
+    eng = None
+    out = []
+    for j, s in enumerate(seeds):
+        t0 = time.time()
+        cfg = config(int(s))
+        if eng is None:
+            eng = PermutationEngine(cfg["Y"], cfg["X"], cfg["surfaces"], cfg["offsets"], cfg.get("H", 2.0),
+                                    cfg.get("E", 1.0), contrast=cfg["contrast"], precision=precision,
+                                    settings=cfg.get("settings"))
+        else:
+            eng.rebind(cfg["Y"], cfg["X"], contrast=cfg["contrast"])
+        t1 = time.time()
+        out.append(eng.run(n_perm, batch=batch, seed=int(s) * 7919).ravel().astype(np.float64))
+        if log is not None:
+            log(f"[study] {cfg.get('name', 'workload')} seed {int(s)} ({j + 1}/{len(seeds)}): data {t1 - t0:.1f}s, "
+                f"{n_perm} permutations {time.time() - t1:.1f}s")
+        del cfg
+    del eng
     return out
 
 
@@ -307,30 +349,38 @@ def c5_workload(seed, precision="fp16x3"):
                         precision=precision)
 
 
-def study(n_train=160, n_train_perm=10000, n_eval=6, n_eval_perm=10000, m0=500, delta=0.0075, steps=6000, seed=0,
-          log=print, streams=None):
-    """Train on random small workloads, then evaluate on the full-size
-    two-hemisphere GLM and mediation workloads against 10,000-permutation
-    plain runs.  Returns a dict of per-null results."""
+def study(n_train=96, n_train_perm=5000, n_eval=6, n_eval_perm=10000, m0=500, delta=0.0075, steps=6000, seed=0,
+          n_random=0, log=print, streams=None):
+    """Train the tail model on nulls of the voxel GLM (91 x 109 x 91 grid,
+    ~232k voxels, n = 200) and the two-hemisphere GLM (327,684 vertices,
+    n = 500) workloads with random seeds - half each, ``n_train_perm``
+    permutations per null (2 values each) - plus ``n_random`` random small
+    workloads; then evaluate on held-out seeds of the two-hemisphere GLM and
+    of the mediation workload (327,684 vertices, n = 300) against their own
+    ``n_eval_perm``-permutation plain runs.  Returns a dict of per-null
+    results."""
     import time
+    from . import synth
     rng = np.random.default_rng(seed)
-    t0 = time.time()
     train = []
-    for i in range(n_train):
-        ts = time.time()
+    half = n_train // 2
+    seeds_c2 = [int(x) for x in rng.integers(1, 1 << 30, half)]
+    seeds_c3 = [int(x) for x in rng.integers(1, 1 << 30, n_train - half)]
+    train += family_nulls(lambda s: synth.config2(seed=s, device="cuda"), seeds_c2, n_train_perm, log=log)
+    train += family_nulls(lambda s: synth.config3(seed=s, device="cuda"), seeds_c3, n_train_perm, batch=256,
+                          log=log)
+    for i in range(n_random):
         wl = random_workload(rng, log=log)
-        tb = time.time()
-        vals = wl.values(n_train_perm, seed=int(rng.integers(1 << 30)), log=log)
-        train.append(vals)
-        log(f"[study] training null {i + 1}/{n_train}: {wl.kind}, set-up {tb - ts:.1f}s, "
-            f"{n_train_perm} permutations {time.time() - tb:.1f}s")
+        train.append(wl.values(n_train_perm, seed=int(rng.integers(1 << 30)), log=log))
+        log(f"[study] random training null {i + 1}/{n_random}: {wl.kind}")
         del wl
     X, Y = make_examples(train, m0=m0, per_null=128, seed=seed + 1)
     model = TailMLP(seed=seed + 2)
-    losses = model.train(X, Y, steps=steps, lr=2e-3)
+    losses = model.train(X, Y, steps=steps, lr=2e-3, chunk=500, log=log)
     log(f"[study] trained on {len(X)} examples: loss {losses[:50].mean():.4f} -> {losses[-200:].mean():.4f}")
     rule = LearnedStop(model, m0=m0, n_max=None, delta=delta)
-    out = {"train_nulls": n_train, "train_perm": n_train_perm, "m0": m0, "delta": delta, "eval": [],
+    out = {"train_nulls": len(train), "train_perm": n_train_perm, "train_seeds": {"c2": seeds_c2, "c3": seeds_c3},
+           "m0": m0, "delta": delta, "steps": steps, "eval": [],
            "rule": dict(m0=m0, delta=delta, m_min=rule.m_min, consecutive=rule.consecutive,
                         check_every=rule.check_every)}
     kept = {}
@@ -365,21 +415,52 @@ def study(n_train=160, n_train_perm=10000, n_eval=6, n_eval_perm=10000, m0=500, 
     return out
 
 
+def summary(res):
+    """Markdown summary of a ``study`` result."""
+    ev = res["eval"]
+    lines = ["| workload | null | perms used | source | rel. err. of reported 95% / 99% | pred@500 err 95% / 99% | "
+             "empirical@500 err 95% / 99% |", "|---|---|---|---|---|---|---|"]
+    for r in ev:
+        full = r["values"] // (2 if r["workload"].startswith("c3") else 1)
+        lines.append(f"| {r['workload']} | {r['seed']} | {r['stop_perms']} / {full} | {r['stop_source']} | "
+                     f"{r['stop_rel_err'][0]:.4f} / {r['stop_rel_err'][1]:.4f} | "
+                     f"{r['rel_err_pred'][0]:.4f} / {r['rel_err_pred'][1]:.4f} | "
+                     f"{r['rel_err_emp'][0]:.4f} / {r['rel_err_emp'][1]:.4f} |")
+    pe = np.array([r["rel_err_pred"] for r in ev])
+    ee = np.array([r["rel_err_emp"] for r in ev])
+    se = np.array([r["stop_rel_err"] for r in ev])
+    frac = np.mean([r["stop_perms"] / (r["values"] // (2 if r["workload"].startswith("c3") else 1)) for r in ev])
+    lines += ["", f"- mean rel. error of the prediction from {res['m0']} values: 95% {pe[:, 0].mean():.4f}, "
+                  f"99% {pe[:, 1].mean():.4f}",
+              f"- mean rel. error of the empirical quantile of the same values: 95% {ee[:, 0].mean():.4f}, "
+              f"99% {ee[:, 1].mean():.4f}",
+              f"- max rel. error of the reported threshold: 95% {se[:, 0].max():.4f}, 99% {se[:, 1].max():.4f}",
+              f"- stopped early on {sum(r['stop_source'] == 'learned' for r in ev)} of {len(ev)} nulls; "
+              f"permutations used: {100 * frac:.1f}% of the plain runs"]
+    return "\n".join(lines)
+
+
 if __name__ == "__main__":
     import argparse
     import json
     ap = argparse.ArgumentParser(description="learned-stop study on synthetic workloads")
-    ap.add_argument("--train", type=int, default=160)
-    ap.add_argument("--train-perm", type=int, default=10000)
+    ap.add_argument("--train", type=int, default=96)
+    ap.add_argument("--train-perm", type=int, default=5000)
+    ap.add_argument("--random", type=int, default=0, help="extra random small training workloads")
     ap.add_argument("--eval", type=int, default=6)
     ap.add_argument("--eval-perm", type=int, default=10000)
     ap.add_argument("--delta", type=float, default=0.0075)
     ap.add_argument("--steps", type=int, default=6000)
     ap.add_argument("--out", default=None)
+    ap.add_argument("--summary", default=None, help="markdown summary file")
     ap.add_argument("--streams", default=None, help="npz file for the evaluation null streams")
     args = ap.parse_args()
     res = study(args.train, args.train_perm, args.eval, args.eval_perm, delta=args.delta, steps=args.steps,
-                log=lambda *a: print(*a, flush=True), streams=args.streams)
+                n_random=args.random, log=lambda *a: print(*a, flush=True), streams=args.streams)
     if args.out:
         with open(args.out, "w") as f:
             json.dump(res, f, indent=1)
+    if args.summary:
+        with open(args.summary, "w") as f:
+            f.write(summary(res) + "\n")
+    print(summary(res), flush=True)

@@ -179,6 +179,17 @@ int tfce_ols_info(const tfce_ols_model_t *model, int32_t *k, int32_t *n_fixed, i
 int tfce_ols_perm_t(const tfce_ols_model_t *model, const int32_t *d_perm, int32_t P, float *d_t,
                     int64_t ldt, hipStream_t stream);
 
+/* Replaces calc_sobelz (pyfunc.py:130-162) of mediation types M and I for a
+ * batch of permutations of x (vertex_tfce_mediation_randomise.py:82-91):
+ * path_a (y_img ~ [1, x]) and path_b (y_img ~ [1, y, x] for type M, [1, x, y]
+ * for type I) are models on one data handle that permute the same single
+ * column (x), each with one contrast (x for path A; y for type M, x for type
+ * I).  X_p'Y is contracted once for both; the epilogue computes both t-values
+ * as tfce_ols_perm_t would and writes d_z[p * ldz + v] = the Sobel-type z of
+ * alg (0 aroian, 1 sobel, 2 goodman) as tfce_sobel_z would, bit for bit. */
+int tfce_ols_perm_sobel(const tfce_ols_model_t *path_a, const tfce_ols_model_t *path_b, const int32_t *d_perm,
+                        int32_t P, int32_t alg, float *d_z, int64_t ldz, hipStream_t stream);
+
 /* Replaces the ancova branch of the voxel randomise loop
  * (voxel_tfce_multiple_regression_randomise.py:79-89: calcF of X[perm],
  * negative F set to 0, square root; calcF is cynumstats.pyx:31-36):

@@ -280,3 +280,27 @@ def test_one_call_entry_points_match_the_handle_path():
             np.testing.assert_allclose(z.cpu().numpy(), ref, rtol=1e-6, atol=1e-7)
         else:
             np.testing.assert_array_equal(z.cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp16x3", "bf16x3"])
+def test_fused_mediation_contraction_matches_two_contractions(torch, prec):
+    """Types M and I: one contraction of the permuted x serving both paths
+    (tfce_ols_perm_sobel) gives the z of the two separate contractions and
+    tfce_sobel_z, bit for bit, for every Sobel variant."""
+    from tfce_mediation_amd import randomise, synth
+    say(f"[gpu] fused mediation contraction {prec}")
+    rng = np.random.default_rng(21)
+    n, V, P = 96, 4100, 37
+    x = rng.standard_normal(n)
+    y = rng.standard_normal(n)
+    M = torch.as_tensor((0.4 * x[:, None] + rng.standard_normal((n, V))).astype(np.float32)).cuda()
+    perms = synth.permutations(n, P, 6, "cuda")
+    for mt in ("M", "I"):
+        for alg in ("aroian", "sobel", "goodman"):
+            fused = randomise.MediationModel(M, x, y, mt, alg=alg, precision=prec)
+            sep = randomise.MediationModel(M, x, y, mt, alg=alg, precision=prec, fused=False)
+            assert fused.fused and not sep.fused
+            a = fused.z(perms).cpu().numpy()
+            b = sep.z(perms).cpu().numpy()
+            np.testing.assert_array_equal(a, b)
+            assert np.isfinite(a).mean() > 0.99

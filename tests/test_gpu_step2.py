@@ -140,3 +140,44 @@ def test_vertex_mediation_from_step1_directory(tmp_path):
     np.testing.assert_array_equal(got, ref)
     assert os.path.exists(os.path.join(tmp_path, "output_med_area", "perm_SobelZ_I",
                                        "perm_Zstat_I_TFCE_maxVertex.csv"))
+
+
+def test_mmr_from_a_tmi_file(tmp_path):
+    """A TMI file (masked mesh + volume mask, faces, data) through
+    tm_io.mmr_inputs into randomise.mmr (merged thresholds: the masked
+    vertices' zeros enter the merged image's maximum) against the oracle on
+    the reference's merged full-mask image (tm_func.py:54-98)."""
+    from tfce_mediation_amd import randomise, tm_io
+    say("[gpu] mmr from a TMI file")
+    rng = np.random.default_rng(8)
+    verts, faces = synth.icosphere(3)
+    keep = rng.random(len(verts)) < 0.8
+    vol = synth.ellipsoid_mask((10, 11, 9), (4.5, 5, 4), (4.2, 4.6, 3.8))
+    nl, nv = int(keep.sum()), int(vol.sum())
+    n = 30
+    data = rng.standard_normal((nl + nv, n)).astype(np.float32)
+    name = tm_io.write_tm_filetype(str(tmp_path / "mmr"), image_array=data, masking_array=[keep[:, None, None], vol],
+                                   vertex_array=[verts], face_array=[faces])
+    merge_y, surfs, offs, vd, zero = tm_io.mmr_inputs(tm_io.read_tm_filetype(name, verbose=False))
+    X = np.column_stack([np.ones(n), rng.standard_normal(n), (rng.random(n) < 0.5).astype(float)])
+    perms = randomise.permutation_indices(n, range(1, 5), seed=4)
+    res = randomise.mmr(merge_y, X[:, 1:], surfs, offs, [(2.0, 1.0)] * 2, perms, merged=True, zero_in_image={0: True},
+                        vdensity=vd, precision="fp32")
+    # the reference's merged image: full mesh (zeros where masked) then the volume
+    rp_full, col_full = adjacency.faces_to_csr(faces, len(verts))
+    mrp, mcol = adjacency.merge_csr([(rp_full, col_full), adjacency.voxel_csr(vol, 26)])
+    fullmask = np.concatenate([keep, np.ones(nv, bool)])
+    pos = [0, nl, nl + nv]
+    for c in (1, 2):
+        for p, perm in enumerate(perms):
+            t = ools.perm_t(X, merge_y, [perm], [c])[0, 0].astype(np.float32)
+            for k, sg in enumerate((1, -1)):
+                img = np.zeros(len(fullmask), np.float32)
+                img[fullmask] = sg * t
+                enh, _, _ = otfce.tfce(mrp, mcol, img, 2.0, 1.0, exact_pow=True)
+                ein, tin = enh[fullmask], img[fullmask]
+                for s in range(2):
+                    a, b = pos[s], pos[s + 1]
+                    exp = (ein[a:b] * (tin[a:b].max() / np.float32(100)) * vd[a:b]).max()
+                    got = res[(s, c)][p, k]
+                    assert abs(got - exp) <= 1e-4 * abs(exp) + 1e-6, (c, p, k, s, got, exp)

@@ -45,6 +45,7 @@ SIGNATURES = {
     "tfce_ols_info": (c_int, [c_p, c_p, c_p, c_p]),
     "tfce_ols_perm_t": (c_int, [c_p, c_p, c_i32, c_p, c_i64, c_p]),
     "tfce_ols_perm_f": (c_int, [c_p, c_p, c_i32, c_p, c_p, c_i64, c_p]),
+    "tfce_ols_perm_sobel": (c_int, [c_p, c_p, c_p, c_i32, c_i32, c_p, c_i64, c_p]),
     "tfce_ols_set_tile": (c_int, [c_i32]),
     "tfce_ols_tile": (c_int, []),
     "tfce_fwe_p": (c_int, [c_p, c_i32, c_p, c_i64, c_i32, c_p, c_p]),

@@ -45,6 +45,7 @@ struct tfce_ols_model {
   float* Linv1 = nullptr;  // shared factors
   double* dc1 = nullptr;
   float zscale[kOlsMaxK] = {1, 1, 1, 1, 1, 1, 1, 1};  // fp16 mode: per permuted column
+  std::vector<double> hZ;  // [n][m] permuted columns (host copy: pairing checks)
 };
 
 namespace {
@@ -316,6 +317,7 @@ int tfce_ols_model_create(const tfce_ols_data_t* data, const double* X, int32_t 
   bool shared = true;
   for (int q = 0; q < f; ++q) shared = shared && constval[order[q]] != 0.0;
   md->shared = shared;
+  md->hZ = Z;
   int rc = 0;
   const int64_t V = data->V;
   if ((rc = dalloc(&md->F, F.size(), "F")) || (rc = dalloc(&md->Z, Z.size(), "Z")) ||
@@ -404,7 +406,7 @@ int tfce_ols_tile(void) { return g_gemm_tile; }
 
 // t-values (tss == null) or sqrt(max(F, 0)) for P permutations
 static int perm_run(const tfce_ols_model_t* md, const int32_t* d_perm, int32_t P, float* d_t, int64_t ldt,
-                    const double* tss, hipStream_t stream) {
+                    const double* tss, hipStream_t stream, const tfce_ols_model_t* md2 = nullptr, int alg = 0) {
   if (!md || !d_t) return set_error(TFCE_EINVAL, "NULL argument");
   const tfce_ols_data* d = md->data;
   if (!d_perm) P = 1;
@@ -422,7 +424,8 @@ static int perm_run(const tfce_ols_model_t* md, const int32_t* d_perm, int32_t P
   const int64_t Mpad = n_mtiles * tile_rows;
   const int pieces = d->pieces;
   const size_t abytes = (size_t)d->Kp * Mpad * (pieces ? 2 * pieces : 4);
-  const size_t fbytes = md->shared ? 0 : (size_t)P * (64 * sizeof(float) + kOlsMaxContrasts * sizeof(double));
+  const size_t fbytes1 = (size_t)P * (64 * sizeof(float) + kOlsMaxContrasts * sizeof(double));
+  const size_t fbytes = (md->shared ? 0 : fbytes1) + ((md2 && !md2->shared) ? fbytes1 + 256 : 0);
   void* ws = nullptr;
   int rc = ols_ws(stream, abytes + fbytes + 512, &ws);
   if (rc) return rc;
@@ -448,31 +451,39 @@ static int perm_run(const tfce_ols_model_t* md, const int32_t* d_perm, int32_t P
   }
   hipError_t e = launch_ols_build_a(ba, stream);
   if (e != hipSuccess) return set_error(-(int)e, "ols build-A launch");
-  const float* Linv = md->Linv1;
-  const double* dc = md->dc1;
-  if (!md->shared) {
-    char* fb = base + ((abytes + 255) / 256 * 256);
+  // per-permutation Gram factors of a model whose fixed columns are not all
+  // constant, in the workspace after A
+  char* fb = base + ((abytes + 255) / 256 * 256);
+  auto factors = [&](const tfce_ols_model_t* mm, const float** Linv, const double** dc) -> int {
+    *Linv = mm->Linv1;
+    *dc = mm->dc1;
+    if (mm->shared) return 0;
     float* Lw = reinterpret_cast<float*>(fb);
     double* dw = reinterpret_cast<double*>(fb + (size_t)P * 64 * sizeof(float));
+    fb += (fbytes1 + 255) / 256 * 256;
     OlsFactorArgs ka{};
-    ka.F = md->F;
-    ka.Z = md->Z;
-    ka.FF = md->FF;
-    ka.ZZ = md->ZZ;
+    ka.F = mm->F;
+    ka.Z = mm->Z;
+    ka.FF = mm->FF;
+    ka.ZZ = mm->ZZ;
     ka.perm = d_perm;
     ka.n = d->n;
-    ka.f = md->f;
-    ka.m = md->m;
+    ka.f = mm->f;
+    ka.m = mm->m;
     ka.P = P;
-    ka.n_con = md->n_con;
-    for (int c = 0; c < kOlsMaxContrasts; ++c) ka.con[c] = md->con[c];
+    ka.n_con = mm->n_con;
+    for (int c = 0; c < kOlsMaxContrasts; ++c) ka.con[c] = mm->con[c];
     ka.Linv = Lw;
     ka.dc = dw;
-    e = launch_ols_factors(ka, stream);
-    if (e != hipSuccess) return set_error(-(int)e, "ols factor launch");
-    Linv = Lw;
-    dc = dw;
-  }
+    hipError_t ef = launch_ols_factors(ka, stream);
+    if (ef != hipSuccess) return set_error(-(int)ef, "ols factor launch");
+    *Linv = Lw;
+    *dc = dw;
+    return 0;
+  };
+  const float* Linv = nullptr;
+  const double* dc = nullptr;
+  if ((rc = factors(md, &Linv, &dc))) return rc;
   OlsGemmArgs ga{};
   ga.a32 = ba.a32;
   for (int p = 0; p < 3; ++p) {
@@ -512,6 +523,23 @@ static int perm_run(const tfce_ols_model_t* md, const int32_t* d_perm, int32_t P
   ga.n_mtiles = (int32_t)n_mtiles;
   ga.n_ntiles = (int32_t)(d->Npad / tile_cols);
   ga.tss = tss;
+  if (md2) {
+    const float* Linv2 = nullptr;
+    const double* dc2 = nullptr;
+    if ((rc = factors(md2, &Linv2, &dc2))) return rc;
+    ga.pair = 1;
+    ga.alg = alg;
+    ga.f2 = md2->f;
+    ga.k2 = md2->k;
+    ga.con2 = md2->con[0];
+    ga.con2_is_intercept = md2->con_is_intercept[0];
+    ga.shared2 = md2->shared ? 1 : 0;
+    ga.Linv2 = Linv2;
+    ga.dc2 = dc2;
+    ga.gF2 = md2->gF;
+    ga.hF2 = md2->hF;
+    ga.r02 = md2->r0;
+  }
   e = launch_ols_gemm(ga, d->precision, tile, stream);
   if (e != hipSuccess) return set_error(-(int)e, "ols gemm launch");
   return 0;
@@ -520,6 +548,19 @@ static int perm_run(const tfce_ols_model_t* md, const int32_t* d_perm, int32_t P
 int tfce_ols_perm_t(const tfce_ols_model_t* md, const int32_t* d_perm, int32_t P, float* d_t, int64_t ldt,
                     hipStream_t stream) {
   return perm_run(md, d_perm, P, d_t, ldt, nullptr, stream);
+}
+
+int tfce_ols_perm_sobel(const tfce_ols_model_t* path_a, const tfce_ols_model_t* path_b, const int32_t* d_perm,
+                        int32_t P, int32_t alg, float* d_z, int64_t ldz, hipStream_t stream) {
+  if (!path_a || !path_b || !d_z) return set_error(TFCE_EINVAL, "NULL argument");
+  if (alg < 0 || alg > 2) return set_error(TFCE_EINVAL, "alg must be 0 (aroian), 1 (sobel) or 2 (goodman)");
+  if (path_a->data != path_b->data) return set_error(TFCE_EINVAL, "the two models must share one data handle");
+  if (path_a->m != 1 || path_b->m != 1)
+    return set_error(TFCE_EINVAL, "each model must permute exactly one column");
+  if (path_a->n_con != 1 || path_b->n_con != 1) return set_error(TFCE_EINVAL, "each model must have one contrast");
+  if (path_a->hZ != path_b->hZ)
+    return set_error(TFCE_EINVAL, "the two models must permute the same column values");
+  return perm_run(path_a, d_perm, P, d_z, ldz, nullptr, stream, path_b, alg);
 }
 
 int tfce_ols_perm_f(const tfce_ols_model_t* md, const int32_t* d_perm, int32_t P, const double* d_tss,

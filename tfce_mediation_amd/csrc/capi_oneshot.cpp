@@ -110,23 +110,21 @@ int sobel_perm_z(const float* d_M, int32_t n, int64_t V, const double* x, const 
   rc = tfce_ols_model_create(h.data, XB.data(), 3, mask_b, &con, 1, &h.b, stream);
   if (rc) return rc;
   hipError_t e;
-  if ((e = hipMalloc(&h.tmp, sizeof(float) * (size_t)P * (size_t)V * (medtype == 2 ? 1 : 2))) != hipSuccess)
-    return set_error(-(int)e, "hipMalloc(t)");
-  float* tb = h.tmp;
-  float* ta = medtype == 2 ? nullptr : h.tmp + (size_t)P * (size_t)V;
-  rc = tfce_ols_perm_t(h.b, d_perm, P, tb, V, stream);
-  if (rc) return rc;
-  float ta_const = 0.0f;
   if (medtype == 2) {
-    ta_const = (float)scalar_t(x, y, n);
+    // type Y: path A is the scalar x -> y regression
+    if ((e = hipMalloc(&h.tmp, sizeof(float) * (size_t)P * (size_t)V)) != hipSuccess)
+      return set_error(-(int)e, "hipMalloc(t)");
+    rc = tfce_ols_perm_t(h.b, d_perm, P, h.tmp, V, stream);
+    if (rc) return rc;
+    rc = tfce_sobel_z(nullptr, h.tmp, (int64_t)P * V, alg, (float)scalar_t(x, y, n), d_z, stream);
+    if (rc) return rc;
   } else {
+    // types M and I: both paths permute x only, one contraction serves both
     rc = tfce_ols_model_create(h.data, XA.data(), 2, 1u << 1, &con, 1, &h.a, stream);
     if (rc) return rc;
-    rc = tfce_ols_perm_t(h.a, d_perm, P, ta, V, stream);
+    rc = tfce_ols_perm_sobel(h.a, h.b, d_perm, P, alg, d_z, V, stream);
     if (rc) return rc;
   }
-  rc = tfce_sobel_z(ta, tb, (int64_t)P * V, alg, ta_const, d_z, stream);
-  if (rc) return rc;
   if ((e = hipStreamSynchronize(stream)) != hipSuccess) return set_error(-(int)e, "sobel_perm_z");
   return 0;
 }

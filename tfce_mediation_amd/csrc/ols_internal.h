@@ -104,6 +104,17 @@ struct OlsGemmArgs {
   const float* yinv;     // [V] per-element inverse data scale, or null
   float zinv[8];         // per permuted column inverse design scale (fp16 mode)
   const double* tss;     // [V] total sum of squares: write sqrt(max(F, 0)) to t[p][col]; null: t-values
+  // Paired mediation epilogue (pair != 0): a second model on the same data with
+  // the same single permuted column (its fixed columns differ); the t of
+  // contrast 0 of each model combine into the Sobel-type z of alg (0 aroian,
+  // 1 sobel, 2 goodman), written to t[p][col].
+  int32_t pair, alg;
+  int32_t f2, k2, con2, con2_is_intercept, shared2;
+  const float* Linv2;    // [P or 1][8][8]
+  const double* dc2;     // [P or 1][kOlsMaxContrasts]
+  const float* gF2;      // [f2][V]
+  const float* hF2;      // [f2][V]
+  const double* r02;     // [V]
 };
 
 constexpr int kOlsDirectMaxN = 2048;  // subjects held in LDS by the FP64 kernel

@@ -323,6 +323,48 @@ __device__ __forceinline__ void ols_epilogue(const OlsGemmArgs& a, int p, int64_
   }
 }
 
+// Sobel-type z of the indirect effect from the two path t-values
+// (pyfunc.py:150-161, calc_indirect :2678-2709): 0 aroian, 1 sobel, 2 goodman.
+__device__ __forceinline__ float sobel_z(double a, double b, int alg) {
+  const double ia = 1.0 / (a * a), ib = 1.0 / (b * b), iab = 1.0 / (a * a * b * b);
+  double d;
+  if (alg == 0) d = ib + ia + iab;
+  else if (alg == 1) d = ib + ia;
+  else d = ib + ia - iab;
+  return (float)(1.0 / sqrt(d));
+}
+
+// t of contrast column ci of a model with f fixed columns (FF > 0: a
+// compile-time f) and one permuted column whose X_p'y is gz0: the arithmetic
+// of ols_epilogue for m = 1, operation for operation.
+template <int FF>
+__device__ __forceinline__ float single_t(int f_rt, int k, int n, const float* L, double dcv, int ci, int is_icpt,
+                                          int centered, const float* gF, const float* hF, double r0, double mu,
+                                          float gz0) {
+  const int f = FF > 0 ? FF : f_rt;
+  float s = 0.0f;
+#pragma unroll
+  for (int j = 0; j < kOlsMaxK; ++j)
+    if (j < f) s = fmaf(L[f * 8 + j], gF[j], s);
+  s = fmaf(L[f * 8 + f], gz0, s);
+  const double ssz = 0.0 + (double)s * (double)s;
+  double sse = r0 - ssz;
+  if (sse < 0.0) sse = 0.0;
+  const double s2 = sse / (double)(n - k);
+  float ac = 0.0f;
+#pragma unroll
+  for (int j = 0; j < kOlsMaxK; ++j) {
+    if (j >= ci && j < k) {
+      const float hj = (j < f) ? hF[j] : s;
+      ac = fmaf(L[j * 8 + ci], hj, ac);
+    }
+  }
+  double av = (double)ac;
+  if (centered && is_icpt) av += mu;
+  const float se = (float)sqrt(s2 * dcv);
+  return (float)(av / (double)se);
+}
+
 // Epilogue through LDS.  The accumulators of a wave cover, for tile (rt, ct),
 // rows row0 + 32 rt + 8 (i / 4) + 4 h + (i % 4) (i = accumulator index 0..15)
 // of column 32 ct + r.  The workgroup writes one 32-column slice of every
@@ -367,6 +409,36 @@ __device__ __forceinline__ void epi_loop(const OlsGemmArgs& a, const float* ep, 
   }
 }
 
+// The paired mediation epilogue: thread (cl, slot) walks its permutations,
+// computes t of both models from the one permuted column's X_p'y, and writes z.
+template <int LDC, int F1, int F2>
+__device__ __forceinline__ void epi_loop_pair(const OlsGemmArgs& a, const float* ep, int cl, int64_t col, int p0,
+                                              int ppt, int slot, int step, double mu, float yinv) {
+  float gF1[kOlsMaxK], hF1[kOlsMaxK], gF2[kOlsMaxK], hF2[kOlsMaxK];
+#pragma unroll
+  for (int q = 0; q < kOlsMaxK; ++q) {
+    gF1[q] = q < a.f ? a.gF[(int64_t)q * a.V + col] : 0.0f;
+    hF1[q] = q < a.f ? a.hF[(int64_t)q * a.V + col] : 0.0f;
+    gF2[q] = q < a.f2 ? a.gF2[(int64_t)q * a.V + col] : 0.0f;
+    hF2[q] = q < a.f2 ? a.hF2[(int64_t)q * a.V + col] : 0.0f;
+  }
+  const double r01 = a.r0[col], r02 = a.r02[col];
+  for (int pp = slot; pp < ppt; pp += step) {
+    const int p = p0 + pp;
+    if (p >= a.P) break;
+    float gz0 = ep[pp * LDC + cl];
+    if (a.yinv) gz0 = gz0 * a.zinv[0] * yinv;  // powers of two: exact
+    if (!a.centered) gz0 += (float)(a.colsumZ[0] * mu);
+    const int fi1 = a.shared_factor ? 0 : p, fi2 = a.shared2 ? 0 : p;
+    const float ta = single_t<F1>(a.f, a.k, a.n, a.Linv + (int64_t)fi1 * 64, a.dc[(int64_t)fi1 * kOlsMaxContrasts],
+                                  a.con[0], a.con_is_intercept[0], a.centered, gF1, hF1, r01, mu, gz0);
+    const float tb = single_t<F2>(a.f2, a.k2, a.n, a.Linv2 + (int64_t)fi2 * 64,
+                                  a.dc2[(int64_t)fi2 * kOlsMaxContrasts], a.con2, a.con2_is_intercept, a.centered,
+                                  gF2, hF2, r02, mu, gz0);
+    a.t[(int64_t)p * a.ldt + col] = sobel_z((double)ta, (double)tb, a.alg);
+  }
+}
+
 template <int LDC>
 __device__ __forceinline__ void epi_run(const OlsGemmArgs& a, const float* ep, int cl, int64_t col, int64_t mrow0,
                                         int rows, int slot, int step) {
@@ -382,6 +454,12 @@ __device__ __forceinline__ void epi_run(const OlsGemmArgs& a, const float* ep, i
   const float yinv = a.yinv ? a.yinv[col] : 1.0f;
   const int ppt = rows / a.rpp;
   const int p0 = (int)(mrow0 / rows) * ppt;
+  if (a.pair) {
+    // mediation types M and I: path A [1, x], path B [1, y, x]
+    if (a.f == 1 && a.f2 == 2) epi_loop_pair<LDC, 1, 2>(a, ep, cl, col, p0, ppt, slot, step, mu, yinv);
+    else epi_loop_pair<LDC, 0, 0>(a, ep, cl, col, p0, ppt, slot, step, mu, yinv);
+    return;
+  }
   // the design shapes of the drivers (one fixed intercept column and up to
   // seven permuted ones, rpp = m; the mediation models' intercept and
   // outcome with one or two permuted columns) get unrolled copies; others
@@ -753,13 +831,7 @@ __global__ void sobel_kernel(const float* ta, const float* tb, int64_t n, int al
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double a = ta ? (double)ta[i] : (double)ta_const;
-  const double b = (double)tb[i];
-  const double ia = 1.0 / (a * a), ib = 1.0 / (b * b), iab = 1.0 / (a * a * b * b);
-  double d;
-  if (alg == 0) d = ib + ia + iab;
-  else if (alg == 1) d = ib + ia;
-  else d = ib + ia - iab;
-  z[i] = (float)(1.0 / sqrt(d));
+  z[i] = sobel_z(a, (double)tb[i], alg);
 }
 
 }  // namespace

@@ -19,13 +19,25 @@ from . import _lib
 
 class PermutedOLS:
     def __init__(self, Y, X, contrasts=None, perm_columns=None, precision="fp16x3", center=None,
-                 stream=None):
+                 stream=None, data_from=None):
+        """data_from: another PermutedOLS whose prepared data (same Y, same
+        precision and centring) this model binds instead of preparing Y again;
+        Y is then ignored."""
         import torch
         lib = _lib.lib()
         X = np.ascontiguousarray(np.asarray(X, dtype=np.float64))
         if X.ndim == 1:
             X = X[:, None]
         n, k = X.shape
+        const = [bool(np.all(X[:, j] == X[0, j]) and X[0, j] != 0) for j in range(k)]
+        if data_from is not None:
+            assert data_from.n == n, "the shared data has another number of subjects"
+            self.n, self.k, self.V = n, k, data_from.V
+            self._owner = data_from  # keeps the data handle alive
+            self.precision = data_from.precision
+            self._data = None
+            self._bind(X, k, const, contrasts, perm_columns, data_from._data, stream)
+            return
         if not (isinstance(Y, torch.Tensor) and Y.is_cuda):
             Y = torch.as_tensor(np.ascontiguousarray(Y, dtype=np.float32)).cuda()
         Y = Y.to(torch.float32)
@@ -35,9 +47,20 @@ class PermutedOLS:
             Y = Y.contiguous()
         assert Y.shape[0] == n, "Y must be [n_subjects, n_elements]"
         self.n, self.k, self.V = n, k, int(Y.shape[1])
-        const = [bool(np.all(X[:, j] == X[0, j]) and X[0, j] != 0) for j in range(k)]
         if center is None:
             center = any(const)
+        self.precision = precision
+        self._Y = Y  # kept alive until the prep kernel has run
+        data = ctypes.c_void_p()
+        _lib.check(lib.tfce_ols_data_create(_lib.ptr(Y), n, self.V, Y.stride(0), _lib.PREC[precision],
+                                            1 if center else 0, ctypes.byref(data),
+                                            _lib.stream_ptr(stream)))
+        self._data = data
+        self._bind(X, k, const, contrasts, perm_columns, self._data, stream)
+        self._Y = None
+
+    def _bind(self, X, k, const, contrasts, perm_columns, data, stream):
+        lib = _lib.lib()
         if contrasts is None:
             contrasts = [j for j in range(k) if not const[j]]
         self.contrasts = [int(c) for c in contrasts]
@@ -46,24 +69,17 @@ class PermutedOLS:
         mask = 0
         for j in perm_columns:
             mask |= 1 << int(j)
-        self.precision = precision
-        self._Y = Y  # kept alive until the prep kernel has run
-        data = ctypes.c_void_p()
-        _lib.check(lib.tfce_ols_data_create(_lib.ptr(Y), n, self.V, Y.stride(0), _lib.PREC[precision],
-                                            1 if center else 0, ctypes.byref(data),
-                                            _lib.stream_ptr(stream)))
-        self._data = data
         con = np.ascontiguousarray(self.contrasts, dtype=np.int32)
         model = ctypes.c_void_p()
-        _lib.check(lib.tfce_ols_model_create(self._data, X.ctypes.data, k, mask, con.ctypes.data,
+        _lib.check(lib.tfce_ols_model_create(data, X.ctypes.data, k, mask, con.ctypes.data,
                                              len(self.contrasts), ctypes.byref(model),
                                              _lib.stream_ptr(stream)))
         self._model = model
-        self._Y = None
 
     def __del__(self):
         if _lib._lib is None:
             return
+        # the model first: a model bound to another's data keeps that owner alive
         m = getattr(self, "_model", None)
         if m is not None and m.value:
             _lib._lib.tfce_ols_model_destroy(m)

@@ -184,7 +184,11 @@ class MediationModel:
     scalar x -> y regression.  Types M and I permute x only, type Y permutes
     x and y together (vertex_tfce_mediation_randomise.py:82-90)."""
 
-    def __init__(self, M, pred_x, depend_y, medtype="M", alg="aroian", precision="fp16x3"):
+    def __init__(self, M, pred_x, depend_y, medtype="M", alg="aroian", precision="fp16x3", fused=True):
+        """fused (types M and I): both paths bind one prepared copy of M and
+        share one contraction per batch (``tfce_ols_perm_sobel``); otherwise
+        each path contracts on its own and ``tfce_sobel_z`` combines the
+        t-values (same numbers, bit for bit)."""
         import torch
         x = np.asarray(pred_x, dtype=np.float64)
         y = np.asarray(depend_y, dtype=np.float64)
@@ -192,13 +196,18 @@ class MediationModel:
         self.medtype, self.alg = medtype, {"aroian": 0, "sobel": 1, "goodman": 2}[alg]
         one = np.ones(n)
         self.ta_const = None
+        self.fused = False
         if medtype in ("M", "I"):
             self.A = PermutedOLS(M, np.column_stack([one, x]), contrasts=[1], perm_columns=[1], precision=precision)
             if medtype == "M":
                 XB, cb, pc = np.column_stack([one, y, x]), 1, [2]
             else:
                 XB, cb, pc = np.column_stack([one, x, y]), 1, [1]
-            self.B = PermutedOLS(M, XB, contrasts=[cb], perm_columns=pc, precision=precision)
+            if fused:
+                self.B = PermutedOLS(None, XB, contrasts=[cb], perm_columns=pc, data_from=self.A)
+                self.fused = True
+            else:
+                self.B = PermutedOLS(M, XB, contrasts=[cb], perm_columns=pc, precision=precision)
         elif medtype == "Y":
             from .cynumstats import tval_int
             Xa = np.column_stack([one, x])
@@ -215,6 +224,18 @@ class MediationModel:
         import torch
         from . import _lib
         s = stream if stream is not None else torch.cuda.current_stream()
+        if self.fused:
+            if perms is None:
+                P = 1
+            else:
+                assert perms.is_cuda and perms.dtype == torch.int32 and perms.is_contiguous()
+                P = int(perms.shape[0])
+            with torch.cuda.stream(s):
+                z = out if out is not None else torch.empty((P, self.V), dtype=torch.float32, device="cuda")
+                assert z.shape == (P, self.V) and z.is_contiguous()
+                _lib.check(_lib.lib().tfce_ols_perm_sobel(self.A._model, self.B._model, _lib.ptr(perms), P,
+                                                          self.alg, _lib.ptr(z), self.V, _lib.stream_ptr(s)))
+            return z
         with torch.cuda.stream(s):
             tb = self.B.t(perms, stream=s)[0]
             ta = self.A.t(perms, stream=s)[0] if self.A is not None else None

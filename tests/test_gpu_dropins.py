@@ -303,4 +303,5 @@ def test_fused_mediation_contraction_matches_two_contractions(torch, prec):
             a = fused.z(perms).cpu().numpy()
             b = sep.z(perms).cpu().numpy()
             np.testing.assert_array_equal(a, b)
-            assert np.isfinite(a).mean() > 0.99
+            if alg != "goodman":  # Goodman's radicand is negative where both paths are weak: NaN, as the reference
+                assert np.isfinite(a).mean() > 0.99

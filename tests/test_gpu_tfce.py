@@ -308,3 +308,29 @@ def test_mixed_set_splits_between_sweeps(torch):
                     assert got[i, p, k] == exp, (i, p, k)
                 else:
                     assert ulp_diff(np.float32([got[i, p, k]]), np.float32([exp])).max() <= 4
+
+
+def test_small_mixed_set_with_vdensity(torch):
+    """Per-element weights (the FULL sweep) on a set of small surfaces, ring-
+    packed and not: every surface takes the global sweep with the kernel
+    copy for mixed layouts, maxima equal to the oracle."""
+    from tfce_mediation_amd.tfce import Adjacency, SurfaceSet
+    graphs = [synth.icosphere_csr(3), GRAPHS["vox26"](), GRAPHS["ico4_w8"]()]
+    sizes = [len(g[0]) - 1 for g in graphs]
+    offs = np.cumsum([0] + sizes[:-1])
+    P = 6
+    say(f"[gpu] small mixed set with vdensity sizes={sizes}")
+    maps = np.zeros((P, int(sum(sizes))), np.float32)
+    vd = np.concatenate([adjacency.vertex_density(g[0]) for g in graphs]).astype(np.float32)
+    for i, (g, o) in enumerate(zip(graphs, offs)):
+        for p in range(P):
+            maps[p, o:o + sizes[i]] = smooth_map(g[0], g[1], 900 + 7 * p + i, shift=0.0)
+    ss = SurfaceSet([Adjacency(g, 2.0, 1.0) for g in graphs], offs)
+    got = ss.batch(torch.from_numpy(maps).cuda(), 3, vdensity=torch.from_numpy(vd).cuda()).cpu().numpy()
+    for i, (g, o) in enumerate(zip(graphs, offs)):
+        for p in range(P):
+            for k, sgn in enumerate((1, -1)):
+                x = (sgn * maps[p, o:o + sizes[i]]).astype(np.float32)
+                enh, _, _ = otfce.tfce(g[0], g[1], x, 2.0, 1.0, exact_pow=True)
+                s = np.float32(x.max() / np.float32(100))
+                assert got[i, p, k] == ((enh * s) * vd[o:o + sizes[i]]).max(), (i, p, k)

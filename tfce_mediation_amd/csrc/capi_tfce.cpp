@@ -471,7 +471,7 @@ static int run_batch(const SurfDev* d_surfs, int32_t n_surf, int32_t v_max, int3
                      int32_t labels_level, int32_t* d_labels, float* d_scaled, float* d_raw,
                      float* d_own, int32_t* d_status, hipStream_t stream, int out_mul = 1,
                      int out_add = 0, uint32_t batch_flags = 0, const ExtScratch* ext = nullptr,
-                     bool ring_only = false, int32_t n_lds_surf = 0) {
+                     bool all_ring = false, int32_t n_lds_surf = 0, bool rest_ring = false) {
   if (P <= 0) return 0;
   if (sign_mask < 1 || sign_mask > 3) return set_error(TFCE_EINVAL, "sign_mask must be 1, 2 or 3");
   const int n_signs = (sign_mask == 3) ? 2 : 1;
@@ -552,6 +552,9 @@ static int run_batch(const SurfDev* d_surfs, int32_t n_surf, int32_t v_max, int3
     if (e0 != hipSuccess) return set_error(-(int)e0, "shared threshold launch");
     a.ext_T0 = T0;
   }
+  // the ring-only sweep kernel when every surface the global sweep takes is
+  // ring-packed: all of them, or those left after the LDS-resident ones
+  const bool ring_only = n_lds_items > 0 ? rest_ring : all_ring;
   hipError_t e = launch_tfce_batch(a, full, ring_only, grid, n_global, stream);
   if (e != hipSuccess) return set_error(-(int)e, std::string("tfce launch: ") + hipGetErrorString(e));
   if (n_lds_items > 0) {
@@ -699,7 +702,7 @@ int tfce_batch_ex(const tfce_surfset_t* s, const float* d_maps, int64_t map_stri
     return set_error(TFCE_EINVAL, "unknown batch flag");
   return run_batch(s->d_surfs, s->n, s->v_max, s->np_w, d_maps, map_stride, P, sign_mask, nullptr, d_vdensity,
                    d_vdensity != nullptr, nullptr, 0.0f, 0.0f, -1, nullptr, d_scaled, d_raw, d_own, d_status,
-                   stream, 1, 0, flags, nullptr, big_all_ring(s), lds_tail(s));
+                   stream, 1, 0, flags, nullptr, s->all_ring, lds_tail(s), big_all_ring(s));
 }
 
 int tfce_surfset_order(const tfce_surfset_t* s, int32_t* user_index) {
@@ -714,7 +717,7 @@ int tfce_batch(const tfce_surfset_t* s, const float* d_maps, int64_t map_stride,
   if (!s || !d_maps || !d_scaled) return set_error(TFCE_EINVAL, "NULL argument");
   return run_batch(s->d_surfs, s->n, s->v_max, s->np_w, d_maps, map_stride, P, sign_mask, d_T0, d_vdensity,
                    d_vdensity != nullptr, nullptr, 0.0f, 0.0f, -1, nullptr, d_scaled, d_raw, d_own, d_status,
-                   stream, 1, 0, 0u, nullptr, big_all_ring(s), lds_tail(s));
+                   stream, 1, 0, 0u, nullptr, s->all_ring, lds_tail(s), big_all_ring(s));
 }
 
 }  // extern "C"

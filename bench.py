@@ -191,22 +191,22 @@ def cpu_baseline(cfg, perms_np, n_workers, per_worker, settings, med=False):
                 aggregate_rate=n_done / max(worker_times), values=values, used_reference=res[0][2])
 
 
-def _measured_traffic(kernel, workload, images):
-    """HBM bytes per launch of `kernel` over `images` images, from the
-    committed rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE,
-    copy-calibrated, measured over the images its line names and scaled per
-    image), or (None, None)."""
+def _measured_traffic(workload, images):
+    """HBM bytes per launch of the workload's sweep kernel over `images`
+    images, from the committed rocprofv3 counter summary (FETCH_SIZE +
+    WRITE_SIZE, copy-calibrated, measured over the images its line names and
+    scaled per image): (bytes, summary path, kernel name), or (None, None,
+    None)."""
     path = os.path.join(ROOT, "profiles", "%s_tfce_hbm_traffic.md" % workload)
     try:
         with open(path) as f:
             for line in f:
                 parts = line.split()
-                if len(parts) == 5 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel \
-                        and parts[2] == workload:
-                    return int(parts[3]) * images // int(parts[4]), os.path.relpath(path, ROOT)
+                if len(parts) == 5 and parts[0] == "traffic_bytes_per_launch" and parts[2] == workload:
+                    return int(parts[3]) * images // int(parts[4]), os.path.relpath(path, ROOT), parts[1]
     except OSError:
         pass
-    return None, None
+    return None, None, None
 
 
 def make_workload(name, precision):
@@ -537,7 +537,7 @@ def main():
             images += P
     del maps
     achieved = touch_bytes / (tfce_ms * 1e-3) / 1e9
-    traffic, traffic_src = _measured_traffic("tfce_batch_kernel<false>", name, images)
+    traffic, traffic_src, traffic_kernel = _measured_traffic(name, images)
     gemm_flops = flops_per_perm * P
     workload, data = WORKLOADS[name]
     if name == "c3":
@@ -586,7 +586,7 @@ def main():
             "contraction_frac_of_dense_peak": round(gemm_flops / (gemm_ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS, 4),
         },
         "roofline": {
-            "kernel": "tfce_batch_kernel<false> (persistent TFCE sweep)",
+            "kernel": (traffic_kernel or "tfce_batch_kernel") + " (persistent TFCE sweep)",
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,

@@ -297,7 +297,7 @@ def main():
     ap.add_argument("--wgs-per-cu", type=int, default=None,
                     help="sweep workgroups per CU instead of autotuning")
     ap.add_argument("--lds-sweep", type=int, default=None, choices=(0, 1),
-                    help="surfaces of at most 12,544 elements in the LDS-resident sweep (1) or the global one (0) "
+                    help="surfaces of at most 13,248 elements in the LDS-resident sweep (1) or the global one (0) "
                          "instead of autotuning")
     ap.add_argument("--pipeline", action="store_true",
                     help="run the contraction of step s+1 on a second stream during the sweep of step s "

@@ -283,7 +283,7 @@ int tfce_batch_ex(const tfce_surfset_t *set, const float *d_maps, int64_t map_st
 int tfce_set_occupancy(int32_t wgs_per_cu);
 
 /* Batched maxima (tfce_batch*, tfce_batch_max*) of symmetric surfaces of at
- * most 12,544 elements with a neighbour table run in an LDS-resident sweep
+ * most 13,248 elements with a neighbour table run in an LDS-resident sweep
  * (the whole union-find of an image in one compute unit's LDS) unless this
  * process-wide knob is 0; the maxima are the same bit for bit either way.
  * No counterpart in the reference. */

@@ -232,7 +232,7 @@ def test_large_hemisphere_maps(torch):
                                      ("ico4_w8", 1.0), ("ico4_w12", 1.0), ("ico4_w20", 0.5), ("grid8", 0.5),
                                      ("vox26", 0.5), ("vox6_walls", 1.0)])
 def test_lds_sweep_matches_global_sweep(torch, gname, E):
-    """Surfaces of at most 12,544 elements run in the LDS-resident sweep; the
+    """Surfaces of at most 13,248 elements run in the LDS-resident sweep; the
     global-memory sweep (knob off) must give the same maxima bit for bit, and
     both match the oracle (asymmetric lists always take the global sweep)."""
     from tfce_mediation_amd import _lib

@@ -5,7 +5,7 @@
 * the number of resident sweep workgroups per compute unit
   (``tfce_set_occupancy``): more hide more latency, fewer keep more of each
   slot's scratch in cache;
-* whether surfaces of at most 12,544 elements take the LDS-resident sweep
+* whether surfaces of at most 13,248 elements take the LDS-resident sweep
   (``tfce_set_lds_sweep``; one workgroup per compute unit holding both signs
   of a map) or the global-memory sweep; both must give the same maxima bit
   for bit, and the faster is kept.

@@ -567,7 +567,8 @@ def main():
             "precision": args.precision,
             "plain_fp32_perms_per_s": None if plain is None else round(plain, 3),
             "plain_path": "per permutation, the reference's call structure: one FP32 t map, then one "
-                          "single-image TFCE launch (one workgroup) per surface and sign",
+                          "single-image CreateAdjSet.run per surface and sign (tfce_run: the whole-GPU "
+                          "level sweep for surfaces of 32,768 elements and more, one workgroup below)",
             "batched_over_plain": None if plain is None else round(value / plain, 2),
             "batched_fp32_perms_per_s": None if "fp32" not in alt_rate else round(alt_rate["fp32"], 1),
             "batched_over_batched_fp32": None if "fp32" not in alt_rate else round(value / alt_rate["fp32"], 3),

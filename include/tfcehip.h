@@ -201,6 +201,19 @@ int tfce_ols_perm_sobel(const tfce_ols_model_t *path_a, const tfce_ols_model_t *
 int tfce_ols_perm_f(const tfce_ols_model_t *model, const int32_t *d_perm, int32_t P, const double *d_tss,
                     float *d_f, int64_t ldf, hipStream_t stream);
 
+/* Replaces the per-variable F of glm_typeI (pyfunc.py:2282-2360) inside the
+ * tm-models GLM randomise loop (tm_models_randomise.py:186-196: rows of the
+ * whole design permuted by rand_array): for each block b of caller columns
+ * [block_start[b], block_start[b] + block_width[b]) (host arrays),
+ * F_b = ((SSE of X[perm_p] without the block) - SSE) / block_width[b] /
+ * (SSE / (n - k)), written to d_f[(b * P + p) * ldf + v].  Blocks must not
+ * overlap, must leave at least one column, and may not hold the intercept of
+ * centred data; 1 <= n_blocks <= 8.  A one-column block's F is the square of
+ * that column's t.  The contrasts of the model are not used. */
+int tfce_ols_perm_partial_f(const tfce_ols_model_t *model, const int32_t *d_perm, int32_t P,
+                            const int32_t *block_start, const int32_t *block_width, int32_t n_blocks,
+                            float *d_f, int64_t ldf, hipStream_t stream);
+
 /* The total sum of squares calcF takes (cynumstats.pyx:34, TSS = sum((y -
  * np.mean(y))**2) summed over rows): d_tss[v] = sum_i (Y[i][v] - m)^2 in FP64,
  * m the grand mean of all n x V values of Y (row stride ldy).  The input to
@@ -278,6 +291,15 @@ int tfce_fwe_p64(const double *d_sorted_null, int32_t n_null, const double *d_va
 int tfce_batch_ex(const tfce_surfset_t *set, const float *d_maps, int64_t map_stride, int P,
                   int sign_mask, uint32_t flags, const float *d_vdensity, float *d_scaled, float *d_raw,
                   float *d_own, int32_t *d_status, hipStream_t stream);
+
+/* How tfce_run / tfce_run_ex sweep one image of a symmetric surface: mode -1
+ * (default) over the whole GPU for surfaces of at least 32,768 elements and in
+ * one workgroup below that, 0 always in one workgroup, 1 always over the whole
+ * GPU; max_workgroups > 0 caps the whole-GPU grid (0: one workgroup per
+ * compute unit).  Both give the same values bit for bit; the labels debug
+ * output of tfce_run_ex always takes the one-workgroup sweep.  Process-wide.
+ * No counterpart in the reference. */
+int tfce_set_single_image_mode(int32_t mode, int32_t max_workgroups);
 
 /* Number of resident workgroups per CU used by the TFCE sweep (autotuner knob). */
 int tfce_set_occupancy(int32_t wgs_per_cu);

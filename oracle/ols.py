@@ -15,6 +15,13 @@ TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
 * ``perm_t``        tval_int on X[perm] for a batch of permutations, the
                     permutation driver of vertex_tfce_multiple_regression_
                     randomise.py:105-107.
+* ``glm_typeI``     pyfunc.py:2282-2360: design [1, exog blocks, covariates],
+                    rows permuted by rand_array; the model F, one F per exog
+                    block from the fit without that block (SS_Total minus the
+                    reduced model's residual sum of squares, cynumstats.pyx:
+                    109-112), and t = a / se of every column (se float32).
+* ``perm_partial_f`` glm_typeI's per-block F for a batch of permutations
+                    (tm_models_randomise.py:190-196).
 """
 import numpy as np
 
@@ -99,4 +106,70 @@ def perm_t(X, Y, perms, contrasts, perm_columns=None):
         invXX = np.linalg.inv(nx.T.dot(nx))
         t = tval_int(nx, invXX, Y, n, k, Y.shape[1])
         out[:, p, :] = t[list(contrasts)]
+    return out
+
+
+def cy_lin_lstsqr_mat_residual(X, y):
+    a = cy_lin_lstsqr_mat(X, y)
+    resids = y - np.dot(X, a)
+    return a, np.sum(resids ** 2, axis=0)
+
+
+def glm_typeI(endog, exog, dmy_covariates=None, rand_array=None):
+    """(Fmodel, Fvar [n_blocks, V], Tvalues [k, V]) of pyfunc.py:2282-2360 with
+    output_fvalues and output_tvalues (the p-value and residual outputs are
+    left out).  ``exog`` is a list of [n] or [n, w] arrays, one per variable."""
+    endog = np.asarray(endog, dtype=np.float64)
+    n = endog.shape[0]
+    kvars = []
+    exog_vars = np.ones(n)
+    for var in exog:
+        var = np.asarray(var, dtype=np.float64)
+        kvars.append(1 if var.ndim == 1 else var.shape[1])
+        exog_vars = np.column_stack((exog_vars, var))
+    if dmy_covariates is not None:
+        exog_vars = np.column_stack((exog_vars, dmy_covariates))
+    if rand_array is not None:
+        exog_vars = exog_vars[np.asarray(rand_array)]
+    k = exog_vars.shape[1]
+    df_between, df_within = k - 1, n - k
+    ss_total = np.sum((endog - np.mean(endog, 0)) ** 2, 0)
+    a, ss_res = cy_lin_lstsqr_mat_residual(exog_vars, endog)
+    ss_between = ss_total - ss_res
+    ms_res = ss_res / df_within
+    fmodel = (ss_between / df_between) / ms_res
+    fvar = []
+    start = 1
+    for w in kvars:
+        stop = start + w
+        ss_model = ss_total - cy_lin_lstsqr_mat_residual(np.delete(exog_vars, np.s_[start:stop], 1), endog)[1]
+        fvar.append((ss_between - ss_model) / (ms_res * w))
+        start += w
+    sigma2 = np.sum((endog - np.dot(exog_vars, a)) ** 2, axis=0) / (n - k)
+    invXX = np.linalg.inv(np.dot(exog_vars.T, exog_vars))
+    se = se_of_slope(endog.shape[1], invXX, sigma2, k)
+    return fmodel, np.array(fvar), a / se
+
+
+def perm_partial_f(X, Y, perms, blocks, perm_columns=None):
+    """F [len(blocks), P, V] in FP64: for each permutation, the F of each
+    column block (start, width) of X[perm] (all columns, or only
+    perm_columns permuted) against the fit without it."""
+    X = np.asarray(X, dtype=np.float64)
+    Y = np.asarray(Y, dtype=np.float64)
+    n, k = X.shape
+    out = np.zeros((len(blocks), len(perms), Y.shape[1]))
+    ss_total = np.sum((Y - Y.mean(0)) ** 2, 0)
+    for p, perm in enumerate(perms):
+        if perm_columns is None:
+            nx = X[np.asarray(perm)]
+        else:
+            nx = X.copy()
+            cols = list(perm_columns)
+            nx[:, cols] = X[np.asarray(perm)][:, cols]
+        _, ss_res = cy_lin_lstsqr_mat_residual(nx, Y)
+        ms_res = ss_res / (n - k)
+        for b, (s0, w) in enumerate(blocks):
+            ss_model = ss_total - cy_lin_lstsqr_mat_residual(np.delete(nx, np.s_[s0:s0 + w], 1), Y)[1]
+            out[b, p] = ((ss_total - ss_res) - ss_model) / (ms_res * w)
     return out

@@ -153,3 +153,63 @@ def test_every_tile_gives_identical_t(torch, prec):
         _lib.check(lib.tfce_ols_set_tile(0))
     for o in outs[1:]:
         np.testing.assert_array_equal(o, outs[0])
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp16x3", "bf16x3"])
+def test_partial_f_matches_fp64(torch, prec):
+    """glm_typeI's per-variable F (tfce_ols_perm_partial_f) against the FP64
+    oracle: blocks of widths 1, 2 and 3 after the intercept, one covariate
+    (k = 8, the largest design),
+    every column permuted (shared Gram factor); bars as for t (F ~ t^2
+    doubles the relative error: norm <= 2e-5, point <= 2e-5 on F >= 1 for the
+    FP32-class modes)."""
+    from tfce_mediation_amd.ols import PermutedOLS
+    n, V, P = 110, 2600, 21
+    say(f"[gpu] ols partial F {prec}")
+    rng = np.random.default_rng(41)
+    x1 = rng.standard_normal(n)
+    grp = rng.integers(0, 3, n)
+    d2 = np.column_stack([(grp == 1), (grp == 2)]).astype(float)
+    b3 = rng.standard_normal((n, 3))
+    cov = rng.standard_normal((n, 1))
+    X = np.column_stack([np.ones(n), x1, d2, b3, cov]).astype(np.float32).astype(np.float64)
+    Y = (rng.standard_normal((n, V)) + 0.5 * x1[:, None] * (rng.random(V) < 0.3)
+         + 0.6 * d2[:, :1] * (rng.random(V) < 0.3)).astype(np.float32)
+    perms = np.stack([rng.permutation(n) for _ in range(P)]).astype(np.int32)
+    blocks = [(1, 1), (2, 2), (4, 3)]
+    model = PermutedOLS(torch.from_numpy(Y).cuda(), X, contrasts=[1], precision=prec)
+    f = model.partial_f(torch.from_numpy(perms).cuda(), blocks).cpu().numpy()
+    ref = ools.perm_partial_f(X, Y, perms, blocks)
+    norm = np.linalg.norm(f - ref) / np.linalg.norm(ref)
+    big = ref >= 1
+    point = np.max(np.abs(f - ref)[big] / ref[big])
+    say(f"    norm={norm:.2e} point={point:.2e}")
+    if prec == "bf16x3":
+        assert norm <= 4e-5
+    else:
+        assert norm <= 2e-5 and point <= 2e-5
+    # one-column block: F = t^2 of the same column from the t path
+    t = model.t(torch.from_numpy(perms).cuda()).cpu().numpy()[0].astype(np.float64)
+    np.testing.assert_allclose(f[0], t ** 2, rtol=2e-5, atol=1e-6)
+
+
+def test_partial_f_with_fixed_columns(torch):
+    """Partial permutation (only columns 1..3 move): per-permutation factors;
+    the block of fixed covariates and a permuted block against the oracle."""
+    from tfce_mediation_amd.ols import PermutedOLS
+    n, V, P = 90, 1500, 13
+    say("[gpu] ols partial F, fixed covariates")
+    rng = np.random.default_rng(42)
+    X = np.column_stack([np.ones(n), rng.standard_normal((n, 3)), rng.standard_normal((n, 2))])
+    X = X.astype(np.float32).astype(np.float64)
+    Y = (rng.standard_normal((n, V)) + 0.4 * X[:, 1:2] + 0.3 * X[:, 4:5]).astype(np.float32)
+    perms = np.stack([rng.permutation(n) for _ in range(P)]).astype(np.int32)
+    blocks = [(1, 3), (4, 2)]
+    model = PermutedOLS(torch.from_numpy(Y).cuda(), X, contrasts=[1], perm_columns=[1, 2, 3], precision="fp16x3")
+    f = model.partial_f(torch.from_numpy(perms).cuda(), blocks).cpu().numpy()
+    ref = ools.perm_partial_f(X, Y, perms, blocks, perm_columns=[1, 2, 3])
+    norm = np.linalg.norm(f - ref) / np.linalg.norm(ref)
+    big = ref >= 1
+    point = np.max(np.abs(f - ref)[big] / ref[big])
+    say(f"    norm={norm:.2e} point={point:.2e}")
+    assert norm <= 2e-5 and point <= 2e-5

@@ -147,6 +147,68 @@ def test_run_accumulates_and_degenerate(torch):
     assert np.array_equal(enh, base)
 
 
+@pytest.mark.parametrize("gname", sorted(GRAPHS))
+def test_whole_gpu_single_image_matches_oracle(torch, gname):
+    """tfce_run over the whole GPU (cooperative level sweep, forced on for
+    every size) against the oracle, at three grid sizes; E = 1 bit-exact,
+    E = 0.5 within 4 ulp; a degenerate image adds nothing."""
+    from tfce_mediation_amd import _lib
+    from tfce_mediation_amd.tfce import CreateAdjSet
+    row_ptr, col = GRAPHS[gname]()
+    lib = _lib.lib()
+    say(f"[gpu] whole-GPU tfce_run {gname} V={len(row_ptr)-1}")
+    try:
+        for grid in (0, 1, 7):
+            _lib.check(lib.tfce_set_single_image_mode(1, grid))
+            for E in (1.0, 0.5):
+                op = CreateAdjSet(2.0, E, (row_ptr, col))
+                for seed in range(2):
+                    x = smooth_map(row_ptr, col, 40 + seed) * np.float32(1 - 2 * seed)
+                    if gname == "vox6_walls":
+                        x = x + np.float32(1e-4) * np.arange(len(x), dtype=np.float32) / len(x)
+                    enh = np.zeros_like(x)
+                    op.run(x, enh)
+                    ref, st, _ = otfce.tfce(row_ptr, col, x, 2.0, E, exact_pow=True)
+                    assert st > 0
+                    check_close(enh, ref, E, 2.0)
+        base = np.linspace(-1, 1, len(row_ptr) - 1).astype(np.float32)
+        enh = base.copy()
+        op.run(np.zeros_like(base), enh)
+        assert np.array_equal(enh, base)
+    finally:
+        _lib.check(lib.tfce_set_single_image_mode(-1, 0))
+
+
+def test_whole_gpu_single_image_large_surfaces(torch):
+    """The default mode takes the whole-GPU sweep for surfaces of 32,768
+    elements and more: an order-7 hemisphere (163,842 vertices) and a
+    26-connected volume, against the oracle and against the one-workgroup
+    sweep, bit for bit."""
+    from tfce_mediation_amd import _lib
+    from tfce_mediation_amd.tfce import CreateAdjSet
+    lib = _lib.lib()
+    cases = [("ico7", synth.icosphere_csr(7)),
+             ("vox26_big", adjacency.voxel_csr(synth.ellipsoid_mask((46, 52, 44), (22.5, 25.5, 21.5),
+                                                                    (21, 24, 20)), 26))]
+    for name, (row_ptr, col) in cases:
+        V = len(row_ptr) - 1
+        assert V >= 32768
+        say(f"[gpu] whole-GPU tfce_run {name} V={V}")
+        op = CreateAdjSet(2.0, 1.0, (row_ptr, col))
+        x = smooth_map(row_ptr, col, 77, passes=5, shift=0.1)
+        enh = np.zeros_like(x)
+        op.run(x, enh)
+        ref, st, _ = otfce.tfce(row_ptr, col, x, 2.0, 1.0, exact_pow=True)
+        check_close(enh, ref, 1.0, 2.0)
+        try:
+            _lib.check(lib.tfce_set_single_image_mode(0, 0))
+            one = np.zeros_like(x)
+            op.run(x, one)
+        finally:
+            _lib.check(lib.tfce_set_single_image_mode(-1, 0))
+        assert np.array_equal(one.view(np.uint32), enh.view(np.uint32))
+
+
 def canon(labels):
     out = np.full(labels.shape, -1, np.int64)
     act = labels >= 0

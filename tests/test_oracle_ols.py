@@ -127,3 +127,73 @@ def test_calcF_matches_r_squared_of_lstsq():
     rss0 = rss[0]
     assert F2[0] == pytest.approx(((tss0 - rss0) / (k - 1)) / (rss0 / (n - k)), rel=1e-9)
     assert tss0 == pytest.approx(tss[0] + n * (Y2[:, 0].mean() - g) ** 2, rel=1e-12)
+
+
+def _dummies(labels, g):
+    """Treatment coding: g - 1 indicator columns (level 0 is the reference)."""
+    return np.column_stack([(labels == j).astype(float) for j in range(1, g)])
+
+
+def test_glm_typeI_one_factor_is_one_way_anova():
+    """Intercept + one dummy-coded factor of g levels: the factor's F (and the
+    model F) is scipy's one-way ANOVA F of the groups, element by element."""
+    rng = np.random.default_rng(21)
+    n, g, V = 45, 4, 12
+    labels = np.arange(n) % g
+    rng.shuffle(labels)
+    Y = rng.standard_normal((n, V)) + 0.5 * labels[:, None] * rng.standard_normal(V)
+    fmodel, fvar, _ = ols.glm_typeI(Y, [_dummies(labels, g)])
+    ref = np.array([stats.f_oneway(*[Y[labels == j, v] for j in range(g)]).statistic for v in range(V)])
+    np.testing.assert_allclose(fvar[0], ref, rtol=1e-9)
+    np.testing.assert_allclose(fmodel, ref, rtol=1e-9)
+
+
+def test_glm_typeI_balanced_two_factor_closed_form():
+    """Balanced additive two-factor design (a x b cells, r per cell): the F of
+    each factor is SS_factor / (levels - 1) over the additive model's mean
+    square error, with SS_A = b r sum_i (mean_i - grand)^2 (no shared code)."""
+    rng = np.random.default_rng(22)
+    A, B, r, V = 3, 4, 5, 7
+    la = np.repeat(np.arange(A), B * r)
+    lb = np.tile(np.repeat(np.arange(B), r), A)
+    n = A * B * r
+    Y = rng.standard_normal((n, V)) + 0.4 * la[:, None] - 0.3 * (lb == 2)[:, None]
+    _, fvar, _ = ols.glm_typeI(Y, [_dummies(la, A), _dummies(lb, B)])
+    g = Y.mean(0)
+    ssa = B * r * sum((Y[la == i].mean(0) - g) ** 2 for i in range(A))
+    ssb = A * r * sum((Y[lb == j].mean(0) - g) ** 2 for j in range(B))
+    sst = ((Y - g) ** 2).sum(0)
+    mse = (sst - ssa - ssb) / (n - (1 + (A - 1) + (B - 1)))
+    np.testing.assert_allclose(fvar[0], (ssa / (A - 1)) / mse, rtol=1e-9)
+    np.testing.assert_allclose(fvar[1], (ssb / (B - 1)) / mse, rtol=1e-9)
+
+
+def test_glm_typeI_single_columns_are_t_squared_and_reparametrisation_free():
+    """A one-column block's F is the square of that column's t (tval_int,
+    pinned by QR above; se is float32 there); a block's F does not change when
+    its columns are replaced by an invertible mix of them, nor do the other
+    blocks' F; the t-values are tval_int's on X[rand_array]."""
+    rng = np.random.default_rng(23)
+    n, V = 70, 15
+    x1 = rng.standard_normal(n)
+    blk = rng.standard_normal((n, 3))
+    cov = rng.standard_normal((n, 2))
+    Y = rng.standard_normal((n, V)) + 0.4 * x1[:, None] + 0.3 * blk[:, :1]
+    perm = rng.permutation(n)
+    fmodel, fvar, t = ols.glm_typeI(Y, [x1, blk], dmy_covariates=cov, rand_array=perm)
+    X = np.column_stack([np.ones(n), x1, blk, cov])[perm]
+    tq = _qr_t(X, Y)
+    np.testing.assert_allclose(fvar[0], tq[1] ** 2, rtol=2e-6)
+    np.testing.assert_allclose(t, tq, rtol=1e-6)
+    T = rng.standard_normal((3, 3)) + 3 * np.eye(3)
+    _, fvar2, _ = ols.glm_typeI(Y, [x1, blk @ T], dmy_covariates=cov, rand_array=perm)
+    np.testing.assert_allclose(fvar2, fvar, rtol=1e-9)
+    # the model F against R^2 of lstsq
+    _, rss, _, _ = np.linalg.lstsq(X, Y, rcond=None)
+    r2 = 1 - rss / ((Y - Y.mean(0)) ** 2).sum(0)
+    k = X.shape[1]
+    np.testing.assert_allclose(fmodel, (r2 / (k - 1)) / ((1 - r2) / (n - k)), rtol=1e-9)
+    # the batched form
+    blocks = [(1, 1), (2, 3)]
+    fp = ols.perm_partial_f(np.column_stack([np.ones(n), x1, blk, cov]), Y, [perm], blocks)
+    np.testing.assert_allclose(fp[:, 0], fvar, rtol=1e-12)

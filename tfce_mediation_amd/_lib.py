@@ -36,6 +36,7 @@ SIGNATURES = {
     "tfce_surfset_order": (c_int, [c_p, c_p]),
     "tfce_batch": (c_int, [c_p, c_p, c_i64, c_int, c_int, c_p, c_p, c_p, c_p, c_p, c_p, c_p]),
     "tfce_set_occupancy": (c_int, [c_i32]),
+    "tfce_set_single_image_mode": (c_int, [c_i32, c_i32]),
     "tfce_set_lds_sweep": (c_int, [c_i32]),
     "tfce_batch_ex": (c_int, [c_p, c_p, c_i64, c_int, c_int, c_u32, c_p, c_p, c_p, c_p, c_p, c_p]),
     "tfce_ols_data_create": (c_int, [c_p, c_i32, c_i64, c_i64, c_i32, c_i32, c_p, c_p]),
@@ -46,6 +47,7 @@ SIGNATURES = {
     "tfce_ols_perm_t": (c_int, [c_p, c_p, c_i32, c_p, c_i64, c_p]),
     "tfce_ols_perm_f": (c_int, [c_p, c_p, c_i32, c_p, c_p, c_i64, c_p]),
     "tfce_ols_perm_sobel": (c_int, [c_p, c_p, c_p, c_i32, c_i32, c_p, c_i64, c_p]),
+    "tfce_ols_perm_partial_f": (c_int, [c_p, c_p, c_i32, c_p, c_p, c_i32, c_p, c_i64, c_p]),
     "tfce_ols_set_tile": (c_int, [c_i32]),
     "tfce_ols_tile": (c_int, []),
     "tfce_fwe_p": (c_int, [c_p, c_i32, c_p, c_i64, c_i32, c_p, c_p]),

@@ -12,7 +12,9 @@
 
 Each candidate is timed with HIP events on the real workload (one batch of P
 permutations of an engine), the contraction and the sweep separately since
-each knob touches one of them.  Every contraction tile must reproduce the
+each knob touches one of them.  Candidates are timed in interleaved rounds
+and each keeps its fastest round, so that a clock ramp or a first-use cost
+early in the search does not fall on whichever candidate happens to be first.  Every contraction tile must reproduce the
 default tile's t-values bit for bit (the k order of the sums does not depend
 on the tile); a tile that does not is dropped.  Results are cached in a JSON
 file keyed by device and workload shape.
@@ -29,20 +31,30 @@ OCCUPANCIES = (2, 3, 4, 5, 6, 8)
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "tfce_mediation_amd", "autotune.json")
 
 
-def _time(fn, reps):
+def _time_once(fn):
     import torch
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
     fn()
+    e1.record()
     torch.cuda.synchronize()
-    ts = []
-    for _ in range(reps):
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record()
-        fn()
-        e1.record()
-        torch.cuda.synchronize()
-        ts.append(e0.elapsed_time(e1))
-    return float(np.median(ts))
+    return e0.elapsed_time(e1)
+
+
+def _time_candidates(cands, setup, fn, rounds):
+    """{candidate: fastest of ``rounds`` timings}, the candidates interleaved
+    within each round (one untimed call after each switch)."""
+    import torch
+    best = {}
+    for _ in range(rounds):
+        for c in cands:
+            setup(c)
+            fn()
+            torch.cuda.synchronize()
+            t = _time_once(fn)
+            best[c] = min(best.get(c, float("inf")), t)
+    return best
 
 
 def workload_key(engine, P):
@@ -78,6 +90,7 @@ def tune(engine, P=256, reps=3, tiles=TILES, occupancies=OCCUPANCIES, cache=DEFA
     perms = synth.permutations(engine.n, P, seed, "cuda")
     timings = {"gemm": {}, "tfce": {}}
     ref = None
+    good = []
     for tile in tiles:
         _lib.check(lib.tfce_ols_set_tile(tile))
         t = engine.tvalues(perms)
@@ -88,7 +101,9 @@ def tune(engine, P=256, reps=3, tiles=TILES, occupancies=OCCUPANCIES, cache=DEFA
             if log:
                 log(f"[autotune] tile {tile} changes the t-values; dropped")
             continue
-        timings["gemm"][tile] = _time(lambda: engine.tvalues(perms), reps)
+        good.append(tile)
+    timings["gemm"] = _time_candidates(good, lambda c: _lib.check(lib.tfce_ols_set_tile(c)),
+                                       lambda: engine.tvalues(perms), reps)
     best_tile = min(timings["gemm"], key=timings["gemm"].get)
     _lib.check(lib.tfce_ols_set_tile(best_tile))
     t = engine.tvalues(perms)
@@ -97,19 +112,25 @@ def tune(engine, P=256, reps=3, tiles=TILES, occupancies=OCCUPANCIES, cache=DEFA
     def sweep():
         return engine.surfaces.batch(t, signs, vdensity=engine.vdensity)
     _lib.check(lib.tfce_set_lds_sweep(0))
-    for occ in occupancies:
-        _lib.check(lib.tfce_set_occupancy(occ))
-        timings["tfce"][occ] = _time(sweep, reps)
-    best_occ = min(timings["tfce"], key=timings["tfce"].get)
-    _lib.check(lib.tfce_set_occupancy(best_occ))
+    _lib.check(lib.tfce_set_occupancy(occupancies[0]))
     ref_null = sweep().cpu().numpy()
     _lib.check(lib.tfce_set_lds_sweep(1))
-    lds = 0
-    if np.array_equal(sweep().cpu().numpy(), ref_null, equal_nan=True):
-        timings["tfce"]["lds"] = _time(sweep, reps)
-        lds = 1 if timings["tfce"]["lds"] < timings["tfce"][best_occ] else 0
-    elif log:
+    lds_ok = bool(np.array_equal(sweep().cpu().numpy(), ref_null, equal_nan=True))
+    if not lds_ok and log:
         log("[autotune] the LDS sweep changes the maxima; dropped")
+    _lib.check(lib.tfce_set_lds_sweep(0))
+    timings["tfce"] = _time_candidates(list(occupancies), lambda c: _lib.check(lib.tfce_set_occupancy(c)), sweep,
+                                       reps)
+    best_occ = min(timings["tfce"], key=timings["tfce"].get)
+    _lib.check(lib.tfce_set_occupancy(best_occ))
+    lds = 0
+    if lds_ok:
+        # the LDS sweep against the global one at the chosen occupancy (which
+        # the global part of a mixed set keeps)
+        tl = _time_candidates([0, 1], lambda c: _lib.check(lib.tfce_set_lds_sweep(c)), sweep, reps)
+        timings["tfce"]["lds"] = tl[1]
+        lds = 1 if tl[1] < tl[0] else 0
+    _lib.check(lib.tfce_set_lds_sweep(lds))
     cfg = {"gemm_tile": int(best_tile), "tfce_wgs_per_cu": int(best_occ), "lds_sweep": lds,
            "timings_ms": {k: {str(a): round(b, 3) for a, b in v.items()} for k, v in timings.items()}}
     apply(cfg)

@@ -46,6 +46,8 @@ struct tfce_ols_model {
   double* dc1 = nullptr;
   float zscale[kOlsMaxK] = {1, 1, 1, 1, 1, 1, 1, 1};  // fp16 mode: per permuted column
   std::vector<double> hZ;  // [n][m] permuted columns (host copy: pairing checks)
+  int32_t internal[kOlsMaxK] = {0};  // caller column -> internal column
+  bool constant[kOlsMaxK] = {false};  // caller column is constant
 };
 
 namespace {
@@ -318,6 +320,10 @@ int tfce_ols_model_create(const tfce_ols_data_t* data, const double* X, int32_t 
   for (int q = 0; q < f; ++q) shared = shared && constval[order[q]] != 0.0;
   md->shared = shared;
   md->hZ = Z;
+  for (int j = 0; j < k; ++j) {
+    md->internal[j] = internal[j];
+    md->constant[j] = constval[j] != 0.0;
+  }
   int rc = 0;
   const int64_t V = data->V;
   if ((rc = dalloc(&md->F, F.size(), "F")) || (rc = dalloc(&md->Z, Z.size(), "Z")) ||
@@ -405,8 +411,10 @@ int tfce_ols_set_tile(int32_t tile) {
 int tfce_ols_tile(void) { return g_gemm_tile; }
 
 // t-values (tss == null) or sqrt(max(F, 0)) for P permutations
+// (n_blk > 0: partial F of the column blocks blk_mask, internal column sets)
 static int perm_run(const tfce_ols_model_t* md, const int32_t* d_perm, int32_t P, float* d_t, int64_t ldt,
-                    const double* tss, hipStream_t stream, const tfce_ols_model_t* md2 = nullptr, int alg = 0) {
+                    const double* tss, hipStream_t stream, const tfce_ols_model_t* md2 = nullptr, int alg = 0,
+                    int n_blk = 0, const uint32_t* blk_mask = nullptr) {
   if (!md || !d_t) return set_error(TFCE_EINVAL, "NULL argument");
   const tfce_ols_data* d = md->data;
   if (!d_perm) P = 1;
@@ -416,7 +424,7 @@ static int perm_run(const tfce_ols_model_t* md, const int32_t* d_perm, int32_t P
   // whole permutations (the epilogue reads a permutation's rows from LDS, so
   // they need not align with the 32-row MFMA tiles)
   const int rpp = md->m > 0 ? md->m : 1;
-  const int tile = d->precision == kPrecFP32 ? 0 : g_gemm_tile;
+  const int tile = (d->precision == kPrecFP32 || n_blk > 0) ? 0 : g_gemm_tile;
   const int tile_rows = d->precision == kPrecFP32 ? kGemmRows : gemm_tile_rows(tile);
   const int tile_cols = d->precision == kPrecFP32 ? kGemmCols : gemm_tile_cols(tile);
   const int per_tile = tile_rows / rpp;
@@ -425,7 +433,14 @@ static int perm_run(const tfce_ols_model_t* md, const int32_t* d_perm, int32_t P
   const int pieces = d->pieces;
   const size_t abytes = (size_t)d->Kp * Mpad * (pieces ? 2 * pieces : 4);
   const size_t fbytes1 = (size_t)P * (64 * sizeof(float) + kOlsMaxContrasts * sizeof(double));
-  const size_t fbytes = (md->shared ? 0 : fbytes1) + ((md2 && !md2->shared) ? fbytes1 + 256 : 0);
+  // the block F's factors: per permutation, or once (P = 1) for a shared Gram matrix
+  const int P_blk = n_blk > 0 ? (md->shared ? 1 : P) : 0;
+  const size_t fbytes_blk =
+      n_blk > 0 ? (size_t)P_blk * (64 * sizeof(float) + kOlsMaxContrasts * sizeof(double)) + 256 +
+                      (size_t)P_blk * n_blk * 64 * sizeof(float) + 256
+                : 0;
+  const size_t fbytes = (md->shared || n_blk > 0 ? 0 : fbytes1) + ((md2 && !md2->shared) ? fbytes1 + 256 : 0) +
+                        fbytes_blk;
   void* ws = nullptr;
   int rc = ols_ws(stream, abytes + fbytes + 512, &ws);
   if (rc) return rc;
@@ -454,36 +469,48 @@ static int perm_run(const tfce_ols_model_t* md, const int32_t* d_perm, int32_t P
   // per-permutation Gram factors of a model whose fixed columns are not all
   // constant, in the workspace after A
   char* fb = base + ((abytes + 255) / 256 * 256);
-  auto factors = [&](const tfce_ols_model_t* mm, const float** Linv, const double** dc) -> int {
+  auto factors = [&](const tfce_ols_model_t* mm, const float** Linv, const double** dc, int nb = 0,
+                     const float** Rb = nullptr) -> int {
     *Linv = mm->Linv1;
     *dc = mm->dc1;
-    if (mm->shared) return 0;
+    if (mm->shared && nb == 0) return 0;
+    const int Pf = mm->shared ? 1 : P;
     float* Lw = reinterpret_cast<float*>(fb);
-    double* dw = reinterpret_cast<double*>(fb + (size_t)P * 64 * sizeof(float));
-    fb += (fbytes1 + 255) / 256 * 256;
+    double* dw = reinterpret_cast<double*>(fb + (size_t)Pf * 64 * sizeof(float));
+    fb += ((size_t)Pf * (64 * sizeof(float) + kOlsMaxContrasts * sizeof(double)) + 255) / 256 * 256;
+    float* Rw = nullptr;
+    if (nb > 0) {
+      Rw = reinterpret_cast<float*>(fb);
+      fb += ((size_t)Pf * nb * 64 * sizeof(float) + 255) / 256 * 256;
+    }
     OlsFactorArgs ka{};
     ka.F = mm->F;
     ka.Z = mm->Z;
     ka.FF = mm->FF;
     ka.ZZ = mm->ZZ;
-    ka.perm = d_perm;
+    ka.perm = mm->shared ? nullptr : d_perm;
     ka.n = d->n;
     ka.f = mm->f;
     ka.m = mm->m;
-    ka.P = P;
+    ka.P = Pf;
     ka.n_con = mm->n_con;
     for (int c = 0; c < kOlsMaxContrasts; ++c) ka.con[c] = mm->con[c];
     ka.Linv = Lw;
     ka.dc = dw;
+    ka.n_blk = nb;
+    for (int b = 0; b < nb; ++b) ka.blk_mask[b] = blk_mask[b];
+    ka.Rb = Rw;
     hipError_t ef = launch_ols_factors(ka, stream);
     if (ef != hipSuccess) return set_error(-(int)ef, "ols factor launch");
     *Linv = Lw;
     *dc = dw;
+    if (Rb) *Rb = Rw;
     return 0;
   };
   const float* Linv = nullptr;
   const double* dc = nullptr;
-  if ((rc = factors(md, &Linv, &dc))) return rc;
+  const float* Rb = nullptr;
+  if ((rc = factors(md, &Linv, &dc, n_blk, &Rb))) return rc;
   OlsGemmArgs ga{};
   ga.a32 = ba.a32;
   for (int p = 0; p < 3; ++p) {
@@ -523,6 +550,12 @@ static int perm_run(const tfce_ols_model_t* md, const int32_t* d_perm, int32_t P
   ga.n_mtiles = (int32_t)n_mtiles;
   ga.n_ntiles = (int32_t)(d->Npad / tile_cols);
   ga.tss = tss;
+  if (n_blk > 0) {
+    ga.n_blk = n_blk;
+    ga.blk_shared = md->shared ? 1 : 0;
+    for (int b = 0; b < n_blk; ++b) ga.blk_w[b] = __builtin_popcount(blk_mask[b]);
+    ga.Rb = Rb;
+  }
   if (md2) {
     const float* Linv2 = nullptr;
     const double* dc2 = nullptr;
@@ -561,6 +594,28 @@ int tfce_ols_perm_sobel(const tfce_ols_model_t* path_a, const tfce_ols_model_t* 
   if (path_a->hZ != path_b->hZ)
     return set_error(TFCE_EINVAL, "the two models must permute the same column values");
   return perm_run(path_a, d_perm, P, d_z, ldz, nullptr, stream, path_b, alg);
+}
+
+int tfce_ols_perm_partial_f(const tfce_ols_model_t* md, const int32_t* d_perm, int32_t P,
+                            const int32_t* block_start, const int32_t* block_width, int32_t n_blocks, float* d_f,
+                            int64_t ldf, hipStream_t stream) {
+  if (!md || !block_start || !block_width || !d_f) return set_error(TFCE_EINVAL, "NULL argument");
+  if (n_blocks < 1 || n_blocks > kOlsMaxK) return set_error(TFCE_EINVAL, "1..8 column blocks");
+  uint32_t masks[kOlsMaxK] = {0};
+  uint32_t used = 0;
+  for (int b = 0; b < n_blocks; ++b) {
+    const int s0 = block_start[b], w = block_width[b];
+    if (w < 1 || s0 < 0 || s0 + w > md->k) return set_error(TFCE_EINVAL, "column block out of range");
+    if (w >= md->k) return set_error(TFCE_EINVAL, "a column block must leave at least one column");
+    for (int j = s0; j < s0 + w; ++j) {
+      if (md->constant[j] && md->data->center)
+        return set_error(TFCE_ENOTSUP, "a column block holds the constant column of centred data");
+      masks[b] |= 1u << md->internal[j];
+    }
+    if (used & masks[b]) return set_error(TFCE_EINVAL, "column blocks overlap");
+    used |= masks[b];
+  }
+  return perm_run(md, d_perm, P, d_f, ldf, nullptr, stream, nullptr, 0, n_blocks, masks);
 }
 
 int tfce_ols_perm_f(const tfce_ols_model_t* md, const int32_t* d_perm, int32_t P, const double* d_tss,

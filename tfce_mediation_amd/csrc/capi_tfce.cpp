@@ -54,6 +54,11 @@ static std::mutex g_ws_mutex;
 static std::map<std::pair<int, hipStream_t>, Workspace> g_ws;
 static int g_wgs_per_cu = 4;
 static bool g_lds_sweep = true;  // small surfaces in the LDS-resident sweep
+// tfce_run on one image: -1 the whole-GPU sweep for symmetric surfaces of at
+// least kCoopMinV elements, 0 never, 1 for every symmetric surface
+static int g_coop_mode = -1;
+static int g_coop_grid = 0;
+constexpr int32_t kCoopMinV = 32768;
 
 int current_device() {
   int d = 0;
@@ -170,6 +175,14 @@ const char* tfce_version(void) { return "tfcehip 0.1 gfx950"; }
 
 int tfce_set_lds_sweep(int32_t enable) {
   g_lds_sweep = enable != 0;
+  return 0;
+}
+
+int tfce_set_single_image_mode(int32_t mode, int32_t max_workgroups) {
+  if (mode < -1 || mode > 1) return set_error(TFCE_EINVAL, "mode must be -1 (auto), 0 or 1");
+  if (max_workgroups < 0) return set_error(TFCE_EINVAL, "max_workgroups must be >= 0");
+  g_coop_mode = mode;
+  g_coop_grid = max_workgroups;
   return 0;
 }
 
@@ -590,6 +603,71 @@ int batch_max_in(tfce_adj_t* adj, const float* d_maps, int P, int sign_mask, con
 
 extern "C" {
 
+}  // extern "C"
+
+// One image over the whole GPU: its record, then the cooperative level sweep.
+static int run_coop(tfce_adj_t* adj, const float* d_image, float* d_enhn, float min_t, float dh,
+                    int32_t* d_status, hipStream_t stream) {
+  const int64_t V = adj->V;
+  const size_t rec_bytes = ((size_t)kPrepWords * sizeof(int32_t) + 255) / 256 * 256;
+  const size_t arr_bytes = ((size_t)V * sizeof(int32_t) + 255) / 256 * 256;
+  const size_t ctl_bytes = (size_t)kCoopCtlWords * sizeof(int32_t);
+  Workspace* ws = nullptr;
+  int rc = workspace_get(stream, rec_bytes + 256 + 8 * arr_bytes + ctl_bytes + 256, &ws);
+  if (rc) return rc;
+  char* base = reinterpret_cast<char*>(ws->ptr);
+  int32_t* rec = reinterpret_cast<int32_t*>(base);
+  float* raw = reinterpret_cast<float*>(base + rec_bytes);
+  char* arr = base + rec_bytes + 256;
+  TfceBatchArgs a{};
+  a.surfs = adj->d_surf;
+  a.n_surf = 1;
+  a.n_maps = 1;
+  a.n_signs = 1;
+  a.sign0 = 1.0f;
+  a.maps = d_image;
+  a.map_stride = V;
+  a.min_t = min_t;
+  a.dh = dh;
+  a.raw_max = raw;
+  a.queue = ws->queue;
+  a.n_items = 1;
+  a.prep = rec;
+  a.v_max = (int32_t)V;
+  hipError_t e = launch_tfce_batch(a, true, false, 1, 0, stream);
+  if (e != hipSuccess) return set_error(-(int)e, std::string("tfce record launch: ") + hipGetErrorString(e));
+  TfceCoopArgs c{};
+  c.row_ptr = adj->d_row_ptr;
+  c.col = adj->d_col;
+  c.sizepow = adj->d_sizepow;
+  c.E = adj->E;
+  c.V = (int32_t)V;
+  c.x = d_image;
+  c.enhn = d_enhn;
+  c.rec = rec;
+  int32_t* w[8];
+  for (int q = 0; q < 8; ++q) w[q] = reinterpret_cast<int32_t*>(arr + q * arr_bytes);
+  c.lev = w[0];
+  c.order = w[1];
+  c.parent = w[2];
+  c.size = w[3];
+  c.roots[0] = w[4];
+  c.roots[1] = w[5];
+  c.rootc = w[6];
+  c.acc = reinterpret_cast<float*>(w[7]);
+  c.ctl = reinterpret_cast<int32_t*>(arr + 8 * arr_bytes);
+  // the grid barrier's counter and generation (the ctl region's last words)
+  e = launch_fill_f32(reinterpret_cast<float*>(c.ctl + kCoopBarWord), 2, 0.0f, stream);
+  if (e != hipSuccess) return set_error(-(int)e, "clear barrier words");
+  c.status = d_status;
+  c.grid_limit = g_coop_grid;
+  e = launch_tfce_coop(c, stream);
+  if (e != hipSuccess) return set_error(-(int)e, std::string("tfce cooperative launch: ") + hipGetErrorString(e));
+  return 0;
+}
+
+extern "C" {
+
 int tfce_run_ex(tfce_adj_t* adj, const float* d_image, float* d_enhn, float min_t, float dh, int accumulate,
                 int32_t labels_level, int32_t* d_labels, int32_t* d_status, hipStream_t stream) {
   if (!adj || !d_image || !d_enhn) return set_error(TFCE_EINVAL, "NULL argument");
@@ -598,6 +676,11 @@ int tfce_run_ex(tfce_adj_t* adj, const float* d_image, float* d_enhn, float min_
     hipError_t e = launch_fill_f32(d_enhn, adj->V, 0.0f, stream);
     if (e != hipSuccess) return set_error(-(int)e, "fill enhn");
   }
+  // a large symmetric surface runs over the whole GPU (the labels debug
+  // output exists only in the one-workgroup sweep)
+  if ((adj->flags & kSurfSymmetric) && labels_level < 0 &&
+      (g_coop_mode == 1 || (g_coop_mode == -1 && adj->V >= kCoopMinV)))
+    return run_coop(adj, d_image, d_enhn, min_t, dh, d_status, stream);
   return run_batch(adj->d_surf, 1, adj->V, np_width(adj), d_image, adj->V, 1, 1, nullptr, nullptr, true, d_enhn, min_t,
                    dh, labels_level, d_labels, nullptr, nullptr, nullptr, d_status, stream, 1, 0, 0u, nullptr,
                    (adj->flags & kSurfRing) != 0);

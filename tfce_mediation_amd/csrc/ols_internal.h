@@ -60,6 +60,13 @@ struct OlsFactorArgs {
   int32_t con[kOlsMaxContrasts];  // internal column index of each contrast
   float* Linv;           // [P][8][8]
   double* dc;            // [P][kOlsMaxContrasts]
+  // Column blocks whose F against the fit without them is wanted: block b's
+  // internal columns are the set bits of blk_mask[b]; Rb[p][b] (8 x 8, rows
+  // < the block's width) maps h = L^-1 X_p'y to a vector whose squared norm
+  // is the block's extra sum of squares.
+  int32_t n_blk;
+  uint32_t blk_mask[kOlsMaxK];
+  float* Rb;             // [P][n_blk][8][8] or null
 };
 
 struct OlsBuildAArgs {
@@ -115,6 +122,11 @@ struct OlsGemmArgs {
   const float* gF2;      // [f2][V]
   const float* hF2;      // [f2][V]
   const double* r02;     // [V]
+  // Partial F of column blocks (n_blk > 0): F_b = |Rb[p][b] h|^2 / w_b over
+  // SSE / (n - k), written to t[(b * P + p) * ldt + col].
+  int32_t n_blk, blk_shared;
+  int32_t blk_w[kOlsMaxK];
+  const float* Rb;       // [P or 1][n_blk][8][8]
 };
 
 constexpr int kOlsDirectMaxN = 2048;  // subjects held in LDS by the FP64 kernel

@@ -209,6 +209,51 @@ __global__ __launch_bounds__(64) void ols_factor_kernel(OlsFactorArgs a) {
       for (int j = a.con[c]; j < k; ++j) s += Li[j][a.con[c]] * Li[j][a.con[c]];
     a.dc[(int64_t)p * kOlsMaxContrasts + c] = s;
   }
+  // Block b with columns B: beta = Li' h, and the extra sum of squares of B
+  // given the other columns is beta_B' [(X'X)^-1_BB]^-1 beta_B = |C^-1 beta_B|^2
+  // with C C' = (X'X)^-1_BB = (Li' Li)_BB; row r of Rb is sum_s C^-1[r][s]
+  // Li[.][B_s], so |Rb h|^2 is that sum of squares.
+  for (int b = 0; b < a.n_blk && a.Rb; ++b) {
+    int B[kOlsMaxK], w = 0;
+    for (int j = 0; j < k; ++j)
+      if ((a.blk_mask[b] >> j) & 1u) B[w++] = j;
+    double S[kOlsMaxK][kOlsMaxK], C[kOlsMaxK][kOlsMaxK], Ci[kOlsMaxK][kOlsMaxK];
+    for (int r = 0; r < w; ++r)
+      for (int q = 0; q < w; ++q) {
+        double s = 0.0;
+        for (int j = (B[r] > B[q] ? B[r] : B[q]); j < k; ++j) s += Li[j][B[r]] * Li[j][B[q]];
+        S[r][q] = s;
+        C[r][q] = 0.0;
+        Ci[r][q] = 0.0;
+      }
+    for (int j = 0; j < w; ++j) {
+      double d = S[j][j];
+      for (int q = 0; q < j; ++q) d -= C[j][q] * C[j][q];
+      d = d > 0.0 ? sqrt(d) : NAN;
+      C[j][j] = d;
+      for (int i = j + 1; i < w; ++i) {
+        double s = S[i][j];
+        for (int q = 0; q < j; ++q) s -= C[i][q] * C[j][q];
+        C[i][j] = s / d;
+      }
+    }
+    for (int j = 0; j < w; ++j) {
+      Ci[j][j] = 1.0 / C[j][j];
+      for (int i = j + 1; i < w; ++i) {
+        double s = 0.0;
+        for (int q = j; q < i; ++q) s += C[i][q] * Ci[q][j];
+        Ci[i][j] = -s / C[i][i];
+      }
+    }
+    float* R = a.Rb + ((int64_t)p * a.n_blk + b) * 64;
+    for (int r = 0; r < 8; ++r)
+      for (int j = 0; j < 8; ++j) {
+        double s = 0.0;
+        if (r < w && j < k)
+          for (int q = 0; q <= r; ++q) s += Ci[r][q] * Li[j][B[q]];
+        R[r * 8 + j] = (float)s;
+      }
+  }
 }
 
 __global__ __launch_bounds__(256) void ols_build_a_kernel(OlsBuildAArgs a) {
@@ -272,7 +317,9 @@ __device__ __forceinline__ bf16x8 ld_bf16x8(const uint16_t* p) {
 // (cynumstats.pyx:47-64), t = a / se in FP64, returned as float.
 // FF / MM > 0: the fixed / permuted column counts as compile-time constants
 // (every loop unrolls with constant bounds); 0: read them from the arguments.
-template <bool UNIFORM, int FF, int MM>
+// BLK: write the partial F of the column blocks instead (a kernel instance of
+// its own, so the t epilogue's register budget does not carry it).
+template <bool UNIFORM, int FF, int MM, bool BLK>
 __device__ __forceinline__ void ols_epilogue(const OlsGemmArgs& a, int p, int64_t col, const float* gz,
                                              const float* gF, const float* hF, double r0, double mu) {
   const int f = FF > 0 ? FF : a.f, m = MM > 0 ? MM : a.m, k = f + m;
@@ -298,6 +345,28 @@ __device__ __forceinline__ void ols_epilogue(const OlsGemmArgs& a, int p, int64_
   }
   double sse = r0 - ssz;
   if (sse < 0.0) sse = 0.0;
+  if constexpr (BLK) {
+    // partial F of each column block: |Rb h|^2 / w over SSE / (n - k)
+    // (pyfunc.py:2340-2352 computes the same extra sum of squares as the
+    // difference of the reduced and the full model's residual sums)
+    const float* R = a.Rb + (int64_t)(a.blk_shared ? 0 : fi) * a.n_blk * 64;
+    const double s2 = sse / (double)(a.n - k);
+#pragma unroll 1
+    for (int b = 0; b < a.n_blk; ++b) {
+      double ex = 0.0;
+#pragma unroll 1
+      for (int r = 0; r < a.blk_w[b]; ++r) {
+        const float* Rr = R + b * 64 + r * 8;
+        float u = 0.0f;
+#pragma unroll
+        for (int j = 0; j < kOlsMaxK; ++j)
+          if (j < k) u = fmaf(Rr[j], (j < f) ? hF[j] : hz[(j - f) & (kOlsMaxK - 1)], u);
+        ex += (double)u * (double)u;
+      }
+      a.t[((int64_t)b * a.P + p) * a.ldt + col] = (float)(ex / (double)a.blk_w[b] / s2);
+    }
+    return;
+  }
   if (a.tss) {
     // F of the whole model against the caller's total sum of squares
     // (cynumstats.pyx:31-36), as sqrt(max(F, 0)) (voxel randomise :85-87)
@@ -389,7 +458,7 @@ __device__ __forceinline__ void epi_put(float* ep, int ldc, const f32x16 (&acc)[
 
 // Thread (cl, slot): column cl of the block (global element `col`),
 // permutations slot, slot + step, ... of the workgroup's row tile.
-template <int LDC, int FF, int MM>
+template <int LDC, int FF, int MM, bool BLK = false>
 __device__ __forceinline__ void epi_loop(const OlsGemmArgs& a, const float* ep, int cl, int64_t col, int p0, int ppt,
                                          int slot, int step, const float* gF, const float* hF, double r0, double mu,
                                          float yinv) {
@@ -405,7 +474,7 @@ __device__ __forceinline__ void epi_loop(const OlsGemmArgs& a, const float* ep, 
       if (!a.centered && q < m) gz[q] += (float)(a.colsumZ[q] * mu);
     }
     // with 64 columns per block a wave's lanes share one permutation
-    ols_epilogue<LDC == 64, FF, MM>(a, p, col, gz, gF, hF, r0, mu);
+    ols_epilogue<LDC == 64, FF, MM, BLK>(a, p, col, gz, gF, hF, r0, mu);
   }
 }
 
@@ -439,7 +508,7 @@ __device__ __forceinline__ void epi_loop_pair(const OlsGemmArgs& a, const float*
   }
 }
 
-template <int LDC>
+template <int LDC, bool BLK>
 __device__ __forceinline__ void epi_run(const OlsGemmArgs& a, const float* ep, int cl, int64_t col, int64_t mrow0,
                                         int rows, int slot, int step) {
   const bool cvalid = col < a.V;
@@ -454,6 +523,10 @@ __device__ __forceinline__ void epi_run(const OlsGemmArgs& a, const float* ep, i
   const float yinv = a.yinv ? a.yinv[col] : 1.0f;
   const int ppt = rows / a.rpp;
   const int p0 = (int)(mrow0 / rows) * ppt;
+  if constexpr (BLK) {
+    epi_loop<LDC, 0, 0, true>(a, ep, cl, col, p0, ppt, slot, step, gF, hF, r0, mu, yinv);
+    return;
+  }
   if (a.pair) {
     // mediation types M and I: path A [1, x], path B [1, y, x]
     if (a.f == 1 && a.f2 == 2) epi_loop_pair<LDC, 1, 2>(a, ep, cl, col, p0, ppt, slot, step, mu, yinv);
@@ -498,6 +571,7 @@ __device__ __forceinline__ bool gemm_tile(const OlsGemmArgs& a, int& mt, int& nt
 
 // Plain FP32 contraction (v_mfma_f32_32x32x2f32): each wave holds 64 rows x
 // 128 columns and reads its fragments straight from global memory.
+template <bool BLK>
 __global__ __launch_bounds__(256) void ols_gemm_fp32_kernel(OlsGemmArgs a) {
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, h = l >> 5, r = l & 31;
   int mt, nt;
@@ -544,7 +618,7 @@ __global__ __launch_bounds__(256) void ols_gemm_fp32_kernel(OlsGemmArgs a) {
     __syncthreads();
     epi_put<2, 4>(ep, 32, acc, pass, w * 64, 0, h, r);
     __syncthreads();
-    epi_run<32>(a, ep, tid & 31, n0 + 32 * pass + (tid & 31), (int64_t)mt * kGemmRows, kGemmRows, tid >> 5, 8);
+    epi_run<32, BLK>(a, ep, tid & 31, n0 + 32 * pass + (tid & 31), (int64_t)mt * kGemmRows, kGemmRows, tid >> 5, 8);
   }
 }
 
@@ -622,7 +696,7 @@ __device__ __forceinline__ void gemm_stage_copy(const OlsGemmArgs& a, uint8_t* s
   }
 }
 
-template <int NP, bool F16, int WR, int WC>
+template <int NP, bool F16, int WR, int WC, bool BLK = false>
 __global__ __launch_bounds__(64 * WR * WC) void ols_gemm_split_kernel(OlsGemmArgs a) {
   using S = GemmStage<NP, WR, WC>;
   __shared__ __attribute__((aligned(16))) uint8_t smem[S::kSmem];
@@ -709,7 +783,7 @@ __global__ __launch_bounds__(64 * WR * WC) void ols_gemm_split_kernel(OlsGemmArg
     epi_put<4, 2>(ep, ldc, acc, pass, wr * 128, wc * 32, h, r);
     __syncthreads();
     const int cl = tid % ldc;
-    epi_run<ldc>(a, ep, cl, ncol0 + (cl >> 5) * 64 + 32 * pass + (cl & 31), mrow0, S::kRows, tid / ldc,
+    epi_run<ldc, BLK>(a, ep, cl, ncol0 + (cl >> 5) * 64 + 32 * pass + (cl & 31), mrow0, S::kRows, tid / ldc,
                  S::kThreads / ldc);
   }
 }
@@ -739,22 +813,30 @@ hipError_t launch_ols_build_a(const OlsBuildAArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <int WR, int WC>
+template <int WR, int WC, bool BLK = false>
 static void launch_split(const OlsGemmArgs& a, int precision, unsigned blocks, hipStream_t s) {
   const dim3 grid(blocks), block(64 * WR * WC);
   if (precision == kPrecBF16x3)
-    hipLaunchKernelGGL((ols_gemm_split_kernel<2, false, WR, WC>), grid, block, 0, s, a);
+    hipLaunchKernelGGL((ols_gemm_split_kernel<2, false, WR, WC, BLK>), grid, block, 0, s, a);
   else if (precision == kPrecFP16x3)
-    hipLaunchKernelGGL((ols_gemm_split_kernel<2, true, WR, WC>), grid, block, 0, s, a);
+    hipLaunchKernelGGL((ols_gemm_split_kernel<2, true, WR, WC, BLK>), grid, block, 0, s, a);
   else
-    hipLaunchKernelGGL((ols_gemm_split_kernel<3, false, WR, WC>), grid, block, 0, s, a);
+    hipLaunchKernelGGL((ols_gemm_split_kernel<3, false, WR, WC, BLK>), grid, block, 0, s, a);
 }
 
 hipError_t launch_ols_gemm(const OlsGemmArgs& a, int precision, int tile, hipStream_t s) {
   const int64_t ntp = (a.n_ntiles + 7) / 8 * 8;
   const unsigned blocks = (unsigned)(ntp * a.n_mtiles);
+  if (a.n_blk > 0) {
+    // block F: the 256 x 128 tile only (the caller sizes rows for it)
+    if (precision == kPrecFP32)
+      hipLaunchKernelGGL(ols_gemm_fp32_kernel<true>, dim3(blocks), dim3(256), 0, s, a);
+    else
+      launch_split<2, 2, true>(a, precision, blocks, s);
+    return hipGetLastError();
+  }
   if (precision == kPrecFP32)
-    hipLaunchKernelGGL(ols_gemm_fp32_kernel, dim3(blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(ols_gemm_fp32_kernel<false>, dim3(blocks), dim3(256), 0, s, a);
   else if (tile == 1)
     launch_split<4, 1>(a, precision, blocks, s);
   else if (tile == 2)

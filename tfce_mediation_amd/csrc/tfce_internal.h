@@ -21,6 +21,7 @@ constexpr int32_t kPosNone = 0xffffff;  // widest table whose per-image neighbou
 // Status codes written per work item (>= 0: number of levels visited).
 constexpr int32_t kStatusDegenerate = -2;  // max >= minT with a step <= 0: the reference never returns
 constexpr int32_t kStatusTooManyLevels = -3;
+constexpr int32_t kStatusNoProgress = -4;  // a union-find join exceeded its retry bound
 
 // Surface flags.
 constexpr int32_t kSurfSymmetric = 1;     // every edge is listed from both ends
@@ -97,6 +98,34 @@ constexpr int kPrepT0 = 2;    // first threshold (float bits)
 constexpr int kPrepDT = 3;    // step (float bits)
 constexpr int kPrepT = 4;     // kMaxLevels thresholds, then kMaxLevels values of T^H
 constexpr int kPrepWords = kPrepT + 2 * kMaxLevels;
+
+// One image over the whole GPU (FULL mode, symmetric adjacency), after the
+// item's record has been written by launch_tfce_batch with n_global = 0.
+struct TfceCoopArgs {
+  const int32_t* row_ptr;  // V+1
+  const int32_t* col;
+  const double* sizepow;   // null when E == 1
+  float E;
+  int32_t V;
+  const float* x;          // image [V]
+  float* enhn;             // [V], accumulated
+  const int32_t* rec;      // the item's record (kPrepWords)
+  int32_t* lev;            // [V] scratch
+  int32_t* order;          // [V]
+  int32_t* parent;         // [V]
+  int32_t* size;           // [V]
+  int32_t* roots[2];       // [V] each
+  int32_t* rootc;          // [V] final root of the element at each position
+  float* acc;              // [V] TFCE value by position
+  int32_t* ctl;            // [kCoopCtlWords], the barrier words zero at launch
+  int32_t* status;         // [1] or null
+  int32_t grid_limit;      // at most this many workgroups (0: one per compute unit)
+};
+// ctl layout: level starts [kMaxLevels + 1], cursors [kMaxLevels], root-list
+// lengths [3], failure flag [1], grid barrier counter and generation [2]
+constexpr int kCoopBarWord = 2 * kMaxLevels + 5;
+constexpr int kCoopCtlWords = kCoopBarWord + 2;
+hipError_t launch_tfce_coop(const TfceCoopArgs& a, hipStream_t stream);
 
 // Clears the work counters, writes every item's record (a.n_items) and runs
 // the global-memory sweep over items [0, n_global) (none when n_global <= 0).

@@ -1525,7 +1525,250 @@ __global__ void zero_queue_kernel(int32_t* q) {
   q[3] = 0;
 }
 
+// ---------------------------------------------------------------------------
+// One image over the whole GPU (FULL mode, symmetric adjacency): the drop-in
+// CreateAdjSet.run on a large surface.  A cooperative grid (every workgroup
+// resident) runs the levels in lock step, two grid-wide barriers per level:
+//   X(l): every element of level l is joined with its active neighbours
+//         (device-scope CAS union-find, larger root hooked under the
+//         smaller); beside it, every element active at level l-1 adds that
+//         level's increment of its component (float)(s^E * fl(T^H)), in
+//         level order as fast_tfce.hpp:72-84 adds them (roots and sizes of
+//         level l-1 were fixed in Y(l-1), and joins change neither);
+//   Y(l): every new element adds one to its final root's size, every root
+//         of the previous level that was hooked adds its whole size to its
+//         final root, the surviving roots form the next root list, and every
+//         active element's final root is recorded by position.
+// TFCE values are accumulated by position (coalesced) and written back to the
+// elements at the end; position i is always visited by the same thread.
+__device__ __forceinline__ int gld(const int32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void gst(int32_t* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ int guf_find(int32_t* parent, int x) {
+  int p = gld(parent + x);
+  while (p != x) {
+    const int gp = gld(parent + p);
+    if (gp != p) gst(parent + x, gp);  // path halving; gp is an ancestor of x
+    x = p;
+    p = gp;
+  }
+  return x;
+}
+
+// false if the join did not settle within the retry bound (the kernel then
+// reports an error status instead of looping)
+__device__ __forceinline__ bool guf_link(int32_t* parent, int a, int b) {
+  for (int it = 0; it < (1 << 20); ++it) {
+    a = guf_find(parent, a);
+    b = guf_find(parent, b);
+    if (a == b) return true;
+    if (a < b) { const int t = a; a = b; b = t; }
+    int expected = a;
+    if (__hip_atomic_compare_exchange_strong(parent + a, &expected, b, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT))
+      return true;
+  }
+  return false;
+}
+
+// Grid-wide barrier of a cooperative launch (every workgroup is resident, so
+// every wait ends): an arrival counter and a generation word at device scope;
+// the last workgroup to arrive resets the counter and opens the next
+// generation.  Workgroup-scope barriers and agent-scope fences around it make
+// every workgroup's writes before it visible to every workgroup after it.  A
+// wait that exceeds its bound (seconds) sets *fail and goes on.
+__device__ __forceinline__ void grid_barrier(int32_t* bar, int32_t* fail, int& gen) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    const int n = (int)gridDim.x;
+    const int arrived = __hip_atomic_fetch_add(bar, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (arrived == n - 1) {
+      __hip_atomic_store(bar, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(bar + 1, gen + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      int spins = 0;
+      while (__hip_atomic_load(bar + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+        __builtin_amdgcn_s_sleep(1);
+        if (++spins > (1 << 25)) {
+          __hip_atomic_store(fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  ++gen;
+  __syncthreads();
+}
+
+constexpr int kCoopThreads = 256;
+
+__global__ __launch_bounds__(kCoopThreads) void tfce_coop_kernel(TfceCoopArgs a) {
+  __shared__ float s_T[kMaxLevels];
+  __shared__ float s_HH[kMaxLevels];
+  __shared__ int s_start[kMaxLevels + 1];
+  const int tid = threadIdx.x;
+  const int status = a.rec[kPrepL];
+  if (blockIdx.x == 0 && tid == 0 && a.status && status <= 0) a.status[0] = status;
+  const int L = status > 0 ? status : 0;
+  if (L == 0) return;  // the same record for every workgroup: uniform
+  const float T0 = __int_as_float(a.rec[kPrepT0]);
+  const float dT = __int_as_float(a.rec[kPrepDT]);
+  for (int l = tid; l < L; l += kCoopThreads) {
+    s_T[l] = __int_as_float(a.rec[kPrepT + l]);
+    s_HH[l] = __int_as_float(a.rec[kPrepT + kMaxLevels + l]);
+  }
+  __syncthreads();
+  const int V = a.V;
+  const int gtid = blockIdx.x * kCoopThreads + tid, gsz = gridDim.x * kCoopThreads;
+  int32_t* start = a.ctl;                 // [kMaxLevels + 1]
+  int32_t* cur = start + kMaxLevels + 1;  // [kMaxLevels]
+  int32_t* nroots = cur + kMaxLevels;     // [3] root-list lengths, cycled by level
+  int32_t* fail = nroots + 3;             // [1] a join or a wait that did not settle
+  int32_t* bar = a.ctl + kCoopBarWord;    // [2] barrier counter and generation, zero at launch
+  int gen = 0;
+  for (int l = gtid; l < L; l += gsz) cur[l] = 0;
+  if (gtid == 0) {
+    nroots[0] = nroots[1] = nroots[2] = 0;
+    *fail = 0;
+  }
+  grid_barrier(bar, fail, gen);
+  // activation level of every element, histogram
+  for (int v = gtid; v < V; v += gsz) {
+    const int lv = level_of(a.x[v], s_T, L, T0, dT);
+    a.lev[v] = lv;
+    a.parent[v] = v;
+    a.size[v] = 0;
+    if (lv != kNever) atomicAdd(&cur[lv], 1);
+  }
+  grid_barrier(bar, fail, gen);
+  if (blockIdx.x == 0 && tid < 64) {
+    const int per = (L + 63) / 64;
+    const int b = tid * per, e = min(L, b + per);
+    int sum = 0;
+    for (int l = b; l < e; ++l) sum += cur[l];
+    const int incl = wave_incl_scan(sum);
+    int run = incl - sum;
+    for (int l = b; l < e; ++l) {
+      const int c = cur[l];
+      start[l] = run;
+      cur[l] = run;
+      run += c;
+    }
+    if (tid == 63) start[L] = incl;
+  }
+  grid_barrier(bar, fail, gen);
+  for (int v = gtid; v < V; v += gsz) {
+    const int lv = a.lev[v];
+    if (lv != kNever) {
+      const int q = atomicAdd(&cur[lv], 1);
+      a.order[q] = v;
+      a.acc[q] = a.enhn[v];
+    }
+  }
+  for (int l = tid; l <= L; l += kCoopThreads) s_start[l] = start[l];
+  grid_barrier(bar, fail, gen);
+  const int32_t* rp = a.row_ptr;
+  const int32_t* col = a.col;
+  bool ok = true;
+  for (int l = 0; l <= L; ++l) {
+    // X(l): level l-1's increments, level l's joins
+    if (l > 0) {
+      // four positions per thread and pass, loads before the dependent ones
+      const float HH = s_HH[l - 1];
+      const int e0 = s_start[l];
+      for (int i0 = gtid; i0 < e0; i0 += 4 * gsz) {
+        int r[4], sz[4];
+        float ac[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int i = i0 + j * gsz;
+          r[j] = i < e0 ? a.rootc[i] : -1;
+          ac[j] = i < e0 ? a.acc[i] : 0.0f;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) sz[j] = r[j] >= 0 ? a.size[r[j]] : 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int i = i0 + j * gsz;
+          if (i < e0) a.acc[i] = ac[j] + tfce_inc(sz[j], HH, a.E, a.sizepow);
+        }
+      }
+    }
+    if (l == L) break;
+    const int b = s_start[l], e = s_start[l + 1];
+    for (int i = b + gtid; i < e; i += gsz) {
+      const int v = a.order[i];
+      for (int j = rp[v]; j < rp[v + 1]; ++j) {
+        const int u = col[j];
+        if (u != v && a.lev[u] <= l) ok = guf_link(a.parent, v, u) && ok;
+      }
+    }
+    if (gtid == 0) nroots[(l + 2) % 3] = 0;
+    grid_barrier(bar, fail, gen);
+    // Y(l): sizes of the surviving roots, next root list, roots by position
+    const int32_t* rc = a.roots[l & 1];
+    int32_t* rn = a.roots[(l + 1) & 1];
+    const int ncur = nroots[l % 3];
+    for (int i = gtid; i < ncur; i += gsz) {
+      const int r = rc[i];
+      const int f = guf_find(a.parent, r);
+      if (f == r) rn[atomicAdd(&nroots[(l + 1) % 3], 1)] = r;
+      else atomicAdd(&a.size[f], a.size[r]);
+    }
+    // an older element's root changes only if that root was hooked
+    for (int i0 = gtid; i0 < b; i0 += 4 * gsz) {
+      int r[4], pr[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) r[j] = i0 + j * gsz < b ? a.rootc[i0 + j * gsz] : -1;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) pr[j] = r[j] >= 0 ? gld(a.parent + r[j]) : -1;
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (pr[j] != r[j]) a.rootc[i0 + j * gsz] = guf_find(a.parent, r[j]);
+    }
+    for (int i = b + gtid; i < e; i += gsz) {
+      const int v = a.order[i];
+      const int f = guf_find(a.parent, v);
+      a.rootc[i] = f;
+      atomicAdd(&a.size[f], 1);
+      if (f == v) rn[atomicAdd(&nroots[(l + 1) % 3], 1)] = v;
+    }
+    grid_barrier(bar, fail, gen);
+  }
+  const int A = s_start[L];
+  for (int i = gtid; i < A; i += gsz) a.enhn[a.order[i]] = a.acc[i];
+  if (!ok) atomicOr(fail, 1);
+  grid_barrier(bar, fail, gen);
+  if (blockIdx.x == 0 && tid == 0 && a.status) a.status[0] = *fail ? kStatusNoProgress : L;
+}
+
 }  // namespace
+
+hipError_t launch_tfce_coop(const TfceCoopArgs& a, hipStream_t stream) {
+  int per_cu = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, tfce_coop_kernel, kCoopThreads, 0);
+  if (e != hipSuccess) return e;
+  int dev = 0, cus = 0;
+  if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+  if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+  // one workgroup per 4,096 elements, at most 64 and one per compute unit:
+  // the grid barrier's cost grows with the workgroups it waits for (an
+  // order-7 hemisphere took 2.9 ms with 32 workgroups, 3.5 ms with 64 and
+  // 6.5 ms with 160)
+  int64_t want = std::min<int64_t>(64, ((int64_t)a.V + 4095) / 4096);
+  int64_t cap = (int64_t)std::min(per_cu, 1) * cus;
+  if (a.grid_limit > 0) cap = std::min<int64_t>(cap, a.grid_limit);
+  const int grid = (int)std::max<int64_t>(1, std::min(want, cap));
+  void* args[] = {const_cast<TfceCoopArgs*>(&a)};
+  return hipLaunchCooperativeKernel(reinterpret_cast<void*>(tfce_coop_kernel), dim3(grid), dim3(kCoopThreads),
+                                    args, 0, stream);
+}
 
 hipError_t launch_tfce_batch(const TfceBatchArgs& a, bool full, bool ring_only, int grid, int64_t n_global,
                              hipStream_t stream) {

@@ -118,6 +118,25 @@ class PermutedOLS:
                                              self.V, _lib.stream_ptr(stream)))
         return out
 
+    def partial_f(self, perms, blocks, out=None, stream=None):
+        """F [len(blocks), P, V] float32 of each column block (start, width) of
+        X[perm] against the fit without it, for every permutation of perms
+        [P, n] (glm_typeI's per-variable F, pyfunc.py:2340-2352)."""
+        import numpy as np
+        import torch
+        assert perms.is_cuda and perms.dtype == torch.int32 and perms.is_contiguous()
+        assert perms.dim() == 2 and perms.shape[1] == self.n
+        P = int(perms.shape[0])
+        starts = np.ascontiguousarray([int(b[0]) for b in blocks], dtype=np.int32)
+        widths = np.ascontiguousarray([int(b[1]) for b in blocks], dtype=np.int32)
+        if out is None:
+            out = torch.empty((len(blocks), P, self.V), dtype=torch.float32, device="cuda")
+        assert out.is_contiguous() and out.shape == (len(blocks), P, self.V)
+        _lib.check(_lib.lib().tfce_ols_perm_partial_f(self._model, _lib.ptr(perms), P, starts.ctypes.data,
+                                                     widths.ctypes.data, len(blocks), _lib.ptr(out), self.V,
+                                                     _lib.stream_ptr(stream)))
+        return out
+
 
 def total_ss(Y):
     """sum_i (y_iv - mean(Y))^2 per element [V] (float64, device): the TSS of

@@ -8,6 +8,7 @@
 * ``write_perm_maxTFCE_vertex`` / ``_voxel``          pyfunc.py:107-126
 * ``calc_sobelz`` / ``calc_indirect``                 pyfunc.py:130-162, 2678-2709
 * ``find_nearest``                                    pyfunc.py:254-261
+* ``glm_typeI``                                       pyfunc.py:2282-2404
 * ``check_blocks`` / ``rand_blocks``                  pyfunc.py:2711-2757
 The TFCE and statistics run on the GPU; the null writers append to the same
 files in the same text format as the reference, with Python file I/O instead
@@ -172,6 +173,82 @@ def calc_sobelz(medtype, pred_x, depend_y, merge_y, n, num_vertex, alg="aroian")
     else:
         raise ValueError("Invalid mediation type")
     return calc_indirect(ta, tb, alg).astype(np.float64)
+
+
+def glm_typeI(endog, exog, dmy_covariates=None, output_fvalues=True, output_tvalues=False, output_pvalues=False,
+              verbose=True, rand_array=None, use_reduced_residuals=False, output_reduced_residuals=False,
+              exog_names=None):
+    """GLM with one F per exog variable (pyfunc.py:2282-2404): design [1, exog
+    variables, covariates], rows permuted by rand_array; the model F, each
+    variable's F from the fit without its columns, and t = a / se of every
+    column (se float32).  The least-squares fits run on the GPU in FP64
+    (``tfce_ols_direct``) on float32 data; with use_reduced_residuals the
+    residuals that replace endog are rounded to float32 as well (the
+    reference keeps them in float64).  Same return tuples as the reference."""
+    from scipy.stats import f as fdist
+    from scipy.stats import t as tdist
+    from .cynumstats import _direct
+    from .randomise import glm_design
+    endog = np.asarray(endog)
+    n = endog.shape[0]
+    X, blocks = glm_design(n, exog, dmy_covariates)
+    if rand_array is not None:
+        if use_reduced_residuals:
+            endog = _direct(X, endog, resid=True)["resid"]
+        X = X[np.asarray(rand_array)]
+    reduced_data = None
+    if output_reduced_residuals:
+        reduced_data = _direct(X, endog, resid=True)["resid"]
+    k = X.shape[1]
+    df_between, df_within, df_total = k - 1, n - k, n - 1
+    full = _direct(X, endog, coef=True, se=output_tvalues, sse=True)
+    ss_total = _direct(np.ones((n, 1)), endog, sse=True)["sse"]
+    ss_res = full["sse"]
+    if output_fvalues:
+        ss_between = ss_total - ss_res
+        ms_res = ss_res / df_within
+        fvalues = (ss_between / df_between) / ms_res
+        if verbose:
+            print("Source\t\tDF\tF(Max)")
+            print("Model\t\t(%d,%d)\t%.2f" % (df_between, df_within, fvalues.max()))
+        fvar, pvar = [], []
+        for i, (s0, w) in enumerate(blocks):
+            ss_model = ss_total - _direct(np.delete(X, np.s_[s0:s0 + w], 1), endog, sse=True)["sse"]
+            ftemp = (ss_between - ss_model) / (ms_res * w)
+            fvar.append(ftemp)
+            if verbose:
+                label = exog_names[i] if exog_names is not None else "Exog%d" % (i + 1)
+                print("%s\t\t(%d,%d)\t%.2f" % (label, w, df_within, ftemp.max()))
+            if output_pvalues:
+                pvar.append(fdist.sf(ftemp, w, df_within))
+    if output_tvalues:
+        tvalues = full["coef"] / full["se"]
+        if endog.ndim == 1:
+            tvalues = tvalues[:, 0]
+    if output_fvalues and endog.ndim == 1:
+        fvalues = fvalues[0]
+        fvar = [v[0] for v in fvar]
+        pvar = [v[0] for v in pvar]
+    if output_tvalues and output_fvalues:
+        if output_pvalues:
+            return (fvalues, np.array(fvar), tvalues, fdist.sf(fvalues, df_between, df_within), np.array(pvar),
+                    tdist.sf(np.abs(tvalues), df_total) * 2)
+        if output_reduced_residuals:
+            return fvalues, np.array(fvar), tvalues, reduced_data
+        return fvalues, np.array(fvar), tvalues
+    if output_tvalues:
+        if output_pvalues:
+            return tvalues, tdist.sf(np.abs(tvalues), df_total) * 2
+        if output_reduced_residuals:
+            return tvalues, reduced_data
+        return tvalues
+    if output_fvalues:
+        if output_pvalues:
+            return fvalues, np.array(fvar), fdist.sf(fvalues, df_between, df_within), np.array(pvar)
+        if output_reduced_residuals:
+            return fvalues, np.array(fvar), reduced_data
+        return fvalues, np.array(fvar)
+    print("No output has been selected")
 
 
 def find_nearest(array, value, p_array):

@@ -14,6 +14,9 @@ the GPU (permuted t on MFMA, one persistent TFCE launch per batch):
 * ``voxel_mediation``  voxel_tfce_mediation_randomise.py:42-76 (Sobel z of a
   voxel mask, positive pass only, ``perm_Zstat_<type>_TFCE_maxVoxel.csv`` in
   "%1.4f");
+* ``tm_glm``           the GLM branch of tm_models_randomise.py:182-243
+  (glm_typeI statistics of the whole permuted design: t per exog column and
+  F per exog variable, ``perm_Tstat_con<j>`` / ``perm_Fstat_<var>`` files);
 * ``mmr``              tm_func.calculate_tfce in a loop (the mmr randomise of
   tm_multimodality_multisurface_regression.py:523-572): per-surface nulls
   ``perm_maxTFCE_surf<s>_tcon<c>.csv`` in "%f"; ``merged=True`` shares the
@@ -287,6 +290,90 @@ def voxel_mediation(ny, pred_x, depend_y, medtype, adjac, optstfce, perms, out_d
     if out_dir is not None:
         _write(os.path.join(out_dir, "perm_Zstat_%s_TFCE_maxVoxel.csv" % medtype), null, "%1.4f")
     return null
+
+
+def glm_design(n, exog, dmy_covariates=None):
+    """(X, column blocks) of glm_typeI (pyfunc.py:2298-2311): [1, exog
+    variables (one or more columns each), covariates]; block i = (first
+    column, width) of exog variable i."""
+    cols, blocks, start = [np.ones(n)], [], 1
+    for var in exog:
+        var = np.asarray(var, dtype=np.float64)
+        w = 1 if var.ndim == 1 else var.shape[1]
+        cols.append(var.reshape(n, w))
+        blocks.append((start, w))
+        start += w
+    if dmy_covariates is not None:
+        cols.append(np.asarray(dmy_covariates, dtype=np.float64).reshape(n, -1))
+    return np.column_stack(cols), blocks
+
+
+def tm_glm(data, exog, dmy_covariates, gstat, optstfce, perms, adjac=None, adjac_lh=None, adjac_rh=None,
+           mask_lh=None, mask_rh=None, vdensity_lh=1, vdensity_rh=1, varnames=None, out_dir=None, batch=None,
+           precision="fp16x3"):
+    """The GLM branch of the tm-models randomise loop
+    (tm_models_randomise.py:182-243): per permutation the rows of the whole
+    design [1, exog, covariates] are permuted (rand_array) and glm_typeI's
+    statistics of ``data`` [n, V] go through TFCE.  gstat "t": the t of every
+    exog column j = 1..numcon, +t then -t, ``perm_Tstat_con<j>_TFCE_max*``;
+    "f": the F of every exog variable (its block of columns against the fit
+    without it), positive only, ``perm_Fstat_<varname>_TFCE_max*``; anything
+    else: both.  Vertex mode (adjac_lh given): both hemispheres as
+    write_perm_maxTFCE_vertex, files ``..._TFCE_maxVertex.csv`` in "%.4f";
+    voxel mode (adjac): ``..._TFCE_maxVoxel.csv`` in "%1.4f", with NaN and
+    infinite statistics set to 0 (the reference's max would be nan there).
+    Returns {statname: null}, [n_perm, 2] for t and [n_perm] for F."""
+    import torch
+    data = np.asarray(data) if not isinstance(data, torch.Tensor) else data
+    n = int(data.shape[0])
+    X, blocks = glm_design(n, exog, dmy_covariates)
+    numcon = sum(w for _, w in blocks)
+    if varnames is None:
+        varnames = ["Exog%d" % (i + 1) for i in range(len(blocks))]
+    gstat = str(gstat)
+    want_t, want_f = gstat != "f", gstat != "t"
+    H, E = float(optstfce[0]), float(optstfce[1])
+    vd = None
+    if adjac_lh is not None:
+        (lh, fl), (rh, fr) = _hemi_csr(adjac_lh, mask_lh), _hemi_csr(adjac_rh, mask_rh)
+        nv_lh = int(np.asarray(mask_lh, bool).sum())
+        if not np.isscalar(vdensity_lh) or not np.isscalar(vdensity_rh):
+            vl = np.broadcast_to(np.asarray(vdensity_lh, np.float32), (len(mask_lh),))[np.asarray(mask_lh, bool)]
+            vr = np.broadcast_to(np.asarray(vdensity_rh, np.float32), (len(mask_rh),))[np.asarray(mask_rh, bool)]
+            vd = torch.as_tensor(np.ascontiguousarray(np.concatenate([vl, vr]))).cuda()
+        (lh, rh), data, vd = _in_layout([lh, rh], [0, nv_lh], data, vd)
+        surfs = SurfaceSet([Adjacency(lh, H, E, fl), Adjacency(rh, H, E, fr)], [0, nv_lh])
+        suffix, fmt, vertex = "maxVertex", "%.4f", True
+    else:
+        (csr,), data, _ = _in_layout([to_csr(adjac)], [0], data)
+        surfs = SurfaceSet([Adjacency(csr, H, E)], [0])
+        suffix, fmt, vertex = "maxVoxel", "%1.4f", False
+    model = PermutedOLS(data, X, contrasts=list(range(1, numcon + 1)), precision=precision)
+    names_t = ["Tstat_con%d" % (j + 1) for j in range(numcon)] if want_t else []
+    names_f = ["Fstat_%s" % varnames[b] for b in range(len(blocks))] if want_f else []
+    nulls = {name: [] for name in names_t + names_f}
+
+    def maxima(stat, signs):
+        if not vertex:
+            stat = torch.where(torch.isfinite(stat), stat, torch.zeros_like(stat))
+        m = surfs.batch(stat, signs, vdensity=vd)
+        return (m.max(0).values if vertex else m[0]).cpu().numpy()
+
+    items = 2 * (2 if vertex else 1)
+    for _, pb in _batches(perms, batch, items):
+        if want_t:
+            t = model.t(pb)
+            for j, name in enumerate(names_t):
+                nulls[name].append(maxima(t[j], 3))
+        if want_f:
+            f = model.partial_f(pb, blocks)
+            for b, name in enumerate(names_f):
+                nulls[name].append(maxima(f[b], 1)[:, 0])
+    nulls = {name: np.concatenate(v, 0) for name, v in nulls.items()}
+    if out_dir is not None:
+        for name, v in nulls.items():
+            _write(os.path.join(out_dir, "perm_%s_TFCE_%s.csv" % (name, suffix)), v, fmt)
+    return nulls
 
 
 def mmr(merge_y, pred_x, surfaces, offsets, settings, perms, groups=None, merged=True, zero_in_image=None,

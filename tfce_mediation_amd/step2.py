@@ -15,7 +15,11 @@ null CSVs where the reference writes them.
   and appends to ``output_med_<surface>/perm_SobelZ_<medtype>/``;
 * ``voxel_mediation``    voxel_tfce_mediation_randomise.py:42-76 reads
   ``python_temp/`` (STEP_1_voxel_tfce_mediation.py:90-95) and appends to
-  ``output_med_<medtype>/perm_SobelZ/``.
+  ``output_med_<medtype>/perm_SobelZ/``;
+* ``tm_glm``             the GLM branch of tm_models_randomise.py:68-243 reads
+  ``tmtemp_GLM_<surface>/`` or ``tmtemp_GLM_volume/`` (saved by
+  STEP_1_tm_models.py:156-175, 664-697) and appends to
+  ``output_GLM_<surface>/perm_GLM/`` or ``output_GLM_volume/perm_GLM/``.
 
 The permutations of permutation numbers ``start..stop`` come from
 ``randomise.permutation_indices`` (numpy's legacy RNG seeded with number +
@@ -117,3 +121,37 @@ def voxel_mediation(medtype, start, stop, root=".", seed=0, allow_pickle=False, 
     out = _out_dir(root, "output_med_%s" % medtype, "perm_SobelZ")
     return randomise.voxel_mediation(v["raw_nonzero_corr"].T, v["pred_x"], v["depend_y"], medtype, v["adjac"],
                                      v["optstfce"], perms, out_dir=out, batch=batch, precision=precision)
+
+
+def tm_glm(start, stop, surface=None, root=".", exchangeblock=None, seed=0, allow_pickle=False, batch=None,
+           precision="fp16x3"):
+    """Permutation numbers start..stop (inclusive) of the tm-models GLM saved
+    for ``surface`` (None: the volume); returns {statname: null}."""
+    name = surface if surface is not None else "volume"
+    tmp = os.path.join(root, "tmtemp_GLM_%s" % name)
+    common = ["exog_flat", "exog_shape", "varnames", "gstat", "data", "optstfce", "dmy_covariates"]
+    if surface is not None:
+        v = load_temp(tmp, common + ["mask_lh", "mask_rh", "adjac_lh", "adjac_rh", "vdensity_lh", "vdensity_rh"],
+                      allow_pickle)
+    else:
+        v = load_temp(tmp, common + ["adjac"], allow_pickle)
+    exog, count = [], 0
+    for nc in np.atleast_1d(v["exog_shape"]):
+        exog.append(v["exog_flat"][:, count:count + int(nc)])
+        count += int(nc)
+    cov = v["dmy_covariates"]
+    if cov.dtype == object and cov.ndim == 0 and cov.item() is None:
+        cov = None  # np.save of None (tm_models_randomise.py:174-175)
+    data = v["data"]
+    blocks = None
+    if exchangeblock is not None:
+        blocks = np.genfromtxt(exchangeblock, dtype=str)
+    perms = _perms(data.shape[0], range(int(start), int(stop) + 1), seed, blocks)
+    out = _out_dir(root, "output_GLM_%s" % name, "perm_GLM")
+    varnames = [str(x) for x in np.atleast_1d(v["varnames"])]
+    kw = dict(varnames=varnames, out_dir=out, batch=batch, precision=precision)
+    if surface is not None:
+        return randomise.tm_glm(data, exog, cov, str(v["gstat"]), v["optstfce"], perms, adjac_lh=v["adjac_lh"],
+                                adjac_rh=v["adjac_rh"], mask_lh=v["mask_lh"], mask_rh=v["mask_rh"],
+                                vdensity_lh=v["vdensity_lh"], vdensity_rh=v["vdensity_rh"], **kw)
+    return randomise.tm_glm(data, exog, cov, str(v["gstat"]), v["optstfce"], perms, adjac=v["adjac"], **kw)

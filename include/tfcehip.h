@@ -77,6 +77,26 @@ int tfce_adj_destroy(tfce_adj_t *adj);
 int tfce_locality_order(const int32_t *row_ptr, const int32_t *col, int32_t V, int32_t *new_of_old);
 int tfce_adj_info(const tfce_adj_t *adj, int32_t *V, int64_t *nnz, int32_t *symmetric);
 
+/* Replaces create_adjac_voxel(data_index, data_mask, num_voxel, dirtype)
+ * (pyfunc.py:48-76) on the GPU, as CSR: d_mask [X][Y][Z] uint8 in C order
+ * (nonzero = in the mask); the elements are the in-mask voxels in C order
+ * (np.where order); row v lists the labels of its in-mask neighbours in the
+ * 3x3x3 window (dirtype 26) or its six face neighbours (dirtype 6), ascending,
+ * with the reference's rules: label 0 is never a neighbour, row 0 is empty, and
+ * with dirtype 6 a voxel whose window is clipped by the volume edge has none.
+ * With d_row_ptr and d_col NULL the call only writes *V and *nnz; then, with
+ * d_row_ptr [V+1] and d_col [nnz], it fills them.  Blocking. */
+int tfce_adj_voxel_csr(const uint8_t *d_mask, int32_t X, int32_t Y, int32_t Z, int32_t dirtype,
+                       int32_t *d_row_ptr, int32_t *d_col, int32_t *V, int64_t *nnz, hipStream_t stream);
+
+/* Replaces create_adjac_vertex(vertices, faces) (pyfunc.py:37-46) on the GPU,
+ * as CSR: d_faces [F][3] int32; row v lists, ascending and once each, every
+ * vertex that shares a face with v (v itself when a face repeats it, as the
+ * reference's set does).  Same two-call protocol as tfce_adj_voxel_csr;
+ * TFCE_EADJ if a face names a vertex outside [0, n_vertices).  Blocking. */
+int tfce_adj_mesh_csr(const int32_t *d_faces, int64_t F, int32_t n_vertices, int32_t *d_row_ptr,
+                      int32_t *d_col, int64_t *nnz, hipStream_t stream);
+
 /* Replaces CreateAdjSet.run(image, enhn) (tfce.pyx:44-45 -> fast_tfce.hpp:11-95):
  * enhn += TFCE(image) over V float32 elements (accumulate != 0), or
  * enhn = TFCE(image) (accumulate == 0).  dh == 0 gives the reference's step

@@ -203,3 +203,15 @@ def test_vertex_adjacency_hand_derived_octahedron():
     assert [sorted(s) for s in graphs.create_adjac_vertex(v, f)] == want
     rp, col = adjacency.faces_to_csr(f, 6)
     assert [col[rp[i]:rp[i + 1]].tolist() for i in range(6)] == want
+
+
+def test_vertex_adjacency_degenerate_face_keeps_the_repeated_vertex():
+    """pyfunc.py:37-46 adds faces[i, 1] to faces[i, 0]'s set even when they are
+    the same vertex: a face (3, 3, 4) puts 3 in its own set.  Vertices no face
+    names keep empty rows."""
+    v = np.zeros((6, 3))
+    f = np.array([[0, 1, 2], [3, 3, 4], [2, 1, 0]])
+    ref = graphs.create_adjac_vertex(v, f)
+    assert sorted(ref[3]) == [3, 4] and sorted(ref[4]) == [3] and ref[5] == set()
+    rp, col = adjacency.faces_to_csr(f, 6)
+    assert [col[rp[i]:rp[i + 1]].tolist() for i in range(6)] == [sorted(s) for s in ref]

@@ -1,7 +1,8 @@
 """Adjacency builders and CSR conversion for the TFCE operator.
 
 * ``create_adjac_vertex`` - triangle-edge adjacency, as ``pyfunc.py:37-46``
-  (each face contributes its three undirected edges).
+  (each face adds its other two vertices to each of its vertices' sets, so a
+  face that repeats a vertex puts that vertex in its own set).
 * ``create_adjac_voxel`` - 6/26-neighbourhood adjacency of a voxel mask with the
   reference's quirks (``pyfunc.py:48-76``): the first in-mask voxel (label 0) is
   nobody's neighbour and has an empty list; with 6-connectivity a voxel whose
@@ -13,7 +14,10 @@
 * ``vertex_density`` - the degree-based weight of
   ``STEP_1_vertex_tfce_multiple_regression.py:165-173`` and
   ``tm_multimodality_multisurface_regression.py:449-462``.
-All of these are vectorised host code run once per analysis.
+All of these are vectorised host code run once per analysis;
+``voxel_csr_device`` and ``faces_to_csr_device`` build the same CSR on the GPU
+(``tfce_adj_voxel_csr`` / ``tfce_adj_mesh_csr``) from a device-resident mask or
+face list.
 """
 import numpy as np
 
@@ -23,8 +27,6 @@ def faces_to_csr(faces, n_vertices):
     a, b, c = faces[:, 0], faces[:, 1], faces[:, 2]
     src = np.concatenate([a, a, b, b, c, c])
     dst = np.concatenate([b, c, a, c, a, b])
-    keep = src != dst
-    src, dst = src[keep], dst[keep]
     key = np.unique(src * n_vertices + dst)
     src = (key // n_vertices).astype(np.int64)
     dst = (key % n_vertices).astype(np.int32)
@@ -216,3 +218,42 @@ def locality_layout(surfaces, offsets, width):
         out.append(csr)
         cols[off:off + len(old_of_new)] = off + old_of_new
     return out, cols
+
+
+def voxel_csr_device(mask, dirtype=26, stream=None):
+    """``voxel_csr`` on the GPU: (row_ptr, col) int32 device tensors of the
+    in-mask voxels of ``mask`` (numpy or torch, any dtype; nonzero = in)."""
+    import ctypes
+    import torch
+    from . import _lib
+    m = torch.as_tensor(np.asarray(mask) != 0 if not isinstance(mask, torch.Tensor) else mask != 0)
+    m = m.to(device="cuda", dtype=torch.uint8).contiguous()
+    X, Y, Z = (int(d) for d in m.shape)
+    lib = _lib.lib()
+    V, nnz = ctypes.c_int32(), ctypes.c_int64()
+    _lib.check(lib.tfce_adj_voxel_csr(_lib.ptr(m), X, Y, Z, int(dirtype), None, None, ctypes.byref(V),
+                                      ctypes.byref(nnz), _lib.stream_ptr(stream)))
+    rp = torch.empty(V.value + 1, dtype=torch.int32, device="cuda")
+    col = torch.empty(max(nnz.value, 1), dtype=torch.int32, device="cuda")
+    _lib.check(lib.tfce_adj_voxel_csr(_lib.ptr(m), X, Y, Z, int(dirtype), _lib.ptr(rp), _lib.ptr(col),
+                                      ctypes.byref(V), ctypes.byref(nnz), _lib.stream_ptr(stream)))
+    return rp, col[:nnz.value]
+
+
+def faces_to_csr_device(faces, n_vertices, stream=None):
+    """``faces_to_csr`` on the GPU: (row_ptr, col) int32 device tensors."""
+    import ctypes
+    import torch
+    from . import _lib
+    f = torch.as_tensor(np.asarray(faces) if not isinstance(faces, torch.Tensor) else faces)
+    f = f.to(device="cuda", dtype=torch.int32).contiguous().reshape(-1, 3)
+    lib = _lib.lib()
+    nnz = ctypes.c_int64()
+    F = int(f.shape[0])
+    _lib.check(lib.tfce_adj_mesh_csr(_lib.ptr(f), F, int(n_vertices), None, None, ctypes.byref(nnz),
+                                     _lib.stream_ptr(stream)))
+    rp = torch.empty(int(n_vertices) + 1, dtype=torch.int32, device="cuda")
+    col = torch.empty(max(nnz.value, 1), dtype=torch.int32, device="cuda")
+    _lib.check(lib.tfce_adj_mesh_csr(_lib.ptr(f), F, int(n_vertices), _lib.ptr(rp), _lib.ptr(col),
+                                     ctypes.byref(nnz), _lib.stream_ptr(stream)))
+    return rp, col[:nnz.value]

@@ -181,3 +181,45 @@ def test_mmr_from_a_tmi_file(tmp_path):
                     exp = (ein[a:b] * (tin[a:b].max() / np.float32(100)) * vd[a:b]).max()
                     got = res[(s, c)][p, k]
                     assert abs(got - exp) <= 1e-4 * abs(exp) + 1e-6, (c, p, k, s, got, exp)
+
+
+def test_vertex_regression_with_exchange_blocks(tmp_path):
+    """-e: permutations within exchange blocks (pyfunc.rand_blocks,
+    pyfunc.py:2733-2757) read from the block file, end to end on the GPU; the
+    null equals the driver's on the same permutations, which keep every
+    subject inside its block, and matches the oracle loop."""
+    from tfce_mediation_amd import randomise, step2
+    say("[gpu] step2.vertex_regression with exchange blocks")
+    rp, col = synth.icosphere_csr(3)
+    V = len(rp) - 1
+    rng = np.random.default_rng(8)
+    m_lh = m_rh = np.ones(V, bool)
+    n = 24
+    merge_y = rng.standard_normal((n, 2 * V)).astype(np.float32)
+    pred_x = (rng.random(n) < 0.5).astype(float)[:, None]
+    blocks = np.repeat(np.array(["a", "b", "c"]), 8)
+    np.savetxt(os.path.join(tmp_path, "blocks.csv"), blocks, fmt="%s")
+    save(os.path.join(tmp_path, "python_temp_thick"), pred_x=pred_x, num_subjects=n, all_vertex=V,
+         num_vertex=2 * V, num_vertex_lh=V, num_vertex_rh=V, bin_mask_lh=m_lh, bin_mask_rh=m_rh,
+         adjac_lh=ragged(rp, col), adjac_rh=ragged(rp, col), merge_y=merge_y, optstfce=np.array([2.0, 1.0]),
+         vdensity_lh=np.ones(V, np.float32), vdensity_rh=np.ones(V, np.float32))
+    got = step2.vertex_regression("thick", 0, 5, root=str(tmp_path), seed=4, allow_pickle=True,
+                                  exchangeblock=os.path.join(tmp_path, "blocks.csv"), precision="fp32")
+    perms = randomise.permutation_indices(n, range(0, 6), seed=4, blocks=blocks)
+    for perm in perms:
+        assert sorted(perm.tolist()) == list(range(n))
+        for lab in "abc":  # a block's positions take their subjects from one block
+            assert len(set(blocks[perm[blocks == lab]])) == 1
+    ref = randomise.vertex_regress(merge_y, pred_x, ragged(rp, col), ragged(rp, col), m_lh, m_rh, [2.0, 1.0],
+                                   perms, precision="fp32")
+    np.testing.assert_array_equal(got[1], ref[1])
+    X = np.column_stack([np.ones(n), pred_x])
+    for p, perm in enumerate(perms):
+        t = ools.perm_t(X, merge_y, [perm], [1])[0, 0].astype(np.float32)
+        for k, s in enumerate((1, -1)):
+            best = 0.0
+            for h in range(2):
+                img = (s * t[h * V:(h + 1) * V]).astype(np.float32)
+                enh, _, _ = otfce.tfce(rp, col, img, 2.0, 1.0, exact_pow=True)
+                best = max(best, float(enh.max() * (img.max() / np.float32(100))))
+            assert abs(got[1][p, k] - best) <= 1e-4 * abs(best), (p, k, got[1][p, k], best)

@@ -299,6 +299,9 @@ def main():
     ap.add_argument("--lds-sweep", type=int, default=None, choices=(0, 1),
                     help="surfaces of at most 13,248 elements in the LDS-resident sweep (1) or the global one (0) "
                          "instead of autotuning")
+    ap.add_argument("--pair-sweep", type=int, default=None, choices=(0, 1),
+                    help="both signs of a map in one workgroup over ring-packed surfaces (1) or one image per "
+                         "workgroup (0) instead of autotuning")
     ap.add_argument("--pipeline", action="store_true",
                     help="run the contraction of step s+1 on a second stream during the sweep of step s "
                          "(engine.pipeline) instead of one after the other on one stream")
@@ -334,11 +337,13 @@ def main():
 
     # ---- launch shapes picked on this workload (not cached: measured here) ----
     tuned = None
-    if args.tile is not None or args.wgs_per_cu is not None or args.lds_sweep is not None:
+    if (args.tile is not None or args.wgs_per_cu is not None or args.lds_sweep is not None
+            or args.pair_sweep is not None):
         from tfce_mediation_amd import autotune
         tuned = {"gemm_tile": 0 if args.tile is None else args.tile,
                  "tfce_wgs_per_cu": 4 if args.wgs_per_cu is None else args.wgs_per_cu,
-                 "lds_sweep": 1 if args.lds_sweep is None else args.lds_sweep}
+                 "lds_sweep": 1 if args.lds_sweep is None else args.lds_sweep,
+                 "pair_sweep": 1 if args.pair_sweep is None else args.pair_sweep}
         autotune.apply(tuned)
     elif not args.no_autotune:
         from tfce_mediation_amd import autotune
@@ -578,7 +583,8 @@ def main():
             "stat_rel_err_vs_fp64": t_err or None,
             "autotuned": None if tuned is None else {"gemm_tile": ["256x128", "512x64", "512x128"][tuned["gemm_tile"]],
                                                      "tfce_wgs_per_cu": tuned["tfce_wgs_per_cu"],
-                                                     "lds_sweep": tuned.get("lds_sweep", 1)},
+                                                     "lds_sweep": tuned.get("lds_sweep", 1),
+                                                     "pair_sweep": tuned.get("pair_sweep", 1)},
             "schedule": mode + (" (contraction of step s+1 on a second stream during the sweep of step s)"
                                 if args.pipeline else " (contraction then sweep, one stream)"),
             "contraction_ms_per_step": round(gemm_ms, 3),

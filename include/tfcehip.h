@@ -331,6 +331,13 @@ int tfce_set_occupancy(int32_t wgs_per_cu);
  * No counterpart in the reference. */
 int tfce_set_lds_sweep(int32_t enable);
 
+/* Batched maxima of both signs (sign_mask 3) over ring-packed surfaces (closed
+ * triangle meshes of at most six neighbours per vertex) run both signs of a map
+ * in one workgroup, sharing one pass over the map and the vertices, unless
+ * this process-wide knob is 0; the maxima are the same bit for bit either way.
+ * No counterpart in the reference. */
+int tfce_set_pair_sweep(int32_t enable);
+
 /* ---------------------------------------------------- learned early stop */
 /* No counterpart in the reference (nearest: the parametric null fits of
  * bin/tm_maths:532-552).  Tail model 32 -> 64 -> 64 -> 2, tanh, FP32, mapping

@@ -147,6 +147,84 @@ def test_run_accumulates_and_degenerate(torch):
     assert np.array_equal(enh, base)
 
 
+@pytest.mark.parametrize("gname,E", [("ico3", 1.0), ("ico4_shuffled", 1.0), ("ico5", 0.5), ("ico5_masked", 1.0),
+                                     ("ico4_w8", 1.0), ("vox26", 0.5)])
+def test_pair_sweep_matches_single_image_sweeps(torch, gname, E):
+    """Both signs of a map in one workgroup (ring-packed surfaces, MAX mode)
+    against the one-image-per-workgroup sweep (knob off), bit for bit, and
+    against the oracle; surfaces that are not ring-packed (ico4_w8, vox26)
+    must take the single-image sweep either way.  The LDS-resident sweep is
+    off so that these small surfaces reach the global-memory sweeps."""
+    from tfce_mediation_amd import _lib
+    from tfce_mediation_amd.tfce import Adjacency
+    row_ptr, col = GRAPHS[gname]()
+    V = len(row_ptr) - 1
+    P = 37
+    say(f"[gpu] paired vs single-image sweep {gname} V={V} P={P} E={E}")
+    maps = np.stack([smooth_map(row_ptr, col, 900 + p, shift=0.05 * (p % 3 - 1)) for p in range(P)])
+    maps[3] = np.abs(maps[3])   # the -image has nothing above 0
+    maps[4] = -np.abs(maps[4])  # the +image is degenerate
+    adj = Adjacency((row_ptr, col), 2.0, E)
+    d = torch.from_numpy(np.ascontiguousarray(maps, np.float32)).cuda()
+    lib = _lib.lib()
+    try:
+        _lib.check(lib.tfce_set_lds_sweep(0))
+        _lib.check(lib.tfce_set_pair_sweep(1))
+        pair = adj.batch_max(d, 3).cpu().numpy()
+        _lib.check(lib.tfce_set_pair_sweep(0))
+        single = adj.batch_max(d, 3).cpu().numpy()
+    finally:
+        _lib.check(lib.tfce_set_pair_sweep(1))
+        _lib.check(lib.tfce_set_lds_sweep(1))
+    np.testing.assert_array_equal(pair.view(np.uint32), single.view(np.uint32))
+    for p in (0, 3, 4, 17, 36):
+        for k, sgn in enumerate((1, -1)):
+            x = (sgn * maps[p]).astype(np.float32)
+            if x.max() <= 0:
+                assert pair[p, k] == 0
+                continue
+            enh, _, _ = otfce.tfce(row_ptr, col, x, 2.0, E, exact_pow=True)
+            exp = np.float32(enh.max() * np.float32(x.max() / np.float32(100)))
+            if E == 1.0:
+                assert pair[p, k] == exp, (p, k, pair[p, k], exp)
+            else:
+                assert ulp_diff(np.float32([pair[p, k]]), np.float32([exp])).max() <= 4
+
+
+def test_pair_sweep_two_hemispheres_and_shared_thresholds(torch):
+    """Two order-7 hemispheres in one set (the headline shape) and the
+    shared-threshold mode of the merged mmr: paired and single-image sweeps
+    give the same maxima bit for bit."""
+    from tfce_mediation_amd import _lib
+    from tfce_mediation_amd.tfce import Adjacency, SurfaceSet
+    row_ptr, col = synth.icosphere_csr(7)
+    V = len(row_ptr) - 1
+    P = 8
+    say(f"[gpu] paired sweep, two hemispheres V={2 * V} P={P}")
+    maps = np.concatenate([np.stack([smooth_map(row_ptr, col, 40 + 2 * p + h, passes=5, shift=0.0)
+                                     for p in range(P)]) for h in range(2)], axis=1)
+    adj = Adjacency((row_ptr, col), 2.0, 1.0)
+    surfs = SurfaceSet([adj, adj], [0, V])
+    d = torch.from_numpy(np.ascontiguousarray(maps, np.float32)).cuda()
+    lib = _lib.lib()
+    outs = {}
+    try:
+        for on in (1, 0):
+            _lib.check(lib.tfce_set_pair_sweep(on))
+            outs[on] = (surfs.batch(d, 3).cpu().numpy(),
+                        surfs.batch(d, 3, shared_thresholds=True).cpu().numpy())
+    finally:
+        _lib.check(lib.tfce_set_pair_sweep(1))
+    for a_, b_ in zip(outs[1], outs[0]):
+        np.testing.assert_array_equal(a_.view(np.uint32), b_.view(np.uint32))
+    for p in (0, 5):
+        for h in range(2):
+            for k, sgn in enumerate((1, -1)):
+                x = (sgn * maps[p, h * V:(h + 1) * V]).astype(np.float32)
+                enh, _, _ = otfce.tfce(row_ptr, col, x, 2.0, 1.0, exact_pow=True)
+                assert outs[1][0][h, p, k] == np.float32(enh.max() * np.float32(x.max() / np.float32(100)))
+
+
 @pytest.mark.parametrize("gname", sorted(GRAPHS))
 def test_whole_gpu_single_image_matches_oracle(torch, gname):
     """tfce_run over the whole GPU (cooperative level sweep, forced on for

@@ -40,6 +40,7 @@ SIGNATURES = {
     "tfce_adj_voxel_csr": (c_int, [c_p, c_i32, c_i32, c_i32, c_i32, c_p, c_p, c_p, c_p, c_p]),
     "tfce_adj_mesh_csr": (c_int, [c_p, c_i64, c_i32, c_p, c_p, c_p, c_p]),
     "tfce_set_lds_sweep": (c_int, [c_i32]),
+    "tfce_set_pair_sweep": (c_int, [c_i32]),
     "tfce_batch_ex": (c_int, [c_p, c_p, c_i64, c_int, c_int, c_u32, c_p, c_p, c_p, c_p, c_p, c_p]),
     "tfce_ols_data_create": (c_int, [c_p, c_i32, c_i64, c_i64, c_i32, c_i32, c_p, c_p]),
     "tfce_ols_data_destroy": (c_int, [c_p]),

@@ -7,8 +7,9 @@
   slot's scratch in cache;
 * whether surfaces of at most 13,248 elements take the LDS-resident sweep
   (``tfce_set_lds_sweep``; one workgroup per compute unit holding both signs
-  of a map) or the global-memory sweep; both must give the same maxima bit
-  for bit, and the faster is kept.
+  of a map) or the global-memory sweep, and whether ring-packed surfaces
+  sweep both signs of a map in one workgroup (``tfce_set_pair_sweep``); each
+  alternative must give the same maxima bit for bit, and the faster is kept.
 
 Each candidate is timed with HIP events on the real workload (one batch of P
 permutations of an engine), the contraction and the sweep separately since
@@ -69,6 +70,7 @@ def apply(cfg):
     _lib.check(lib.tfce_ols_set_tile(int(cfg["gemm_tile"])))
     _lib.check(lib.tfce_set_occupancy(int(cfg["tfce_wgs_per_cu"])))
     _lib.check(lib.tfce_set_lds_sweep(int(cfg.get("lds_sweep", 1))))
+    _lib.check(lib.tfce_set_pair_sweep(int(cfg.get("pair_sweep", 1))))
 
 
 def tune(engine, P=256, reps=3, tiles=TILES, occupancies=OCCUPANCIES, cache=DEFAULT_CACHE, seed=0, log=None):
@@ -112,8 +114,20 @@ def tune(engine, P=256, reps=3, tiles=TILES, occupancies=OCCUPANCIES, cache=DEFA
     def sweep():
         return engine.surfaces.batch(t, signs, vdensity=engine.vdensity)
     _lib.check(lib.tfce_set_lds_sweep(0))
+    _lib.check(lib.tfce_set_pair_sweep(0))
     _lib.check(lib.tfce_set_occupancy(occupancies[0]))
     ref_null = sweep().cpu().numpy()
+    _lib.check(lib.tfce_set_pair_sweep(1))
+    pair_ok = bool(np.array_equal(sweep().cpu().numpy(), ref_null, equal_nan=True))
+    if not pair_ok and log:
+        log("[autotune] the paired sweep changes the maxima; dropped")
+    pair = 0
+    if pair_ok:
+        tp = _time_candidates([0, 1], lambda c: _lib.check(lib.tfce_set_pair_sweep(c)), sweep, reps)
+        timings["tfce"]["pair"] = tp[1]
+        timings["tfce"]["single"] = tp[0]
+        pair = 1 if tp[1] < tp[0] else 0
+    _lib.check(lib.tfce_set_pair_sweep(pair))
     _lib.check(lib.tfce_set_lds_sweep(1))
     lds_ok = bool(np.array_equal(sweep().cpu().numpy(), ref_null, equal_nan=True))
     if not lds_ok and log:
@@ -131,11 +145,12 @@ def tune(engine, P=256, reps=3, tiles=TILES, occupancies=OCCUPANCIES, cache=DEFA
         timings["tfce"]["lds"] = tl[1]
         lds = 1 if tl[1] < tl[0] else 0
     _lib.check(lib.tfce_set_lds_sweep(lds))
-    cfg = {"gemm_tile": int(best_tile), "tfce_wgs_per_cu": int(best_occ), "lds_sweep": lds,
+    cfg = {"gemm_tile": int(best_tile), "tfce_wgs_per_cu": int(best_occ), "lds_sweep": lds, "pair_sweep": pair,
            "timings_ms": {k: {str(a): round(b, 3) for a, b in v.items()} for k, v in timings.items()}}
     apply(cfg)
     if log:
-        log(f"[autotune] {key}: tile {best_tile}, {best_occ} sweep workgroups per CU, LDS sweep {lds}; "
+        log(f"[autotune] {key}: tile {best_tile}, {best_occ} sweep workgroups per CU, LDS sweep {lds}, "
+            f"paired sweep {pair}; "
             f"{cfg['timings_ms']}")
     if cache:
         try:

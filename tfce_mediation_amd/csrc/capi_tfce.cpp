@@ -54,6 +54,7 @@ static std::mutex g_ws_mutex;
 static std::map<std::pair<int, hipStream_t>, Workspace> g_ws;
 static int g_wgs_per_cu = 4;
 static bool g_lds_sweep = true;  // small surfaces in the LDS-resident sweep
+static bool g_pair_sweep = true;  // both signs of a map in one workgroup (ring-packed MAX sweeps)
 // tfce_run on one image: -1 the whole-GPU sweep for symmetric surfaces of at
 // least kCoopMinV elements, 0 never, 1 for every symmetric surface
 static int g_coop_mode = -1;
@@ -175,6 +176,11 @@ const char* tfce_version(void) { return "tfcehip 0.1 gfx950"; }
 
 int tfce_set_lds_sweep(int32_t enable) {
   g_lds_sweep = enable != 0;
+  return 0;
+}
+
+int tfce_set_pair_sweep(int32_t enable) {
+  g_pair_sweep = enable != 0;
   return 0;
 }
 
@@ -568,7 +574,14 @@ static int run_batch(const SurfDev* d_surfs, int32_t n_surf, int32_t v_max, int3
   // the ring-only sweep kernel when every surface the global sweep takes is
   // ring-packed: all of them, or those left after the LDS-resident ones
   const bool ring_only = n_lds_items > 0 ? rest_ring : all_ring;
-  hipError_t e = launch_tfce_batch(a, full, ring_only, grid, n_global, stream);
+  // both signs of a map in one workgroup: MAX mode over ring-packed surfaces,
+  // thresholds >= 0 (the two images' active sets are then disjoint); two
+  // 512-thread workgroups per compute unit, within the slots allocated above
+  int pair_grid = 0;
+  if (g_pair_sweep && !full && n_signs == 2 && ring_only && min_t >= 0.0f && n_global >= 2)
+    pair_grid = (int)std::max<int64_t>(1, std::min<int64_t>({n_global / 2, (int64_t)grid,
+                                                              2 * (int64_t)cu_count(current_device())}));
+  hipError_t e = launch_tfce_batch(a, full, ring_only, grid, n_global, stream, pair_grid);
   if (e != hipSuccess) return set_error(-(int)e, std::string("tfce launch: ") + hipGetErrorString(e));
   if (n_lds_items > 0) {
     // one workgroup per compute unit, each taking both signs of a map at a time

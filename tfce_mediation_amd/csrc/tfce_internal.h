@@ -129,8 +129,11 @@ hipError_t launch_tfce_coop(const TfceCoopArgs& a, hipStream_t stream);
 
 // Clears the work counters, writes every item's record (a.n_items) and runs
 // the global-memory sweep over items [0, n_global) (none when n_global <= 0).
+// pair_grid > 0: the paired sweep (MAX mode, ring-packed surfaces, both signs
+// of every map, items in (surface, map, sign) order, thresholds >= 0) with
+// that many 512-thread workgroups, each with its own slot, instead.
 hipError_t launch_tfce_batch(const TfceBatchArgs& a, bool full, bool ring_only, int grid, int64_t n_global,
-                             hipStream_t stream);
+                             hipStream_t stream, int pair_grid = 0);
 // The LDS-resident MAX sweep over items [item0, a.n_items) (symmetric surfaces
 // of at most kLdsMaxV elements with a neighbour table; item0 a multiple of
 // a.n_signs: one workgroup takes both signs of a map); its work counter is the

@@ -1092,6 +1092,332 @@ __global__ __launch_bounds__(kTfceThreads, FULL ? 3 : 4) void tfce_batch_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// Paired sweep (MAX mode, ring-packed surfaces, both signs of every map): one
+// workgroup takes a map's +image and -image together.  The elements active
+// in the two are disjoint (x > T >= 0 against -x > T >= 0), so they share one
+// set of per-element arrays: positions of the + image's elements first (by
+// level), then the - image's; a neighbour-position row keeps only neighbours
+// of its own element's sign, so the union-find of the two images never
+// meets.  One pass over the map assigns both images' levels and positions and
+// one pass over the vertices writes both images' rows (the separate sweeps
+// read the map and walk the vertices once per image); the levels of the two
+// run side by side, each with its own thresholds.  Same results as
+// tfce_batch_kernel<false, true>, bit for bit.
+constexpr int kPairThreads = 512;
+
+__global__ __launch_bounds__(kPairThreads, 2) void tfce_pair_kernel(TfceBatchArgs a) {
+  __shared__ float s_T[2][kMaxLevels];
+  __shared__ float s_HH[2][kMaxLevels];
+  __shared__ int s_start[2][kMaxLevels + 1];
+  __shared__ int s_cur[2][kMaxLevels];
+  __shared__ float s_red[2][kPairThreads / 64];
+  __shared__ int s_unit_lo, s_unit_hi, s_nroots_new;
+
+  const int tid = threadIdx.x;
+  const int64_t vstride = (a.v_max + 63) & ~63;
+  int32_t* slot = a.scratch + (int64_t)blockIdx.x * a.slot_stride;
+  int32_t* pos = slot;
+  int32_t* parent = slot + 2 * vstride;
+  int32_t* size = slot + 3 * vstride;
+  float* val = reinterpret_cast<float*>(slot + 4 * vstride);
+  int32_t* roots[2] = {slot + 5 * vstride, slot + 6 * vstride};
+  int32_t* npos = slot + 8 * vstride;  // [position][4] ring-packed neighbour positions
+  const int64_t n_units = a.n_items / 2;
+
+  while (true) {
+    if (tid == 0) {
+      unsigned long long it = atomicAdd(reinterpret_cast<unsigned long long*>(a.queue), 1ull);
+      s_unit_lo = (int)(it & 0x7fffffff);
+      s_unit_hi = (int)(it >> 31);
+    }
+    __syncthreads();
+    const int64_t unit = ((int64_t)s_unit_hi << 31) | s_unit_lo;
+    if (unit >= n_units) break;  // uniform across the workgroup
+
+    const int64_t item0 = 2 * unit;  // +image; item0 + 1 is the -image
+    const ItemRef itr = decode_item(a, item0);
+    const SurfDev S = a.surfs[itr.s];
+    const int V = S.V;
+    const float* x = a.maps + (int64_t)itr.p * a.map_stride + S.offset;
+    int L[2], status[2];
+    float T0[2], dT[2], own[2];
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+      const int32_t* rec = a.prep + (item0 + g) * kPrepWords;
+      status[g] = rec[kPrepL];
+      L[g] = status[g] > 0 ? status[g] : 0;
+      T0[g] = __int_as_float(rec[kPrepT0]);
+      dT[g] = __int_as_float(rec[kPrepDT]);
+      own[g] = __int_as_float(rec[kPrepOwn]);
+      for (int l = tid; l < L[g]; l += kPairThreads) {
+        s_T[g][l] = __int_as_float(rec[kPrepT + l]);
+        s_HH[g][l] = __int_as_float(rec[kPrepT + kMaxLevels + l]);
+        s_cur[g][l] = 0;
+      }
+    }
+    __syncthreads();
+    // ---- levels of both images from one read of the map -----------------------
+    auto level2 = [&](float xv, int& g) -> int {
+      int lv = kNever;
+      g = 0;
+      if (L[0] > 0) lv = level_of(xv, s_T[0], L[0], T0[0], dT[0]);
+      if (lv == kNever && L[1] > 0) {
+        lv = level_of(-xv, s_T[1], L[1], T0[1], dT[1]);
+        g = 1;
+      }
+      return lv;
+    };
+    for (int v0 = tid; v0 < V; v0 += 8 * kPairThreads) {
+      float xs[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int v = v0 + j * kPairThreads;
+        xs[j] = v < V ? x[v] : 0.0f;
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int v = v0 + j * kPairThreads;
+        if (v < V) {
+          int g;
+          const int lev = level2(xs[j], g);
+          if (lev != kNever) atomicAdd(&s_cur[g][lev], 1);
+        }
+      }
+    }
+    __syncthreads();
+    // exclusive scans of the two histograms (waves 0 and 1), the - image's
+    // positions after all of the + image's
+    if (tid < 128) {
+      const int g = tid >> 6, lane = tid & 63;
+      const int Lg = L[g];
+      const int per = (Lg + 63) / 64;
+      const int b = lane * per, e = min(Lg, b + per);
+      int sum = 0;
+      for (int l = b; l < e; ++l) sum += s_cur[g][l];
+      const int incl = wave_incl_scan(sum);
+      int run = incl - sum;
+      for (int l = b; l < e; ++l) {
+        const int c = s_cur[g][l];
+        s_start[g][l] = run;
+        s_cur[g][l] = run;
+        run += c;
+      }
+      if (lane == 63) s_start[g][Lg] = incl;
+    }
+    __syncthreads();
+    const int A0 = s_start[0][L[0]];  // the + image's active elements
+    if (tid < 64) {
+      for (int l = tid; l <= L[1]; l += 64) {
+        s_start[1][l] += A0;
+        if (l < L[1]) s_cur[1][l] += A0;
+      }
+    }
+    __syncthreads();
+    for (int v0 = tid; v0 < V; v0 += 8 * kPairThreads) {
+      float xs[8];
+      int lv[8], gs[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int v = v0 + j * kPairThreads;
+        xs[j] = v < V ? x[v] : 0.0f;
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int v = v0 + j * kPairThreads;
+        lv[j] = v < V ? level2(xs[j], gs[j]) : kNever;
+        if (v < V && lv[j] == kNever) pos[v] = kNever;
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (lv[j] != kNever) pos[v0 + j * kPairThreads] = atomicAdd(&s_cur[gs[j]][lv[j]], 1);
+    }
+    if (tid == 0) s_nroots_new = 0;
+    __syncthreads();
+    // ---- one row of same-sign neighbour positions per active element ----------
+    {
+      const int4* tab = reinterpret_cast<const int4*>(S.nadj);
+      for (int v0 = tid; v0 < V; v0 += 4 * kPairThreads) {
+        int qv[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int v = v0 + j * kPairThreads;
+          qv[j] = v < V ? pos[v] : kNever;
+        }
+        int4 tw[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) tw[t] = qv[t] != kNever ? tab[v0 + t * kPairThreads] : make_int4(-1, -1, -1, -1);
+        int q[4][kRingMaxDeg];
+        unsigned tail[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          int u[kRingMaxDeg];
+          ring_fields(tw[t], u, tail[t]);
+          const bool neg = qv[t] >= A0;
+#pragma unroll
+          for (int j = 0; j < kRingMaxDeg; ++j) {
+            const int qu = u[j] != kRingNone ? pos[u[j]] : kNever;
+            q[t][j] = (qu != kNever && (qu >= A0) == neg) ? qu : kNever;
+          }
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+          if (qv[t] != kNever) reinterpret_cast<int4*>(npos)[qv[t]] = ring_pack(q[t], tail[t]);
+      }
+    }
+    __syncthreads();
+    // ---- the two threshold sweeps side by side ---------------------------------
+    const int Lmax = max(L[0], L[1]);
+    int n_roots = 0;
+    int cur = 0;
+    const int4* rows = reinterpret_cast<const int4*>(npos);
+    const int4 none4 = make_int4(-1, -1, -1, -1);
+    for (int l = 0; l < Lmax; ++l) {
+      // this level's elements of each image: [b0[g], b1[g])
+      int b0[2], b1[2];
+#pragma unroll
+      for (int g = 0; g < 2; ++g) {
+        b0[g] = l < L[g] ? s_start[g][l] : 0;
+        b1[g] = l < L[g] ? s_start[g][l + 1] : 0;
+      }
+      const int n0 = b1[0] - b0[0], n_new = n0 + (b1[1] - b0[1]);
+      int32_t* rold = roots[cur];
+      int32_t* rnew = roots[cur ^ 1];
+      if (n_new > 0) {
+        for (int k = tid; k < n_new; k += kPairThreads) {
+          const int i = k < n0 ? b0[0] + k : b0[1] + (k - n0);
+          parent[i] = i;
+          size[i] = 1;
+        }
+        __syncthreads();
+        // join this level's elements with their joined neighbours
+        auto elem = [&](int k, int& bb) -> int {
+          bb = k < n0 ? b0[0] : b0[1];
+          return k < n0 ? b0[0] + k : b0[1] + (k - n0);
+        };
+        int bb_n;
+        int4 w_n = tid < n_new ? rows[elem(tid, bb_n)] : none4;
+        for (int k = tid; k < n_new; k += kPairThreads) {
+          int bb;
+          const int i = elem(k, bb);
+          const int4 w = w_n;
+          if (k + kPairThreads < n_new) w_n = rows[elem(k + kPairThreads, bb_n)];
+          int p[8], r[9];
+          unsigned adj[8];
+          ring_unpack(w, p, adj);
+          prune8(p, adj, i, bb, r);
+          join_roots<false>(parent, size, r, i, bb);
+        }
+        __syncthreads();
+        // merge sizes and values into surviving roots; collect roots (as the
+        // single-image sweep's merge step, over both images' candidates)
+        const int n_cand = n_roots + n_new;
+        for (int k0 = 0; k0 < n_cand; k0 += kIlp * kPairThreads) {
+          int c[kIlp], r[kIlp], sz[kIlp];
+          bool live[kIlp], fresh[kIlp], surv[kIlp], need[kIlp];
+#pragma unroll
+          for (int j = 0; j < kIlp; ++j) {
+            const int k = k0 + j * kPairThreads + tid;
+            live[j] = k < n_cand;
+            fresh[j] = live[j] && k >= n_roots;
+            int bb;
+            c[j] = live[j] ? (fresh[j] ? elem(k - n_roots, bb) : rold[k]) : 0;
+          }
+#pragma unroll
+          for (int j = 0; j < kIlp; ++j) {
+            r[j] = fresh[j] ? ld(parent + c[j]) : c[j];
+            sz[j] = fresh[j] ? ld(size + c[j]) : 0;
+          }
+#pragma unroll
+          for (int j = 0; j < kIlp; ++j) {
+            surv[j] = fresh[j] && r[j] == c[j];
+            need[j] = live[j] && (!fresh[j] || (r[j] != c[j] && sz[j] > 1));
+          }
+          find_batch(parent, r, need);
+#pragma unroll
+          for (int j = 0; j < kIlp; ++j) {
+            if (live[j] && !fresh[j]) {
+              if (r[j] != c[j]) {
+                __hip_atomic_fetch_add(size + r[j], ld(size + c[j]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_fetch_max(reinterpret_cast<int32_t*>(val) + r[j],
+                                       ld(reinterpret_cast<const int32_t*>(val) + c[j]), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+              } else {
+                surv[j] = true;
+              }
+            } else if (need[j]) {
+              __hip_atomic_fetch_add(size + r[j], sz[j] - 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+            wave_append(surv[j], c[j], rnew, &s_nroots_new);
+          }
+        }
+        __syncthreads();
+        n_roots = s_nroots_new;
+        __syncthreads();
+        if (tid == 0) s_nroots_new = 0;
+        cur ^= 1;
+      }
+      int32_t* rl = roots[cur];
+      // level increment of every root of an image that still has levels; a
+      // root of this level's elements starts from 0
+      const float HH0 = l < L[0] ? s_HH[0][l] : 0.0f, HH1 = l < L[1] ? s_HH[1][l] : 0.0f;
+      for (int k0 = 0; k0 < n_roots; k0 += kIlp * kPairThreads) {
+        int r[kIlp], sz[kIlp];
+        float cv[kIlp];
+        bool go[kIlp];
+#pragma unroll
+        for (int j = 0; j < kIlp; ++j) {
+          const int k = k0 + j * kPairThreads + tid;
+          r[j] = k < n_roots ? rl[k] : -1;
+          const int g = r[j] >= A0 ? 1 : 0;
+          go[j] = r[j] >= 0 && l < L[g];
+        }
+#pragma unroll
+        for (int j = 0; j < kIlp; ++j) {
+          const int g = r[j] >= A0 ? 1 : 0;
+          sz[j] = go[j] ? ld(size + r[j]) : 0;
+          cv[j] = (go[j] && r[j] < b0[g]) ? __int_as_float(ld(reinterpret_cast<const int32_t*>(val) + r[j])) : 0.0f;
+        }
+#pragma unroll
+        for (int j = 0; j < kIlp; ++j)
+          if (go[j]) val[r[j]] = cv[j] + tfce_inc(sz[j], r[j] >= A0 ? HH1 : HH0, S.E, S.sizepow);
+      }
+      __syncthreads();
+    }
+
+    // ---- finalize both items ---------------------------------------------------
+    float best[2] = {0.0f, 0.0f};
+    {
+      int32_t* rl = roots[cur];
+      for (int k = tid; k < n_roots; k += kPairThreads) {
+        const int r = rl[k];
+        const float vr = __int_as_float(ld(reinterpret_cast<const int32_t*>(val) + r));
+        if (r < A0) best[0] = fmaxf(best[0], vr);
+        else best[1] = fmaxf(best[1], vr);
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+      float v = wave_max(best[g]);
+      if ((tid & 63) == 0) s_red[g][tid >> 6] = v;
+    }
+    __syncthreads();
+    if (tid < 2) {
+      const int g = tid;
+      float b = s_red[g][0];
+      for (int w = 1; w < kPairThreads / 64; ++w) b = fmaxf(b, s_red[g][w]);
+      const int64_t item = item0 + g;
+      const float smax = (S.flags & kSurfZeroInScale) ? fmaxf(own[g], 0.0f) : own[g];
+      const float sc = __fdiv_rn(smax, 100.0f);
+      a.raw_max[item] = b;
+      if (a.scaled_max) a.scaled_max[item * a.out_mul + a.out_add] = __fmul_rn(b, sc);
+      if (a.own_max) a.own_max[item] = own[g];
+      if (a.status) a.status[item] = status[g];
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // LDS-resident sweep (MAX mode) for symmetric surfaces of at most kLdsMaxV
 // elements.  One 1,024-thread workgroup per compute unit takes one map of a
 // surface with both of its signs: the elements active in +map and in -map
@@ -1771,13 +2097,15 @@ hipError_t launch_tfce_coop(const TfceCoopArgs& a, hipStream_t stream) {
 }
 
 hipError_t launch_tfce_batch(const TfceBatchArgs& a, bool full, bool ring_only, int grid, int64_t n_global,
-                             hipStream_t stream) {
+                             hipStream_t stream, int pair_grid) {
   hipLaunchKernelGGL(zero_queue_kernel, dim3(1), dim3(1), 0, stream, a.queue);
   hipLaunchKernelGGL(tfce_prepare_kernel, dim3((unsigned)a.n_items), dim3(kTfceThreads), 0, stream, a);
   if (n_global <= 0) return hipGetLastError();
   TfceBatchArgs g = a;
   g.n_items = n_global;
-  if (full && ring_only)
+  if (pair_grid > 0)
+    hipLaunchKernelGGL(tfce_pair_kernel, dim3(pair_grid), dim3(kPairThreads), 0, stream, g);
+  else if (full && ring_only)
     hipLaunchKernelGGL((tfce_batch_kernel<true, true>), dim3(grid), dim3(kTfceThreads), 0, stream, g);
   else if (full)
     hipLaunchKernelGGL((tfce_batch_kernel<true, false>), dim3(grid), dim3(kTfceThreads), 0, stream, g);

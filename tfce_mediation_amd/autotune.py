@@ -113,38 +113,32 @@ def tune(engine, P=256, reps=3, tiles=TILES, occupancies=OCCUPANCIES, cache=DEFA
 
     def sweep():
         return engine.surfaces.batch(t, signs, vdensity=engine.vdensity)
+    # the alternatives must reproduce the one-image-per-workgroup maxima
     _lib.check(lib.tfce_set_lds_sweep(0))
     _lib.check(lib.tfce_set_pair_sweep(0))
     _lib.check(lib.tfce_set_occupancy(occupancies[0]))
     ref_null = sweep().cpu().numpy()
-    _lib.check(lib.tfce_set_pair_sweep(1))
-    pair_ok = bool(np.array_equal(sweep().cpu().numpy(), ref_null, equal_nan=True))
-    if not pair_ok and log:
-        log("[autotune] the paired sweep changes the maxima; dropped")
-    pair = 0
-    if pair_ok:
-        tp = _time_candidates([0, 1], lambda c: _lib.check(lib.tfce_set_pair_sweep(c)), sweep, reps)
-        timings["tfce"]["pair"] = tp[1]
-        timings["tfce"]["single"] = tp[0]
-        pair = 1 if tp[1] < tp[0] else 0
-    _lib.check(lib.tfce_set_pair_sweep(pair))
-    _lib.check(lib.tfce_set_lds_sweep(1))
-    lds_ok = bool(np.array_equal(sweep().cpu().numpy(), ref_null, equal_nan=True))
-    if not lds_ok and log:
-        log("[autotune] the LDS sweep changes the maxima; dropped")
-    _lib.check(lib.tfce_set_lds_sweep(0))
+    ok = {}
+    for knob, fn in (("pair", lib.tfce_set_pair_sweep), ("lds", lib.tfce_set_lds_sweep)):
+        _lib.check(fn(1))
+        ok[knob] = bool(np.array_equal(sweep().cpu().numpy(), ref_null, equal_nan=True))
+        _lib.check(fn(0))
+        if not ok[knob] and log:
+            log(f"[autotune] the {knob} sweep changes the maxima; dropped")
+    # occupancy of the one-image sweep, then each alternative at that occupancy
     timings["tfce"] = _time_candidates(list(occupancies), lambda c: _lib.check(lib.tfce_set_occupancy(c)), sweep,
                                        reps)
     best_occ = min(timings["tfce"], key=timings["tfce"].get)
     _lib.check(lib.tfce_set_occupancy(best_occ))
-    lds = 0
-    if lds_ok:
-        # the LDS sweep against the global one at the chosen occupancy (which
-        # the global part of a mixed set keeps)
-        tl = _time_candidates([0, 1], lambda c: _lib.check(lib.tfce_set_lds_sweep(c)), sweep, reps)
-        timings["tfce"]["lds"] = tl[1]
-        lds = 1 if tl[1] < tl[0] else 0
-    _lib.check(lib.tfce_set_lds_sweep(lds))
+    chosen = {"pair": 0, "lds": 0}
+    for knob, fn in (("pair", lib.tfce_set_pair_sweep), ("lds", lib.tfce_set_lds_sweep)):
+        if ok[knob]:
+            tk = _time_candidates([0, 1], lambda c: _lib.check(fn(c)), sweep, reps)
+            timings["tfce"][knob + " off"] = tk[0]
+            timings["tfce"][knob + " on"] = tk[1]
+            chosen[knob] = 1 if tk[1] < tk[0] else 0
+        _lib.check(fn(chosen[knob]))
+    pair, lds = chosen["pair"], chosen["lds"]
     cfg = {"gemm_tile": int(best_tile), "tfce_wgs_per_cu": int(best_occ), "lds_sweep": lds, "pair_sweep": pair,
            "timings_ms": {k: {str(a): round(b, 3) for a, b in v.items()} for k, v in timings.items()}}
     apply(cfg)

@@ -293,7 +293,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default contraction tile and sweep occupancy")
     ap.add_argument("--tile", type=int, default=None,
-                    help="contraction tile (0: 256x128, 1: 512x64, 2: 512x128) instead of autotuning")
+                    help="contraction tile (0: 256x128, 1: 512x64, 2: 512x128, 3: 256x256) instead of autotuning")
     ap.add_argument("--wgs-per-cu", type=int, default=None,
                     help="sweep workgroups per CU instead of autotuning")
     ap.add_argument("--lds-sweep", type=int, default=None, choices=(0, 1),
@@ -581,7 +581,7 @@ def main():
             "fp16x3_perms_per_s": None if "fp16x3" not in alt_rate else round(alt_rate["fp16x3"], 1),
             "hipgraph_replay_perms_per_s": None if graph_rate is None else round(graph_rate, 1),
             "stat_rel_err_vs_fp64": t_err or None,
-            "autotuned": None if tuned is None else {"gemm_tile": ["256x128", "512x64", "512x128"][tuned["gemm_tile"]],
+            "autotuned": None if tuned is None else {"gemm_tile": ["256x128", "512x64", "512x128", "256x256"][tuned["gemm_tile"]],
                                                      "tfce_wgs_per_cu": tuned["tfce_wgs_per_cu"],
                                                      "lds_sweep": tuned.get("lds_sweep", 1),
                                                      "pair_sweep": tuned.get("pair_sweep", 1)},

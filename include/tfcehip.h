@@ -242,7 +242,7 @@ int tfce_total_ss(const float *d_Y, int32_t n, int64_t V, int64_t ldy, double *d
 
 /* Workgroup tile of the split-precision contraction (process-wide; the
  * autotuner's knob): 0 = 256 rows x 128 elements (default), 1 = 512 x 64,
- * 2 = 512 x 128.  No counterpart in the reference. */
+ * 2 = 512 x 128, 3 = 256 x 256.  No counterpart in the reference. */
 int tfce_ols_set_tile(int32_t tile);
 int tfce_ols_tile(void);
 

@@ -1,7 +1,8 @@
 """Autotuner for the launch-shape knobs of the batched path.
 
 * the workgroup tile of the split-precision contraction
-  (``tfce_ols_set_tile``: 256 x 128, 512 x 64 or 512 x 128 rows x elements);
+  (``tfce_ols_set_tile``: 256 x 128, 512 x 64, 512 x 128 or 256 x 256 rows x
+  elements);
 * the number of resident sweep workgroups per compute unit
   (``tfce_set_occupancy``): more hide more latency, fewer keep more of each
   slot's scratch in cache;
@@ -27,7 +28,7 @@ import numpy as np
 
 from . import _lib
 
-TILES = (0, 1, 2)
+TILES = (0, 1, 2, 3)
 OCCUPANCIES = (2, 3, 4, 5, 6, 8)
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "tfce_mediation_amd", "autotune.json")
 

@@ -403,7 +403,7 @@ int tfce_ols_model_destroy(tfce_ols_model_t* m) {
 static int g_gemm_tile = 0;
 
 int tfce_ols_set_tile(int32_t tile) {
-  if (tile < 0 || tile >= kGemmTiles) return set_error(TFCE_EINVAL, "tile must be 0, 1 or 2");
+  if (tile < 0 || tile >= kGemmTiles) return set_error(TFCE_EINVAL, "tile must be 0, 1, 2 or 3");
   g_gemm_tile = tile;
   return 0;
 }

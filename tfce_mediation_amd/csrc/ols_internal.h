@@ -159,10 +159,11 @@ hipError_t launch_ols_fixed(const OlsFixedArgs& a, hipStream_t s);
 hipError_t launch_ols_factors(const OlsFactorArgs& a, hipStream_t s);
 hipError_t launch_ols_build_a(const OlsBuildAArgs& a, hipStream_t s);
 // Workgroup tile of the split contraction: 0 = 256 x 128 (2 x 2 waves),
-// 1 = 512 x 64 (4 x 1 waves), 2 = 512 x 128 (4 x 2 waves).  FP32 uses 256 x 128.
-constexpr int kGemmTiles = 3;
-inline int gemm_tile_rows(int tile) { return tile == 0 ? 256 : 512; }
-inline int gemm_tile_cols(int tile) { return tile == 1 ? 64 : 128; }
+// 1 = 512 x 64 (4 x 1 waves), 2 = 512 x 128 (4 x 2 waves), 3 = 256 x 256
+// (2 x 4 waves).  FP32 uses 256 x 128.
+constexpr int kGemmTiles = 4;
+inline int gemm_tile_rows(int tile) { return (tile == 0 || tile == 3) ? 256 : 512; }
+inline int gemm_tile_cols(int tile) { return tile == 1 ? 64 : (tile == 3 ? 256 : 128); }
 hipError_t launch_ols_gemm(const OlsGemmArgs& a, int precision, int tile, hipStream_t s);
 
 }  // namespace tfcehip

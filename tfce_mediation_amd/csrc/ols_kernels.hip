@@ -841,6 +841,8 @@ hipError_t launch_ols_gemm(const OlsGemmArgs& a, int precision, int tile, hipStr
     launch_split<4, 1>(a, precision, blocks, s);
   else if (tile == 2)
     launch_split<4, 2>(a, precision, blocks, s);
+  else if (tile == 3)
+    launch_split<2, 4>(a, precision, blocks, s);
   else
     launch_split<2, 2>(a, precision, blocks, s);
   return hipGetLastError();

@@ -27,6 +27,12 @@
 //                     largest TFCE of a component is held by its earliest vertex,
 //                     so max over roots == max over vertices, bit for bit;
 //          FULL mode: add the increment of its root to every active vertex.
+//   tfce_pair_kernel    - the MAX sweep of ring-packed surfaces with both signs
+//     of a map in one workgroup (the two images' active sets are disjoint and
+//     share one slot), one pass over the map and the vertices for the two.
+//   tfce_lds_kernel     - the MAX sweep of small surfaces with the whole
+//     union-find of a map's two images in LDS.
+//   tfce_coop_kernel    - one image over the whole GPU (cooperative launch).
 // Every level touches only the new vertices and the current roots (MAX) or the
 // active prefix (FULL).  The sweep kernel carries no transcendental math (T^H
 // comes from the prepare kernel, s^E from a per-adjacency table) so it keeps

@@ -100,7 +100,10 @@ int tfce_adj_mesh_csr(const int32_t *d_faces, int64_t F, int32_t n_vertices, int
 /* Replaces CreateAdjSet.run(image, enhn) (tfce.pyx:44-45 -> fast_tfce.hpp:11-95):
  * enhn += TFCE(image) over V float32 elements (accumulate != 0), or
  * enhn = TFCE(image) (accumulate == 0).  dh == 0 gives the reference's step
- * fl(max/100); dh > 0 an explicit step.  minT is 0 as in tfce.pyx:45. */
+ * fl(max/100); dh > 0 an explicit step.  minT is 0 as in tfce.pyx:45.  A
+ * symmetric surface of 32,768 elements or more is swept over the whole GPU
+ * (a cooperative launch, levels in lock step), a smaller one by one
+ * workgroup; tfce_set_single_image_mode changes the choice, not the values. */
 int tfce_run(tfce_adj_t *adj, const float *d_image, float *d_enhn, float dh, int accumulate,
              hipStream_t stream);
 

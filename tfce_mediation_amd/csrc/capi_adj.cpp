@@ -12,11 +12,15 @@ using namespace tfcehip;
 
 namespace {
 
-// device buffers of one builder call, freed on every return path
+// device buffers of one builder call, freed on every return path once the
+// stream has drained (an error return may leave launches in flight)
 struct Buffers {
+  explicit Buffers(hipStream_t s) : stream(s) {}
+  hipStream_t stream;
   void* p[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   int n = 0;
   ~Buffers() {
+    if (n > 0) (void)hipStreamSynchronize(stream);
     for (int i = 0; i < n; ++i)
       if (p[i]) (void)hipFree(p[i]);
   }
@@ -58,7 +62,7 @@ int tfce_adj_voxel_csr(const uint8_t* d_mask, int32_t X, int32_t Y, int32_t Z, i
     return set_error(TFCE_EINVAL, "give both d_row_ptr and d_col (fill) or neither (sizes only)");
   const int64_t n = (int64_t)X * Y * Z;
   if (n >= INT32_MAX) return set_error(TFCE_EINVAL, "the volume holds 2^31 voxels or more");
-  Buffers b;
+  Buffers b(stream);
   int32_t *flags = nullptr, *label = nullptr, *rp = nullptr;
   int64_t* sums = nullptr;
   int rc = 0;
@@ -94,7 +98,7 @@ int tfce_adj_mesh_csr(const int32_t* d_faces, int64_t F, int32_t n_vertices, int
     return set_error(TFCE_EINVAL, "give both d_row_ptr and d_col (fill) or neither (sizes only)");
   if (6 * F >= INT32_MAX) return set_error(TFCE_EINVAL, "too many faces");
   const int32_t nv = n_vertices;
-  Buffers b;
+  Buffers b(stream);
   int32_t *cnt = nullptr, *off = nullptr, *tmp = nullptr, *bad = nullptr, *rp = nullptr;
   int64_t* sums = nullptr;
   const int64_t sw = scan_scratch_words(nv);

@@ -618,6 +618,20 @@ extern "C" {
 
 }  // extern "C"
 
+// Whether the current device takes cooperative launches (asked once per device).
+static bool coop_supported() {
+  static std::mutex mu;
+  static std::map<int, bool> known;
+  const int dev = current_device();
+  std::lock_guard<std::mutex> g(mu);
+  auto it = known.find(dev);
+  if (it != known.end()) return it->second;
+  int v = 0;
+  const bool ok = hipDeviceGetAttribute(&v, hipDeviceAttributeCooperativeLaunch, dev) == hipSuccess && v != 0;
+  known[dev] = ok;
+  return ok;
+}
+
 // One image over the whole GPU: its record, then the cooperative level sweep.
 static int run_coop(tfce_adj_t* adj, const float* d_image, float* d_enhn, float min_t, float dh,
                     int32_t* d_status, hipStream_t stream) {
@@ -690,9 +704,10 @@ int tfce_run_ex(tfce_adj_t* adj, const float* d_image, float* d_enhn, float min_
     if (e != hipSuccess) return set_error(-(int)e, "fill enhn");
   }
   // a large symmetric surface runs over the whole GPU (the labels debug
-  // output exists only in the one-workgroup sweep)
+  // output exists only in the one-workgroup sweep) where the device takes
+  // cooperative launches
   if ((adj->flags & kSurfSymmetric) && labels_level < 0 &&
-      (g_coop_mode == 1 || (g_coop_mode == -1 && adj->V >= kCoopMinV)))
+      (g_coop_mode == 1 || (g_coop_mode == -1 && adj->V >= kCoopMinV)) && coop_supported())
     return run_coop(adj, d_image, d_enhn, min_t, dh, d_status, stream);
   return run_batch(adj->d_surf, 1, adj->V, np_width(adj), d_image, adj->V, 1, 1, nullptr, nullptr, true, d_enhn, min_t,
                    dh, labels_level, d_labels, nullptr, nullptr, nullptr, d_status, stream, 1, 0, 0u, nullptr,

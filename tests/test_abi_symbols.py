@@ -29,3 +29,18 @@ def test_library_exports_declared_symbols():
     # the Python binding declares a signature for each of them
     assert names <= set(_lib.SIGNATURES), sorted(names - set(_lib.SIGNATURES))
     assert lib.tfce_version().decode().startswith("tfcehip")
+
+
+def test_knobs_validate_their_arguments():
+    """The process-wide knobs refuse out-of-range values with TFCE_EINVAL and
+    a message (host-only calls, no GPU needed), and take valid ones."""
+    lib = _lib.load_library()
+    EINVAL = -1000
+    assert lib.tfce_set_single_image_mode(2, 0) == EINVAL
+    assert b"mode" in lib.tfce_last_error()
+    assert lib.tfce_set_single_image_mode(-1, -3) == EINVAL
+    assert lib.tfce_set_single_image_mode(-1, 0) == 0
+    assert lib.tfce_set_occupancy(0) == EINVAL
+    assert lib.tfce_ols_set_tile(4) == EINVAL
+    assert lib.tfce_ols_set_tile(0) == 0
+    assert lib.tfce_set_pair_sweep(1) == 0 and lib.tfce_set_lds_sweep(1) == 0

@@ -223,3 +223,76 @@ def test_vertex_regression_with_exchange_blocks(tmp_path):
                 enh, _, _ = otfce.tfce(rp, col, img, 2.0, 1.0, exact_pow=True)
                 best = max(best, float(enh.max() * (img.max() / np.float32(100))))
             assert abs(got[1][p, k] - best) <= 1e-4 * abs(best), (p, k, got[1][p, k], best)
+
+
+def _tmi_temp(root, n, rng, mediation=False):
+    """A tmi_temp folder as tm_mmr_rand_low_ram.py:150-201 saves it: a masked
+    order-3 icosphere (vertex image: full-surface mask and adjacency, data
+    and vdensity of the in-mask vertices) and a voxel image (adjacency over
+    its voxels, all-true mask)."""
+    import argparse
+    d = os.path.join(root, "tmi_temp")
+    os.makedirs(d, exist_ok=True)
+    rp, col = synth.icosphere_csr(3)
+    V = len(rp) - 1
+    m0 = rng.random(V) < 0.85
+    vmask = synth.ellipsoid_mask((9, 10, 9), (4, 4.5, 4), (3.6, 4.1, 3.6))
+    adj1 = adjacency.create_adjac_voxel(vmask, vmask, int(vmask.sum()), 26)
+    a1 = np.empty(len(adj1), dtype=object)
+    for i, l in enumerate(adj1):
+        a1[i] = l
+    deg0 = np.diff(rp).astype(float)
+    vd0 = (1 - deg0 / deg0.max() + deg0.mean() / deg0.max()).astype(np.float32)[m0]
+    x = rng.standard_normal(n)
+    y = rng.standard_normal(n)
+    np.savetxt(os.path.join(root, "x.csv"), x, delimiter=",")
+    np.savetxt(os.path.join(root, "y.csv"), y, delimiter=",")
+    data0 = (rng.standard_normal((n, int(m0.sum()))) + 0.6 * x[:, None] * (rng.random(int(m0.sum())) < 0.3))
+    data1 = rng.standard_normal((n, len(adj1))) + 0.5 * x[:, None]
+    for s, (adj, mask, data, vd) in enumerate([(ragged(rp, col), m0, data0, vd0),
+                                               (a1, np.ones(len(adj1), bool), data1, np.array([1.0]))]):
+        np.save(os.path.join(d, "%d_adjacency_temp" % s), adj)
+        np.save(os.path.join(d, "%d_mask_temp" % s), mask)
+        np.save(os.path.join(d, "%d_data_temp" % s), data.astype(np.float32))
+        np.save(os.path.join(d, "%d_vdensity_temp" % s), np.asarray(vd, np.float32))
+    opts = argparse.Namespace(input=None if mediation else ["x.csv"], tfce=[2.0, 1.0, 2.0, 0.5],
+                              assigntfcesettings=[0, 1], inputmediation=["M", "x.csv", "y.csv"] if mediation else None)
+    np.save(os.path.join(d, "opts"), np.array(opts, dtype=object))
+    return {0: (ragged(rp, col), m0, data0.astype(np.float32), vd0, (2.0, 1.0)),
+            1: (a1, np.ones(len(adj1), bool), data1.astype(np.float32), np.array([1.0], np.float32), (2.0, 0.5))}, x, y
+
+
+@pytest.mark.parametrize("mediation", [False, True])
+def test_mmr_lr_from_tmi_temp(tmp_path, mediation):
+    """mmr-lr's permutation branch (tm_mmr_rand_low_ram_parallel.py:174-212)
+    from a tmi_temp folder, batched, against the per-permutation drop-in
+    (tm_func.low_ram_calculate_tfce / _mediation_tfce, themselves checked
+    against the oracle), file by file."""
+    from tfce_mediation_amd import step2, tm_func
+    from tfce_mediation_amd.tfce import CreateAdjSet
+    say(f"[gpu] step2.mmr_lr mediation={mediation}")
+    rng = np.random.default_rng(12)
+    n = 32
+    surf, x, y = _tmi_temp(str(tmp_path), n, rng, mediation)
+    with pytest.raises(ValueError, match="allow_pickle"):
+        step2.mmr_lr(0, 1, 3, os.path.join(tmp_path, "out"), 5, root=str(tmp_path))
+    out, ref = os.path.join(tmp_path, "out"), os.path.join(tmp_path, "ref")
+    os.makedirs(ref)
+    for s, (adj, mask, data, vd, (H, E)) in surf.items():
+        step2.mmr_lr(s, 1, 6, out, 5, root=str(tmp_path), allow_pickle=True, precision="fp32")
+        calc = CreateAdjSet(H, E, adj)
+        for p in range(1, 7):
+            if mediation:
+                tm_func.low_ram_calculate_mediation_tfce("M", data, mask, x, y, calc, vd, set_surf_count=s,
+                                                         perm_number=p, randomise=True, output_dir=ref,
+                                                         perm_seed=5)
+            else:
+                tm_func.low_ram_calculate_tfce(data, mask, x, calc, vd, set_surf_count=s, perm_number=p,
+                                               randomise=True, output_dir=ref, perm_seed=5)
+    names = sorted(os.listdir(ref))
+    assert names == sorted(os.listdir(out)) and len(names) == 2
+    for name in names:
+        got = np.loadtxt(os.path.join(out, name))
+        exp = np.loadtxt(os.path.join(ref, name))
+        assert got.shape == exp.shape == ((6,) if mediation else (12,))
+        np.testing.assert_allclose(got, exp, rtol=1e-4, atol=1e-6)

@@ -16,6 +16,11 @@ null CSVs where the reference writes them.
 * ``voxel_mediation``    voxel_tfce_mediation_randomise.py:42-76 reads
   ``python_temp/`` (STEP_1_voxel_tfce_mediation.py:90-95) and appends to
   ``output_med_<medtype>/perm_SobelZ/``;
+* ``mmr_lr``             the permutation branch of tm_mmr_rand_low_ram_parallel.py:
+  174-212 (mmr-lr, one surface of a multimodal set per call) reads
+  ``tmi_temp/`` (saved by tm_mmr_rand_low_ram.py:150-201) and appends
+  ``perm_maxTFCE_surf<s>_tcon<c>.csv`` (t, +t then -t) or
+  ``perm_maxTFCE_surf<s>_<medtype>_zstat.csv`` (Sobel z) to the output path;
 * ``tm_glm``             the GLM branch of tm_models_randomise.py:68-243 reads
   ``tmtemp_GLM_<surface>/`` or ``tmtemp_GLM_volume/`` (saved by
   STEP_1_tm_models.py:156-175, 664-697) and appends to
@@ -155,3 +160,65 @@ def tm_glm(start, stop, surface=None, root=".", exchangeblock=None, seed=0, allo
                                 adjac_rh=v["adjac_rh"], mask_lh=v["mask_lh"], mask_rh=v["mask_rh"],
                                 vdensity_lh=v["vdensity_lh"], vdensity_rh=v["vdensity_rh"], **kw)
     return randomise.tm_glm(data, exog, cov, str(v["gstat"]), v["optstfce"], perms, adjac=v["adjac"], **kw)
+
+
+def mmr_lr(surface, start, stop, out_path, seed, root=".", allow_pickle=False, batch=None, precision="fp16x3"):
+    """Permutation numbers start..stop of surface ``surface`` of an mmr-lr
+    analysis saved in ``root/tmi_temp`` (low_ram_calculate_tfce /
+    low_ram_calculate_mediation_tfce per permutation, tm_func.py:144-185 and
+    269-305, batched): permutation p is np.random.seed(p + seed) and a
+    permutation of the subjects.  The saved options are a pickled argparse
+    namespace and the adjacency a pickled list, so ``allow_pickle`` must be
+    given for a folder the caller's own mmr-lr run wrote.  Returns the null
+    {contrast: [n_perm, 2]} (t) or [n_perm] (mediation)."""
+    import torch
+    from .ols import PermutedOLS
+    from .tfce import Adjacency, SurfaceSet, TFCE_ADJ_IMPLICIT_ZERO
+    tmp = os.path.join(root, "tmi_temp")
+    s = int(surface)
+    v = load_temp(tmp, ["opts", "%d_adjacency_temp" % s, "%d_mask_temp" % s, "%d_data_temp" % s,
+                        "%d_vdensity_temp" % s], allow_pickle)
+    opts = v["opts"].tolist() if isinstance(v["opts"], np.ndarray) else v["opts"]
+    if getattr(opts, "assigntfcesettings", None):
+        ptr = int(opts.assigntfcesettings[s] * 2)
+        H, E = float(opts.tfce[ptr]), float(opts.tfce[ptr + 1])
+    else:
+        H, E = float(opts.tfce[0]), float(opts.tfce[1])
+    mask = np.asarray(v["%d_mask_temp" % s]).astype(bool)
+    data = np.asarray(v["%d_data_temp" % s], dtype=np.float32)
+    vd = np.broadcast_to(np.asarray(v["%d_vdensity_temp" % s], dtype=np.float32), (int(mask.sum()),))
+    vd = torch.as_tensor(np.ascontiguousarray(vd)).cuda()
+    n = data.shape[0]
+    perms = _perms(n, range(int(start), int(stop) + 1), seed)
+    csr, flags = randomise._hemi_csr(v["%d_adjacency_temp" % s], mask)
+
+    def csv(path):
+        return np.genfromtxt(os.path.join(root, path), delimiter=",")
+    os.makedirs(out_path, exist_ok=True)
+    if getattr(opts, "inputmediation", None):
+        medtype = str(opts.inputmediation[0])
+        # the z rescale takes the largest in-mask z only (tm_func.py:295-297)
+        surfs = SurfaceSet([Adjacency(csr, H, E, flags & TFCE_ADJ_IMPLICIT_ZERO)], [0])
+        med = randomise.MediationModel(data, csv(opts.inputmediation[1]), csv(opts.inputmediation[2]), medtype,
+                                       precision=precision)
+        null = []
+        for _, pb in randomise._batches(perms, batch, 1):
+            null.append(surfs.batch(med.z(pb), 1, vdensity=vd)[0, :, 0].cpu().numpy())
+        null = np.concatenate(null)
+        randomise._write(os.path.join(out_path, "perm_maxTFCE_surf%d_%s_zstat.csv" % (s, medtype)), null, "%f")
+        return null
+    cols = [csv(p) for p in opts.input]
+    pred_x = np.column_stack(cols) if len(cols) > 1 else cols[0]
+    X = np.column_stack([np.ones(n), pred_x])
+    contrasts = list(range(1, X.shape[1]))
+    surfs = SurfaceSet([Adjacency(csr, H, E, flags)], [0])
+    model = PermutedOLS(data, X, contrasts=contrasts, precision=precision)
+    nulls = {c: [] for c in contrasts}
+    for _, pb in randomise._batches(perms, batch, 2):
+        t = model.t(pb)
+        for ci, c in enumerate(contrasts):
+            nulls[c].append(surfs.batch(t[ci], 3, vdensity=vd)[0].cpu().numpy())
+    nulls = {c: np.concatenate(x, 0) for c, x in nulls.items()}
+    for c, x in nulls.items():
+        randomise._write(os.path.join(out_path, "perm_maxTFCE_surf%d_tcon%d.csv" % (s, c)), x, "%f")
+    return nulls

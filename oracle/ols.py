@@ -19,7 +19,8 @@ TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
                     rows permuted by rand_array; the model F, one F per exog
                     block from the fit without that block (SS_Total minus the
                     reduced model's residual sum of squares, cynumstats.pyx:
-                    109-112), and t = a / se of every column (se float32).
+                    109-112), and t = a / se of every column (se float32; FP64
+                    for a one-dimensional endog, :2365-2366).
 * ``perm_partial_f`` glm_typeI's per-block F for a batch of permutations
                     (tm_models_randomise.py:190-196).
 """
@@ -147,7 +148,10 @@ def glm_typeI(endog, exog, dmy_covariates=None, rand_array=None):
         start += w
     sigma2 = np.sum((endog - np.dot(exog_vars, a)) ** 2, axis=0) / (n - k)
     invXX = np.linalg.inv(np.dot(exog_vars.T, exog_vars))
-    se = se_of_slope(endog.shape[1], invXX, sigma2, k)
+    if endog.ndim == 1:
+        se = np.sqrt(np.diag(sigma2 * invXX))  # pyfunc.py:2365-2366: one column, FP64 se
+    else:
+        se = se_of_slope(endog.shape[1], invXX, sigma2, k)
     return fmodel, np.array(fvar), a / se
 
 

@@ -52,3 +52,40 @@ def test_tm_glm_folder_refuses_pickles_by_default(tmp_path):
     np.save(d / "adjac", obj)
     with pytest.raises(ValueError, match="allow_pickle"):
         step2.tm_glm(1, 2, root=str(tmp_path))
+
+
+def test_tm_mediation_folder_refuses_pickles_by_default(tmp_path):
+    d = tmp_path / "tmtemp_mediation_volume"
+    d.mkdir()
+    n = 6
+    for k, v in dict(dmy_leftvar=np.arange(n, dtype=float), dmy_rightvar=np.ones(n), medtype=np.array("M"),
+                     data=np.zeros((n, 3), np.float32), optstfce=np.array([2.0, 1.0]),
+                     dmy_covariates=None).items():
+        np.save(d / k, v)
+    obj = np.empty(3, dtype=object)
+    obj[0], obj[1], obj[2] = [1], [0, 2], [1]
+    np.save(d / "adjac", obj)
+    with pytest.raises(ValueError, match="allow_pickle"):
+        step2.tm_mediation(1, 2, root=str(tmp_path))
+
+
+def test_type_y_path_a_with_covariates_matches_oracle():
+    """Type Y's path A with covariates (tm_models_randomise.py:482-488): the
+    t of left in right[perm] ~ [1, left[perm], covariates], solved on the
+    host per permutation, against row 1 of the oracle's glm_typeI t."""
+    import types
+    from oracle import ols as ools
+    from tfce_mediation_amd.randomise import MediationModel, permutation_indices
+    rng = np.random.default_rng(3)
+    n = 30
+    for wx in (1, 2):
+        left = rng.standard_normal((n, wx)) if wx > 1 else rng.standard_normal(n)
+        right = 0.5 * left.reshape(n, -1)[:, 0] + rng.standard_normal(n)
+        cov = rng.standard_normal((n, 2))
+        Xa = np.column_stack([np.ones(n), left.reshape(n, wx), cov])
+        stub = types.SimpleNamespace(ta_fit=(Xa, right, wx), n=n)
+        perms = permutation_indices(n, range(5), seed=8)
+        got = MediationModel.path_a_t(stub, perms)
+        for p, perm in enumerate(perms):
+            exp = ools.glm_typeI(right[perm], [left[perm]], dmy_covariates=cov)[2][1]
+            assert abs(got[p] - exp) <= 1e-10 * abs(exp)

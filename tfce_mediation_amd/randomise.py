@@ -17,6 +17,9 @@ the GPU (permuted t on MFMA, one persistent TFCE launch per batch):
 * ``tm_glm``           the GLM branch of tm_models_randomise.py:182-243
   (glm_typeI statistics of the whole permuted design: t per exog column and
   F per exog variable, ``perm_Tstat_con<j>`` / ``perm_Fstat_<var>`` files);
+* ``tm_mediation``     the mediation branch of tm_models_randomise.py:430-510
+  (Sobel z of glm_typeI t-values with covariates, types I, M and Y,
+  ``perm_Zstat_<type>`` files);
 * ``mmr``              tm_func.calculate_tfce in a loop (the mmr randomise of
   tm_multimodality_multisurface_regression.py:523-572): per-surface nulls
   ``perm_maxTFCE_surf<s>_tcon<c>.csv`` in "%f"; ``merged=True`` shares the
@@ -185,42 +188,80 @@ class MediationModel:
     set: path A is y_img ~ [1, x]; path B is y_img ~ [1, y, x] (type M, the
     mediator t of y), [1, x, y] (type I, t of x) or type Y with path A the
     scalar x -> y regression.  Types M and I permute x only, type Y permutes
-    x and y together (vertex_tfce_mediation_randomise.py:82-90)."""
+    x and y together (vertex_tfce_mediation_randomise.py:82-90).
 
-    def __init__(self, M, pred_x, depend_y, medtype="M", alg="aroian", precision="fp16x3", fused=True):
-        """fused (types M and I): both paths bind one prepared copy of M and
-        share one contraction per batch (``tfce_ols_perm_sobel``); otherwise
-        each path contracts on its own and ``tfce_sobel_z`` combines the
-        t-values (same numbers, bit for bit)."""
-        import torch
+    With ``covariates`` (columns of no interest, never permuted) every design
+    gains them after its variables, as glm_typeI builds [1, exog...,
+    covariates] (pyfunc.py:2301-2314), and each path's t is row 1 of that fit
+    (tm_models_randomise.py:430-510); x and y may then hold several columns
+    (dummy-coded factors), of which row 1 is the first.  Type Y's path A,
+    y ~ [1, x, covariates] with x and y permuted together and the covariates
+    fixed, then changes with the permutation: it is solved in FP64 on the
+    host, one small fit per permutation."""
+
+    def __init__(self, M, pred_x, depend_y, medtype="M", alg="aroian", precision="fp16x3", fused=True,
+                 covariates=None):
+        """fused (types M and I with a one-column x): both paths bind one
+        prepared copy of M and share one contraction per batch
+        (``tfce_ols_perm_sobel``); otherwise each path contracts on its own
+        and ``tfce_sobel_z`` combines the t-values (same numbers, bit for
+        bit)."""
         x = np.asarray(pred_x, dtype=np.float64)
         y = np.asarray(depend_y, dtype=np.float64)
         n = x.shape[0]
+        x2, y2 = x.reshape(n, -1), y.reshape(n, -1)
+        wx, wy = x2.shape[1], y2.shape[1]
+        C = np.zeros((n, 0)) if covariates is None else np.asarray(covariates, dtype=np.float64).reshape(n, -1)
         self.medtype, self.alg = medtype, {"aroian": 0, "sobel": 1, "goodman": 2}[alg]
-        one = np.ones(n)
+        one = np.ones((n, 1))
         self.ta_const = None
+        self.ta_fit = None
         self.fused = False
+        xcols = list(range(1, 1 + wx))
         if medtype in ("M", "I"):
-            self.A = PermutedOLS(M, np.column_stack([one, x]), contrasts=[1], perm_columns=[1], precision=precision)
+            self.A = PermutedOLS(M, np.column_stack([one, x2, C]), contrasts=[1], perm_columns=xcols,
+                                 precision=precision)
             if medtype == "M":
-                XB, cb, pc = np.column_stack([one, y, x]), 1, [2]
+                XB, pc = np.column_stack([one, y2, x2, C]), [1 + wy + j for j in range(wx)]
             else:
-                XB, cb, pc = np.column_stack([one, x, y]), 1, [1]
-            if fused:
-                self.B = PermutedOLS(None, XB, contrasts=[cb], perm_columns=pc, data_from=self.A)
+                XB, pc = np.column_stack([one, x2, y2, C]), xcols
+            if fused and wx == 1:
+                self.B = PermutedOLS(None, XB, contrasts=[1], perm_columns=pc, data_from=self.A)
                 self.fused = True
             else:
-                self.B = PermutedOLS(M, XB, contrasts=[cb], perm_columns=pc, precision=precision)
+                self.B = PermutedOLS(M, XB, contrasts=[1], perm_columns=pc, precision=precision)
         elif medtype == "Y":
-            from .cynumstats import tval_int
-            Xa = np.column_stack([one, x])
-            self.ta_const = float(tval_int(Xa, np.linalg.inv(Xa.T @ Xa), y[:, None].astype(np.float32), n, 2, 1)[1, 0])
+            if wy != 1:
+                raise ValueError("mediation type Y needs a one-column dependent variable")
+            self.n = n
+            self.ta_fit = (np.column_stack([one, x2, C]), y2[:, 0], wx)
+            if C.shape[1] == 0:
+                # x and y move together: the x -> y fit (linregress's slope over
+                # its standard error) is that of the saved rows
+                self.ta_const = float(self.path_a_t(np.arange(n)[None])[0])
+                self.ta_fit = None
             self.A = None
-            self.B = PermutedOLS(M, np.column_stack([one, y, x]), contrasts=[1], perm_columns=[1, 2],
-                                 precision=precision)
+            self.B = PermutedOLS(M, np.column_stack([one, y2, x2, C]), contrasts=[1],
+                                 perm_columns=list(range(1, 1 + wy + wx)), precision=precision)
         else:
             raise ValueError("Invalid mediation type")
+        self.n = n
         self.V = self.B.V
+
+    def path_a_t(self, perms):
+        """FP64 [P] t of x in y[perm] ~ [1, x[perm], covariates] (type Y;
+        perms [P, n] host indices): a / sqrt(sigma2 * inv(X'X)_11), the
+        one-column se of glm_typeI (pyfunc.py:2363-2366)."""
+        Xa, y, wx = self.ta_fit
+        perms = np.asarray(perms, dtype=np.int64).reshape(-1, self.n)
+        X = np.repeat(Xa[None], len(perms), 0)
+        X[:, :, 1:1 + wx] = Xa[perms][:, :, 1:1 + wx]
+        yp = y[perms]
+        n, k = Xa.shape
+        inv = np.linalg.inv(np.einsum("pik,pil->pkl", X, X))
+        a = np.einsum("pkl,pil,pi->pk", inv, X, yp)
+        sigma2 = np.sum((yp - np.einsum("pik,pk->pi", X, a)) ** 2, 1) / (n - k)
+        return a[:, 1] / np.sqrt(sigma2 * inv[:, 1, 1])
 
     def z(self, perms=None, out=None, stream=None):
         """[P, V] z on ``stream`` (default: the current stream), into ``out`` if given."""
@@ -241,9 +282,15 @@ class MediationModel:
             return z
         with torch.cuda.stream(s):
             tb = self.B.t(perms, stream=s)[0]
-            ta = self.A.t(perms, stream=s)[0] if self.A is not None else None
             z = out if out is not None else torch.empty_like(tb)
             assert z.shape == tb.shape and z.is_contiguous()
+            if self.ta_fit is not None:
+                host = np.arange(self.n)[None] if perms is None else perms.cpu().numpy()
+                for p, ta in enumerate(self.path_a_t(host)):
+                    _lib.check(_lib.lib().tfce_sobel_z(None, _lib.ptr(tb[p]), self.V, self.alg, float(ta),
+                                                       _lib.ptr(z[p]), _lib.stream_ptr(s)))
+                return z
+            ta = self.A.t(perms, stream=s)[0] if self.A is not None else None
             _lib.check(_lib.lib().tfce_sobel_z(_lib.ptr(ta), _lib.ptr(tb), tb.numel(), self.alg,
                                                float(self.ta_const or 0.0), _lib.ptr(z), _lib.stream_ptr(s)))
         return z
@@ -308,6 +355,44 @@ def glm_design(n, exog, dmy_covariates=None):
     return np.column_stack(cols), blocks
 
 
+class _TmSurfaces:
+    """The TFCE side of the tm-models randomise loop: both hemispheres with
+    masks and vdensity as write_perm_maxTFCE_vertex (pyfunc.py:107-119,
+    files ``..._TFCE_maxVertex.csv`` in "%.4f"), or one voxel set as
+    write_perm_maxTFCE_voxel (pyfunc.py:121-126, ``..._TFCE_maxVoxel.csv`` in
+    "%1.4f", NaN and infinite statistics set to 0 where the reference's max
+    would be nan).  ``data`` is moved into the sweep's element order."""
+
+    def __init__(self, data, optstfce, adjac=None, adjac_lh=None, adjac_rh=None, mask_lh=None, mask_rh=None,
+                 vdensity_lh=1, vdensity_rh=1):
+        import torch
+        H, E = float(optstfce[0]), float(optstfce[1])
+        self.vd = None
+        if adjac_lh is not None:
+            (lh, fl), (rh, fr) = _hemi_csr(adjac_lh, mask_lh), _hemi_csr(adjac_rh, mask_rh)
+            nv_lh = int(np.asarray(mask_lh, bool).sum())
+            if not np.isscalar(vdensity_lh) or not np.isscalar(vdensity_rh):
+                ml, mr = np.asarray(mask_lh, bool), np.asarray(mask_rh, bool)
+                vl = np.broadcast_to(np.asarray(vdensity_lh, np.float32), (len(ml),))[ml]
+                vr = np.broadcast_to(np.asarray(vdensity_rh, np.float32), (len(mr),))[mr]
+                self.vd = torch.as_tensor(np.ascontiguousarray(np.concatenate([vl, vr]))).cuda()
+            (lh, rh), self.data, self.vd = _in_layout([lh, rh], [0, nv_lh], data, self.vd)
+            self.surfs = SurfaceSet([Adjacency(lh, H, E, fl), Adjacency(rh, H, E, fr)], [0, nv_lh])
+            self.suffix, self.fmt, self.vertex = "maxVertex", "%.4f", True
+        else:
+            (csr,), self.data, _ = _in_layout([to_csr(adjac)], [0], data)
+            self.surfs = SurfaceSet([Adjacency(csr, H, E)], [0])
+            self.suffix, self.fmt, self.vertex = "maxVoxel", "%1.4f", False
+
+    def maxima(self, stat, signs):
+        """[P, n_signs] scaled maxima of the maps stat [P, V] (host)."""
+        import torch
+        if not self.vertex:
+            stat = torch.where(torch.isfinite(stat), stat, torch.zeros_like(stat))
+        m = self.surfs.batch(stat, signs, vdensity=self.vd)
+        return (m.max(0).values if self.vertex else m[0]).cpu().numpy()
+
+
 def tm_glm(data, exog, dmy_covariates, gstat, optstfce, perms, adjac=None, adjac_lh=None, adjac_rh=None,
            mask_lh=None, mask_rh=None, vdensity_lh=1, vdensity_rh=1, varnames=None, out_dir=None, batch=None,
            precision="fp16x3"):
@@ -332,34 +417,13 @@ def tm_glm(data, exog, dmy_covariates, gstat, optstfce, perms, adjac=None, adjac
         varnames = ["Exog%d" % (i + 1) for i in range(len(blocks))]
     gstat = str(gstat)
     want_t, want_f = gstat != "f", gstat != "t"
-    H, E = float(optstfce[0]), float(optstfce[1])
-    vd = None
-    if adjac_lh is not None:
-        (lh, fl), (rh, fr) = _hemi_csr(adjac_lh, mask_lh), _hemi_csr(adjac_rh, mask_rh)
-        nv_lh = int(np.asarray(mask_lh, bool).sum())
-        if not np.isscalar(vdensity_lh) or not np.isscalar(vdensity_rh):
-            vl = np.broadcast_to(np.asarray(vdensity_lh, np.float32), (len(mask_lh),))[np.asarray(mask_lh, bool)]
-            vr = np.broadcast_to(np.asarray(vdensity_rh, np.float32), (len(mask_rh),))[np.asarray(mask_rh, bool)]
-            vd = torch.as_tensor(np.ascontiguousarray(np.concatenate([vl, vr]))).cuda()
-        (lh, rh), data, vd = _in_layout([lh, rh], [0, nv_lh], data, vd)
-        surfs = SurfaceSet([Adjacency(lh, H, E, fl), Adjacency(rh, H, E, fr)], [0, nv_lh])
-        suffix, fmt, vertex = "maxVertex", "%.4f", True
-    else:
-        (csr,), data, _ = _in_layout([to_csr(adjac)], [0], data)
-        surfs = SurfaceSet([Adjacency(csr, H, E)], [0])
-        suffix, fmt, vertex = "maxVoxel", "%1.4f", False
-    model = PermutedOLS(data, X, contrasts=list(range(1, numcon + 1)), precision=precision)
+    tm = _TmSurfaces(data, optstfce, adjac, adjac_lh, adjac_rh, mask_lh, mask_rh, vdensity_lh, vdensity_rh)
+    model = PermutedOLS(tm.data, X, contrasts=list(range(1, numcon + 1)), precision=precision)
     names_t = ["Tstat_con%d" % (j + 1) for j in range(numcon)] if want_t else []
     names_f = ["Fstat_%s" % varnames[b] for b in range(len(blocks))] if want_f else []
     nulls = {name: [] for name in names_t + names_f}
-
-    def maxima(stat, signs):
-        if not vertex:
-            stat = torch.where(torch.isfinite(stat), stat, torch.zeros_like(stat))
-        m = surfs.batch(stat, signs, vdensity=vd)
-        return (m.max(0).values if vertex else m[0]).cpu().numpy()
-
-    items = 2 * (2 if vertex else 1)
+    maxima, suffix, fmt = tm.maxima, tm.suffix, tm.fmt
+    items = 2 * (2 if tm.vertex else 1)
     for _, pb in _batches(perms, batch, items):
         if want_t:
             t = model.t(pb)
@@ -374,6 +438,40 @@ def tm_glm(data, exog, dmy_covariates, gstat, optstfce, perms, adjac=None, adjac
         for name, v in nulls.items():
             _write(os.path.join(out_dir, "perm_%s_TFCE_%s.csv" % (name, suffix)), v, fmt)
     return nulls
+
+
+def tm_mediation(data, leftvar, rightvar, medtype, dmy_covariates, optstfce, perms, adjac=None, adjac_lh=None,
+                 adjac_rh=None, mask_lh=None, mask_rh=None, vdensity_lh=1, vdensity_rh=1, out_dir=None,
+                 batch=None, precision="fp16x3"):
+    """The mediation branch of the tm-models randomise loop
+    (tm_models_randomise.py:430-510): per permutation, path A is row 1 of
+    glm_typeI's t for data ~ [1, left, covariates] and path B row 1 for
+    [1, left, right, covariates] (type I) or [1, right, left, covariates]
+    (type M), left permuted; type Y permutes left and right together, with
+    path A the fit right ~ [1, left, covariates].  The Aroian z
+    (calc_indirect, pyfunc.py:2678-2709) goes through TFCE, positive only,
+    into ``perm_Zstat_<medtype>_TFCE_maxVertex.csv`` (both hemispheres) or
+    ``..._maxVoxel.csv`` (see ``_TmSurfaces``).  The covariates never move.
+
+    The reference permutes its already permuted variables again on every
+    iteration (``dmy_leftvar = dmy_leftvar[rand_array]``), so iteration i
+    sees the composition of the first i permutations; a composition of
+    uniform permutations is uniform, so the null is the same, and here
+    permutation p is applied to the saved variables, which keeps any range
+    of permutation numbers reproducible on its own.  Returns the null
+    [n_perm]."""
+    import torch
+    data = np.asarray(data) if not isinstance(data, torch.Tensor) else data
+    medtype = str(medtype)
+    tm = _TmSurfaces(data, optstfce, adjac, adjac_lh, adjac_rh, mask_lh, mask_rh, vdensity_lh, vdensity_rh)
+    med = MediationModel(tm.data, leftvar, rightvar, medtype, precision=precision, covariates=dmy_covariates)
+    out = []
+    for _, pb in _batches(perms, batch, 2 if tm.vertex else 1):
+        out.append(tm.maxima(med.z(pb), 1)[:, 0])
+    null = np.concatenate(out)
+    if out_dir is not None:
+        _write(os.path.join(out_dir, "perm_Zstat_%s_TFCE_%s.csv" % (medtype, tm.suffix)), null, tm.fmt)
+    return null
 
 
 def mmr(merge_y, pred_x, surfaces, offsets, settings, perms, groups=None, merged=True, zero_in_image=None,

@@ -24,7 +24,12 @@ null CSVs where the reference writes them.
 * ``tm_glm``             the GLM branch of tm_models_randomise.py:68-243 reads
   ``tmtemp_GLM_<surface>/`` or ``tmtemp_GLM_volume/`` (saved by
   STEP_1_tm_models.py:156-175, 664-697) and appends to
-  ``output_GLM_<surface>/perm_GLM/`` or ``output_GLM_volume/perm_GLM/``.
+  ``output_GLM_<surface>/perm_GLM/`` or ``output_GLM_volume/perm_GLM/``;
+* ``tm_mediation``       the mediation branch of tm_models_randomise.py:
+  92-102, 430-510 reads ``tmtemp_mediation_<surface>/`` or
+  ``tmtemp_mediation_volume/`` (saved by STEP_1_tm_models.py:952-1012) and
+  appends ``perm_Zstat_<medtype>_TFCE_max*.csv`` to
+  ``output_mediation_<surface|volume>/perm_mediation/``.
 
 The permutations of permutation numbers ``start..stop`` come from
 ``randomise.permutation_indices`` (numpy's legacy RNG seeded with number +
@@ -146,7 +151,7 @@ def tm_glm(start, stop, surface=None, root=".", exchangeblock=None, seed=0, allo
         count += int(nc)
     cov = v["dmy_covariates"]
     if cov.dtype == object and cov.ndim == 0 and cov.item() is None:
-        cov = None  # np.save of None (tm_models_randomise.py:174-175)
+        cov = None  # np.save of None (tm_models_randomise.py:177-178)
     data = v["data"]
     blocks = None
     if exchangeblock is not None:
@@ -160,6 +165,36 @@ def tm_glm(start, stop, surface=None, root=".", exchangeblock=None, seed=0, allo
                                 adjac_rh=v["adjac_rh"], mask_lh=v["mask_lh"], mask_rh=v["mask_rh"],
                                 vdensity_lh=v["vdensity_lh"], vdensity_rh=v["vdensity_rh"], **kw)
     return randomise.tm_glm(data, exog, cov, str(v["gstat"]), v["optstfce"], perms, adjac=v["adjac"], **kw)
+
+
+def tm_mediation(start, stop, surface=None, root=".", exchangeblock=None, seed=0, allow_pickle=False, batch=None,
+                 precision="fp16x3"):
+    """Permutation numbers start..stop (inclusive) of the tm-models mediation
+    saved for ``surface`` (None: the volume); returns the null [n_perm]."""
+    name = surface if surface is not None else "volume"
+    tmp = os.path.join(root, "tmtemp_mediation_%s" % name)
+    common = ["dmy_leftvar", "dmy_rightvar", "medtype", "data", "optstfce", "dmy_covariates"]
+    if surface is not None:
+        v = load_temp(tmp, common + ["mask_lh", "mask_rh", "adjac_lh", "adjac_rh", "vdensity_lh", "vdensity_rh"],
+                      allow_pickle)
+    else:
+        v = load_temp(tmp, common + ["adjac"], allow_pickle)
+    cov = v["dmy_covariates"]
+    if cov.dtype == object and cov.ndim == 0 and cov.item() is None:
+        cov = None  # np.save of None (tm_models_randomise.py:177-178)
+    data = v["data"]
+    blocks = None
+    if exchangeblock is not None:
+        blocks = np.genfromtxt(exchangeblock, dtype=str)
+    perms = _perms(data.shape[0], range(int(start), int(stop) + 1), seed, blocks)
+    out = _out_dir(root, "output_mediation_%s" % name, "perm_mediation")
+    kw = dict(out_dir=out, batch=batch, precision=precision)
+    args = (data, v["dmy_leftvar"], v["dmy_rightvar"], str(v["medtype"]), cov, v["optstfce"], perms)
+    if surface is not None:
+        return randomise.tm_mediation(*args, adjac_lh=v["adjac_lh"], adjac_rh=v["adjac_rh"], mask_lh=v["mask_lh"],
+                                      mask_rh=v["mask_rh"], vdensity_lh=v["vdensity_lh"],
+                                      vdensity_rh=v["vdensity_rh"], **kw)
+    return randomise.tm_mediation(*args, adjac=v["adjac"], **kw)
 
 
 def mmr_lr(surface, start, stop, out_path, seed, root=".", allow_pickle=False, batch=None, precision="fp16x3"):
@@ -187,7 +222,7 @@ def mmr_lr(surface, start, stop, out_path, seed, root=".", allow_pickle=False, b
     mask = np.asarray(v["%d_mask_temp" % s]).astype(bool)
     data = np.asarray(v["%d_data_temp" % s], dtype=np.float32)
     vd = np.broadcast_to(np.asarray(v["%d_vdensity_temp" % s], dtype=np.float32), (int(mask.sum()),))
-    vd = torch.as_tensor(np.ascontiguousarray(vd)).cuda()
+    vd = torch.as_tensor(np.array(vd, dtype=np.float32)).cuda()  # a writable copy of the broadcast view
     n = data.shape[0]
     perms = _perms(n, range(int(start), int(stop) + 1), seed)
     csr, flags = randomise._hemi_csr(v["%d_adjacency_temp" % s], mask)

@@ -474,3 +474,56 @@ def test_small_mixed_set_with_vdensity(torch):
                 enh, _, _ = otfce.tfce(g[0], g[1], x, 2.0, 1.0, exact_pow=True)
                 s = np.float32(x.max() / np.float32(100))
                 assert got[i, p, k] == ((enh * s) * vd[o:o + sizes[i]]).max(), (i, p, k)
+
+
+@pytest.mark.parametrize("E", [1.0, 0.5])
+def test_wide_table_joined_rows_with_overflow(torch, E):
+    """26-connected volumes take rows of at most four joined positions per
+    element; an element with more (a voxel whose earlier neighbours fall in
+    separate groups) is joined from its table row instead.  A lattice of
+    high voxels at even coordinates gives every all-odd voxel eight separate
+    earlier neighbours (the cube corners around it), and noise gives ties
+    and other groupings; the maxima and the FULL-mode enhancement must match
+    the oracle, with the LDS-resident sweep off so the global sweep runs."""
+    from tfce_mediation_amd import _lib
+    from tfce_mediation_amd.tfce import Adjacency
+    mask = synth.ellipsoid_mask((24, 26, 22), (11.5, 12.5, 10.5), (11, 12, 10))
+    row_ptr, col = adjacency.voxel_csr(mask, 26)
+    V = len(row_ptr) - 1
+    coords = np.argwhere(mask)
+    lattice = np.all(coords % 2 == 0, axis=1)
+    rng = np.random.default_rng(31)
+    P = 6
+    maps = []
+    for p in range(P):
+        x = rng.standard_normal(V) * (0.05 if p % 2 == 0 else 1.0)
+        x[lattice] += 3.0 + 0.5 * p
+        x = np.round(x, 1 if p == 4 else 6)  # p = 4: many equal values (same-level neighbours)
+        maps.append(x.astype(np.float32))
+    maps = np.stack(maps)
+    say(f"[gpu] joined-position rows with overflow V={V} P={P} E={E}")
+    adj = Adjacency((row_ptr, col), 2.0, E)
+    d = torch.from_numpy(np.ascontiguousarray(maps)).cuda()
+    lib = _lib.lib()
+    try:
+        _lib.check(lib.tfce_set_lds_sweep(0))
+        got = adj.batch_max(d, 3).cpu().numpy()
+        full = []
+        for p in range(2):
+            e = torch.zeros(V, dtype=torch.float32, device="cuda")
+            adj.run_device(d[p].contiguous(), e)
+            full.append(e.cpu().numpy())
+    finally:
+        _lib.check(lib.tfce_set_lds_sweep(1))
+    for p in range(P):
+        for k, sgn in enumerate((1, -1)):
+            x = (sgn * maps[p]).astype(np.float32)
+            if x.max() <= 0:
+                assert got[p, k] == 0
+                continue
+            enh, _, _ = otfce.tfce(row_ptr, col, x, 2.0, E, exact_pow=True)
+            exp = np.float32(enh.max() * np.float32(x.max() / np.float32(100)))
+            check_close(np.float32([got[p, k]]), np.float32([exp]), E, 2.0)
+    for p in range(2):
+        enh, _, _ = otfce.tfce(row_ptr, col, maps[p], 2.0, E, exact_pow=True)
+        check_close(full[p], enh.astype(np.float32), E, 2.0)

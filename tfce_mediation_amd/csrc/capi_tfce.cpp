@@ -24,6 +24,7 @@ struct tfce_adj {
   int32_t* d_col = nullptr;
   int32_t* d_ell = nullptr;     // [V][ell_width], -1 padded; null for long lists
   uint8_t* d_nadj = nullptr;    // [V][ell_width] adjacency among a row's neighbours (width <= 8)
+  uint32_t* d_adjw = nullptr;   // [V][ell_width] full row adjacency, rows wider than kNposMaxWidth
   int32_t ell_width = 0;
   double* d_sizepow = nullptr;  // (double)s^E for s in [0, V] when E != 1
   SurfDev* d_surf = nullptr;    // one descriptor, offset 0
@@ -244,6 +245,7 @@ int tfce_adj_create_ex(const int32_t* row_ptr, const int32_t* col, int32_t V, in
     if (a->d_col) (void)hipFree(a->d_col);
     if (a->d_ell) (void)hipFree(a->d_ell);
     if (a->d_nadj) (void)hipFree(a->d_nadj);
+    if (a->d_adjw) (void)hipFree(a->d_adjw);
     if (a->d_sizepow) (void)hipFree(a->d_sizepow);
     if (a->d_surf) (void)hipFree(a->d_surf);
     delete a;
@@ -345,6 +347,34 @@ int tfce_adj_create_ex(const int32_t* row_ptr, const int32_t* col, int32_t V, in
           if ((e = hipMalloc(&a->d_nadj, nadj.size())) != hipSuccess) return cleanup(-(int)e, "hipMalloc(nadj)");
           if ((e = hipMemcpy(a->d_nadj, nadj.data(), nadj.size(), hipMemcpyHostToDevice)) != hipSuccess)
             return cleanup(-(int)e, "upload nadj");
+          if (width > kNposMaxWidth) {
+            // wide rows: the adjacency among all of a row's neighbours, one
+            // 32-bit mask per entry (bit k: adjacent to entry k), from which
+            // the sweep groups an element's earlier neighbours exactly
+            std::vector<uint32_t> adjw((size_t)V * width, 0u);
+            std::vector<int32_t> mark(V, -1);  // entry of each neighbour in the current row
+            for (int32_t v = 0; v < V; ++v) {
+              const int32_t* row = ell.data() + (size_t)v * width;
+              for (int32_t j = 0; j < width; ++j)
+                if (row[j] >= 0) mark[row[j]] = j;
+              for (int32_t j = 0; j < width; ++j) {
+                const int32_t uj = row[j];
+                if (uj < 0) continue;
+                uint32_t m = 0u;
+                for (int32_t e2 = row_ptr[uj]; e2 < row_ptr[uj + 1]; ++e2) {
+                  const int32_t k = mark[col[e2]];
+                  if (k >= 0 && k != j) m |= 1u << k;
+                }
+                adjw[(size_t)v * width + j] = m;
+              }
+              for (int32_t j = 0; j < width; ++j)
+                if (row[j] >= 0) mark[row[j]] = -1;
+            }
+            const size_t wbytes = adjw.size() * sizeof(uint32_t);
+            if ((e = hipMalloc(&a->d_adjw, wbytes)) != hipSuccess) return cleanup(-(int)e, "hipMalloc(adjw)");
+            if ((e = hipMemcpy(a->d_adjw, adjw.data(), wbytes, hipMemcpyHostToDevice)) != hipSuccess)
+              return cleanup(-(int)e, "upload adjw");
+          }
         }
       }
     }
@@ -361,6 +391,7 @@ int tfce_adj_create_ex(const int32_t* row_ptr, const int32_t* col, int32_t V, in
   sd.ell = a->d_ell;
   sd.ell_width = a->ell_width;
   sd.nadj = a->d_nadj;
+  sd.adjw = a->d_adjw;
   sd.sizepow = a->d_sizepow;
   sd.offset = 0;
   sd.V = V;
@@ -457,6 +488,7 @@ int tfce_adj_destroy(tfce_adj_t* a) {
   (void)hipFree(a->d_col);
   if (a->d_ell) (void)hipFree(a->d_ell);
   if (a->d_nadj) (void)hipFree(a->d_nadj);
+  if (a->d_adjw) (void)hipFree(a->d_adjw);
   if (a->d_sizepow) (void)hipFree(a->d_sizepow);
   (void)hipFree(a->d_surf);
   delete a;
@@ -471,11 +503,13 @@ int tfce_adj_info(const tfce_adj_t* a, int32_t* V, int64_t* nnz, int32_t* symmet
   return 0;
 }
 
-// Width of the per-image neighbour-position table the sweep keeps for a surface
-// (symmetric, fixed-width table of at most kNposMaxWidth), 0 if none.
+// Words per element of the per-image row table the sweep keeps for a surface:
+// neighbour positions (symmetric, fixed-width table of at most kNposMaxWidth),
+// joined positions (wider symmetric tables with full row adjacency), 0 if none.
 static int32_t np_width(const tfce_adj* a) {
   if (a->flags & kSurfRing) return kRingRowWords;
-  return ((a->flags & kSurfSymmetric) && a->ell_width > 0 && a->ell_width <= kNposMaxWidth) ? a->ell_width : 0;
+  if ((a->flags & kSurfSymmetric) && a->ell_width > 0 && a->ell_width <= kNposMaxWidth) return a->ell_width;
+  return a->d_adjw ? kCandRowWords : 0;
 }
 
 // Whether the LDS-resident sweep takes a surface: symmetric, with a neighbour
@@ -774,6 +808,7 @@ int tfce_surfset_create(const tfce_adj_t* const* adjs, int32_t n, const int64_t*
     sd.ell = a->d_ell;
     sd.ell_width = a->ell_width;
     sd.nadj = a->d_nadj;
+    sd.adjw = a->d_adjw;
     sd.sizepow = a->d_sizepow;
     sd.offset = offsets[idx[k]];
     sd.V = a->V;

@@ -38,6 +38,15 @@ constexpr int32_t kSurfRing = 8;
 constexpr int kRingMaxDeg = 6;
 constexpr int32_t kRingNone = 0x3ffff;  // 18-bit position of an inactive neighbour
 constexpr int32_t kRingRowWords = 4;
+// Symmetric surface with a wide table (rows of more than kNposMaxWidth, the
+// 26-connected voxel grids) and full row adjacency (adjw): the sweep keeps one
+// 16-byte row per active element of the positions it joins (kCandRowWords
+// words, -1 past the last), found once per image in element order: every
+// neighbour activated earlier in its level, and one neighbour per group of
+// mutually adjacent neighbours activated at earlier levels.  An element with
+// more than four such candidates holds -2 - (its element) in the first word
+// and is joined from the table.
+constexpr int32_t kCandRowWords = 4;
 // Largest surface the LDS-resident sweep holds (12 bytes per element plus the
 // level tables in 160 KB of LDS), and the most levels it takes (a step of
 // max/100 gives at most 102).
@@ -52,6 +61,8 @@ struct SurfDev {
   const uint8_t* nadj;     // [V][ell_width]: bit k of entry j set iff neighbours j and 8*(j/8)+k are
                            // adjacent (groups of 8 entries); [V][16] packed rows with kSurfRing;
                            // null if not built
+  const uint32_t* adjw;    // [V][ell_width] for rows wider than kNposMaxWidth: bit k of entry j set iff
+                           // neighbours j and k are adjacent; null if not built
   const double* sizepow;   // [V+1] (double)s^E, null when E == 1
   int64_t offset;          // first element of this surface inside a map row
   int32_t V;

@@ -283,23 +283,6 @@ __device__ __forceinline__ void prune_row(const int (&e)[8], int i, int b0, int 
   prune8(p, adj, i, b0, r);
 }
 
-// Candidates of element i from one chunk of 8 entries u[] (-1: none) of a
-// symmetric table row with adjacency bytes (wlo, whi): the neighbours'
-// positions, joined = smaller position, adjacent old ones pruned.
-__device__ __forceinline__ void chunk_cands(const int32_t* pos, const int (&u)[8], int i, int b0, unsigned wlo,
-                                            unsigned whi, int (&r)[9]) {
-  int q[8];
-  unsigned adj[8];
-#pragma unroll
-  for (int j = 0; j < 8; ++j) q[j] = u[j] >= 0 ? pos[u[j]] : kNever;
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    adj[j] = (wlo >> (8 * j)) & 0xffu;
-    adj[4 + j] = (whi >> (8 * j)) & 0xffu;
-  }
-  prune8(q, adj, i, b0, r);
-}
-
 // 3b for one element i (vertex v) and up to 8 of its neighbours u[] (-1: none)
 // of a table or CSR row without adjacency bytes: gather their positions, find
 // the roots of the joined ones and of i's set (ri, an ancestor) as interleaved
@@ -329,6 +312,28 @@ __device__ __forceinline__ int join_chunk(int32_t* parent, int32_t* size, const 
     r[j] = take ? q[j] : -1;
   }
   return join_roots<true>(parent, size, r, ri, b0);
+}
+
+// Joins element i (vertex row `row` of a symmetric table of width W) with
+// every neighbour of a smaller position, eight entries at a time, without
+// pruning: the path of elements whose joined-position row overflowed.
+__device__ int join_table_row(int32_t* parent, int32_t* size, const int32_t* pos, const int32_t* row, int W, int i,
+                              int b0) {
+  int ri = i;
+  for (int e = 0; e < W; e += 8) {
+    const int4 lo = reinterpret_cast<const int4*>(row)[e >> 2];
+    const int4 hi = (e + 4 < W) ? reinterpret_cast<const int4*>(row)[(e >> 2) + 1] : make_int4(-1, -1, -1, -1);
+    const int u[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+    int r[9];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int q = u[j] >= 0 ? pos[u[j]] : kNever;
+      r[j] = q < i ? q : -1;
+    }
+    ri = join_roots<true>(parent, size, r, ri, b0);
+    if (u[7] < 0) break;  // -1 padding: the row has ended
+  }
+  return ri;
 }
 
 // Compare-exchange for a descending sorting network.
@@ -605,7 +610,10 @@ __global__ __launch_bounds__(kTfceThreads, FULL ? 3 : 4) void tfce_batch_kernel(
     // (vertex of a position): only the positions are written then.
     const bool ring = RING || (S.flags & kSurfRing) != 0;
     const bool use_np = RING || (symmetric && ell != nullptr && (ring ? kRingRowWords : W) <= a.np_w);
-    const bool need_order = FULL || !use_np;
+    // wide symmetric tables: rows of joined positions (kCandRowWords)
+    const bool use_cr = !RING && !use_np && symmetric && ell != nullptr && S.adjw != nullptr &&
+                        a.np_w >= kCandRowWords;
+    const bool need_order = FULL || !(use_np || use_cr);
     if (L > 0) {
       // the levels again from the map (cheaper than storing and re-reading
       // them); every element's position is written here, kNever if inactive
@@ -735,6 +743,75 @@ __global__ __launch_bounds__(kTfceThreads, FULL ? 3 : 4) void tfce_batch_kernel(
       }
       __syncthreads();
     }
+    // Wide symmetric table: the joined positions of every active element, in
+    // element order (table rows and neighbours' positions near each other):
+    // the neighbours activated earlier in its level, and one of each group of
+    // mutually adjacent neighbours activated at earlier levels (one set
+    // already), found with the row's full adjacency.  The union step then
+    // reads one 16-byte row per element instead of a table row, its
+    // adjacency and 26 positions.
+    if (use_cr && L > 0) {
+      for (int v = tid; v < V; v += kTfceThreads) {
+        const int q = pos[v];
+        if (q == kNever) continue;
+        // first position of v's level: the last level start <= q
+        int lo = 0, hi = L;
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (s_start[mid] <= q) lo = mid;
+          else hi = mid;
+        }
+        const int b0v = s_start[lo];
+        const int32_t* trow = ell + (size_t)v * W;
+        unsigned old_m = 0u, same_m = 0u;
+        for (int e = 0; e < W; e += 8) {
+          const int4 l4 = reinterpret_cast<const int4*>(trow)[e >> 2];
+          const int4 h4 = (e + 4 < W) ? reinterpret_cast<const int4*>(trow)[(e >> 2) + 1] : make_int4(-1, -1, -1, -1);
+          const int u[8] = {l4.x, l4.y, l4.z, l4.w, h4.x, h4.y, h4.z, h4.w};
+          int qu[8];
+#pragma unroll
+          for (int j = 0; j < 8; ++j) qu[j] = u[j] >= 0 ? pos[u[j]] : kNever;
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            if (qu[j] < b0v) old_m |= 1u << (e + j);
+            else if (qu[j] < q) same_m |= 1u << (e + j);
+          }
+          if (u[7] < 0) break;  // -1 padding: the row has ended
+        }
+        // one seed per connected group of the earlier levels' neighbours
+        const uint32_t* aw = S.adjw + (size_t)v * W;
+        unsigned reps = 0u, rem = old_m;
+        while (rem) {
+          const unsigned seed = rem & (0u - rem);
+          unsigned comp = seed, front = seed;
+          while (front) {
+            const int j = __ffs(front) - 1;
+            front &= front - 1u;
+            const unsigned nb = aw[j] & old_m & ~comp;
+            comp |= nb;
+            front |= nb;
+          }
+          reps |= seed;
+          rem &= ~comp;
+        }
+        unsigned cand = reps | same_m;
+        int w4[kCandRowWords] = {-1, -1, -1, -1};
+        if (__popc(cand) > kCandRowWords) {
+          w4[0] = -2 - v;
+        } else {
+#pragma unroll
+          for (int k = 0; k < kCandRowWords; ++k) {
+            if (cand) {
+              const int j = __ffs(cand) - 1;
+              cand &= cand - 1u;
+              w4[k] = pos[trow[j]];
+            }
+          }
+        }
+        reinterpret_cast<int4*>(npos)[q] = make_int4(w4[0], w4[1], w4[2], w4[3]);
+      }
+      __syncthreads();
+    }
 
     // ---- phase 3: threshold sweep -------------------------------------------
     int n_roots = 0;
@@ -771,6 +848,20 @@ __global__ __launch_bounds__(kTfceThreads, FULL ? 3 : 4) void tfce_batch_kernel(
             unsigned adj[8];
             ring_unpack(w, p, adj);
             prune8(p, adj, i, b0, r);
+            join_roots<false>(parent, size, r, i, b0);
+          }
+        } else if (use_cr) {
+          // one row of joined positions per element, the next one loaded ahead
+          const int4* rows = reinterpret_cast<const int4*>(npos);
+          int4 w_n = (b0 + tid < b1) ? rows[b0 + tid] : none4;
+          for (int i = b0 + tid; i < b1; i += kTfceThreads) {
+            const int4 w = w_n;
+            if (i + kTfceThreads < b1) w_n = rows[i + kTfceThreads];
+            if (w.x <= -2) {
+              join_table_row(parent, size, pos, ell + (size_t)(-2 - w.x) * W, W, i, b0);
+              continue;
+            }
+            int r[9] = {w.x, w.y, w.z, w.w, -1, -1, -1, -1, 0};
             join_roots<false>(parent, size, r, i, b0);
           }
         } else if (use_np) {
@@ -849,90 +940,7 @@ __global__ __launch_bounds__(kTfceThreads, FULL ? 3 : 4) void tfce_batch_kernel(
             v_nn = (i + 2 * kTfceThreads < b1) ? order[i + 2 * kTfceThreads] : 0;
             const float xv = symmetric ? 0.0f : sign * x[v];
             int ri = i;  // root (or an ancestor) of i's set so far
-            if (ell && S.nadj && !ring) {
-              // symmetric rows with adjacency bytes: four chunks per join.
-              // Each chunk's pruned candidates sorted descending; the larger
-              // half of the bitonic split of two sorted lists holds all of
-              // their candidates when they are 8 or fewer, the smaller half
-              // any others (joined at once, rarely needed).  Two such
-              // halves, sorted, are split again.
-              const int4* row = reinterpret_cast<const int4*>(ell + (size_t)v * W);
-              int u[8] = {lo0.x, lo0.y, lo0.z, lo0.w, hi0.x, hi0.y, hi0.z, hi0.w};
-              // rows of 32 (the padded wide tables): the row's 32 adjacency
-              // bytes in two 16-byte loads
-              const bool w32 = W == kEllMaxWidth;
-              uint4 na[2] = {make_uint4(0u, 0u, 0u, 0u), make_uint4(0u, 0u, 0u, 0u)};
-              if (w32) {
-                const uint4* nr = reinterpret_cast<const uint4*>(S.nadj + (size_t)v * kEllMaxWidth);
-                na[0] = nr[0];
-                na[1] = nr[1];
-              }
-              for (int e = 0;; e += 32) {
-                int half[2][9];
-                bool live = true;  // this chunk holds entries
-#pragma unroll
-                for (int p2 = 0; p2 < 2; ++p2) {
-                  int ra[9], rb[9];
-#pragma unroll
-                  for (int j = 0; j < 9; ++j) ra[j] = rb[j] = -1;
-#pragma unroll
-                  for (int c = 0; c < 2; ++c) {
-                    const int ec = e + 16 * p2 + 8 * c;
-                    if (!(p2 == 0 && c == 0)) {
-                      live = live && ec < W && u[7] >= 0;
-                      if (live) {
-                        const int4 lo = row[ec >> 2];
-                        const int4 hi = (ec + 4 < W) ? row[(ec >> 2) + 1] : none4;
-                        u[0] = lo.x; u[1] = lo.y; u[2] = lo.z; u[3] = lo.w;
-                        u[4] = hi.x; u[5] = hi.y; u[6] = hi.z; u[7] = hi.w;
-                      }
-                    }
-                    if (live) {
-                      unsigned wlo = 0, whi = 0;
-                      if (w32) {
-                        wlo = c == 0 ? na[p2].x : na[p2].z;
-                        whi = c == 0 ? na[p2].y : na[p2].w;
-                      } else {
-                        chunk_adj(S.nadj, v, W, ec, wlo, whi);
-                      }
-                      if (c == 0) {
-                        chunk_cands(pos, u, i, b0, wlo, whi, ra);
-                        sort8_desc(ra);
-                      } else {
-                        chunk_cands(pos, u, i, b0, wlo, whi, rb);
-                        sort8_desc(rb);
-                      }
-                    }
-                  }
-                  int bot[9];
-                  bool spill = false;
-#pragma unroll
-                  for (int j = 0; j < 8; ++j) {
-                    half[p2][j] = max(ra[j], rb[7 - j]);
-                    bot[j] = min(ra[j], rb[7 - j]);
-                    spill |= bot[j] >= 0;
-                  }
-                  if (__any(spill)) ri = join_roots<true>(parent, size, bot, ri, b0);
-                }
-                sort8_desc(half[0]);
-                sort8_desc(half[1]);
-                int top[9], bot[9];
-                bool spill = false;
-#pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                  top[j] = max(half[0][j], half[1][7 - j]);
-                  bot[j] = min(half[0][j], half[1][7 - j]);
-                  spill |= bot[j] >= 0;
-                }
-                ri = join_roots<true>(parent, size, top, ri, b0);
-                if (__any(spill)) ri = join_roots<true>(parent, size, bot, ri, b0);
-                if (!live || e + 32 >= W || u[7] < 0) break;
-                const int4 lo = row[(e + 32) >> 2];
-                const int4 hi = (e + 36 < W) ? row[((e + 32) >> 2) + 1] : none4;
-                u[0] = lo.x; u[1] = lo.y; u[2] = lo.z; u[3] = lo.w;
-                u[4] = hi.x; u[5] = hi.y; u[6] = hi.z; u[7] = hi.w;
-              }
-            } else if (ell) {
+            if (ell) {
               // no adjacency bytes (asymmetric lists): chunk by chunk
               int u[8] = {lo0.x, lo0.y, lo0.z, lo0.w, hi0.x, hi0.y, hi0.z, hi0.w};
               ri = join_chunk(parent, size, pos, x, u, i, v, ri, b0, b1, symmetric, sign, xv);

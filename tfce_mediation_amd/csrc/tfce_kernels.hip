@@ -16,7 +16,9 @@
 //        at level l is the id prefix [0, end_l) and "is u active" is
 //        `pos[u] < end_l`; for symmetric surfaces with a fixed-width
 //        neighbour table, one row of neighbour positions per active element
-//        (a joined neighbour is then simply one with a smaller position);
+//        (a joined neighbour is then simply one with a smaller position), or
+//        for the wide voxel tables one row of the positions to join (the
+//        earlier neighbours, one per adjacent group of older ones);
 //     2. per level: hook the newly active vertices into a lock-free union-find
 //        (roots of an element's joined neighbours found as interleaved
 //        chains, every other root hooked under the smallest by CAS, larger

@@ -522,7 +522,7 @@ __global__ __launch_bounds__(kTfceThreads, FULL ? 3 : 4) void tfce_batch_kernel(
   __shared__ int s_start[kMaxLevels + 1];
   __shared__ int s_cur[kMaxLevels];
   __shared__ float s_red[kTfceThreads / 64];
-  __shared__ int s_item_lo, s_item_hi, s_nroots_new;
+  __shared__ int s_item_lo, s_item_hi, s_nroots_new, s_nact;
 
   const int tid = threadIdx.x;
   const int64_t vstride = (a.v_max + 63) & ~63;
@@ -569,6 +569,7 @@ __global__ __launch_bounds__(kTfceThreads, FULL ? 3 : 4) void tfce_batch_kernel(
       s_HH[l] = __int_as_float(rec[kPrepT + kMaxLevels + l]);
       s_cur[l] = 0;
     }
+    if (tid == 0) s_nact = 0;
     __syncthreads();
     // eight elements per thread and pass, loads before stores
     if (L > 0) {
@@ -614,7 +615,7 @@ __global__ __launch_bounds__(kTfceThreads, FULL ? 3 : 4) void tfce_batch_kernel(
     const bool use_np = RING || (symmetric && ell != nullptr && (ring ? kRingRowWords : W) <= a.np_w);
     // wide symmetric tables: rows of joined positions (kCandRowWords)
     const bool use_cr = !RING && !use_np && symmetric && ell != nullptr && S.adjw != nullptr &&
-                        a.np_w >= kCandRowWords;
+                        W == kEllMaxWidth && a.np_w >= kCandRowWords;
     const bool need_order = FULL || !(use_np || use_cr);
     if (L > 0) {
       // the levels again from the map (cheaper than storing and re-reading
@@ -641,6 +642,13 @@ __global__ __launch_bounds__(kTfceThreads, FULL ? 3 : 4) void tfce_batch_kernel(
             if (need_order) order[q] = v;
             pos[v] = q;
           }
+        }
+        // joined-position rows in MAX mode: the active elements, in element
+        // order (appended per wave), so that the rows pass visits only them
+        // (FULL mode keeps the level order there, which lists them too)
+        if (use_cr && !need_order) {
+#pragma unroll
+          for (int j = 0; j < 8; ++j) wave_append(lv[j] != kNever, v0 + j * kTfceThreads, order, &s_nact);
         }
       }
     }
@@ -746,16 +754,37 @@ __global__ __launch_bounds__(kTfceThreads, FULL ? 3 : 4) void tfce_batch_kernel(
       __syncthreads();
     }
     // Wide symmetric table: the joined positions of every active element, in
-    // element order (table rows and neighbours' positions near each other):
+    // element order over the list of active elements (table rows and
+    // neighbours' positions near each other):
     // the neighbours activated earlier in its level, and one of each group of
     // mutually adjacent neighbours activated at earlier levels (one set
     // already), found with the row's full adjacency.  The union step then
     // reads one 16-byte row per element instead of a table row, its
     // adjacency and 26 positions.
     if (use_cr && L > 0) {
-      for (int v = tid; v < V; v += kTfceThreads) {
+      constexpr int CW = kEllMaxWidth;
+      for (int k = tid; k < A; k += kTfceThreads) {
+        const int v = order[k];  // the k-th active element
         const int q = pos[v];
-        if (q == kNever) continue;
+        // the table row (one line), then every neighbour's position and the
+        // row's adjacency masks (one line): two rounds of independent loads
+        const int4* trow4 = reinterpret_cast<const int4*>(ell + (size_t)v * CW);
+        const uint4* arow4 = reinterpret_cast<const uint4*>(S.adjw + (size_t)v * CW);
+        int u[CW];
+        unsigned m[CW];
+#pragma unroll
+        for (int c = 0; c < CW / 4; ++c) {
+          const int4 t4 = trow4[c];
+          u[4 * c] = t4.x; u[4 * c + 1] = t4.y; u[4 * c + 2] = t4.z; u[4 * c + 3] = t4.w;
+        }
+        int qu[CW];
+#pragma unroll
+        for (int j = 0; j < CW; ++j) qu[j] = u[j] >= 0 ? pos[u[j]] : kNever;
+#pragma unroll
+        for (int c = 0; c < CW / 4; ++c) {
+          const uint4 a4 = arow4[c];
+          m[4 * c] = a4.x; m[4 * c + 1] = a4.y; m[4 * c + 2] = a4.z; m[4 * c + 3] = a4.w;
+        }
         // first position of v's level: the last level start <= q
         int lo = 0, hi = L;
         while (hi - lo > 1) {
@@ -764,53 +793,44 @@ __global__ __launch_bounds__(kTfceThreads, FULL ? 3 : 4) void tfce_batch_kernel(
           else hi = mid;
         }
         const int b0v = s_start[lo];
-        const int32_t* trow = ell + (size_t)v * W;
         unsigned old_m = 0u, same_m = 0u;
-        for (int e = 0; e < W; e += 8) {
-          const int4 l4 = reinterpret_cast<const int4*>(trow)[e >> 2];
-          const int4 h4 = (e + 4 < W) ? reinterpret_cast<const int4*>(trow)[(e >> 2) + 1] : make_int4(-1, -1, -1, -1);
-          const int u[8] = {l4.x, l4.y, l4.z, l4.w, h4.x, h4.y, h4.z, h4.w};
-          int qu[8];
 #pragma unroll
-          for (int j = 0; j < 8; ++j) qu[j] = u[j] >= 0 ? pos[u[j]] : kNever;
-#pragma unroll
-          for (int j = 0; j < 8; ++j) {
-            if (qu[j] < b0v) old_m |= 1u << (e + j);
-            else if (qu[j] < q) same_m |= 1u << (e + j);
-          }
-          if (u[7] < 0) break;  // -1 padding: the row has ended
+        for (int j = 0; j < CW; ++j) {
+          if (qu[j] < b0v) old_m |= 1u << j;
+          else if (qu[j] < q) same_m |= 1u << j;
         }
         // one seed per connected group of the earlier levels' neighbours
-        const uint32_t* aw = S.adjw + (size_t)v * W;
+        // (flood fill over the masks, one hop per round, in registers)
         unsigned reps = 0u, rem = old_m;
         while (rem) {
           const unsigned seed = rem & (0u - rem);
-          unsigned comp = seed, front = seed;
-          while (front) {
-            const int j = __ffs(front) - 1;
-            front &= front - 1u;
-            const unsigned nb = aw[j] & old_m & ~comp;
-            comp |= nb;
-            front |= nb;
-          }
+          unsigned comp = seed, prev;
+          do {
+            prev = comp;
+            unsigned grow = 0u;
+#pragma unroll
+            for (int j = 0; j < CW; ++j) grow |= (comp >> j) & 1u ? m[j] : 0u;
+            comp = (comp | grow) & old_m;
+          } while (comp != prev);
           reps |= seed;
           rem &= ~comp;
         }
-        unsigned cand = reps | same_m;
-        int w4[kCandRowWords] = {-1, -1, -1, -1};
+        const unsigned cand = reps | same_m;
+        int4 w = make_int4(-1, -1, -1, -1);
         if (__popc(cand) > kCandRowWords) {
-          w4[0] = -2 - v;
+          w.x = -2 - v;  // joined from the table row
         } else {
+          // the candidates' positions, in entry order, through a 4-slot shift
 #pragma unroll
-          for (int k = 0; k < kCandRowWords; ++k) {
-            if (cand) {
-              const int j = __ffs(cand) - 1;
-              cand &= cand - 1u;
-              w4[k] = pos[trow[j]];
+          for (int j = CW - 1; j >= 0; --j)
+            if ((cand >> j) & 1u) {
+              w.w = w.z;
+              w.z = w.y;
+              w.y = w.x;
+              w.x = qu[j];
             }
-          }
         }
-        reinterpret_cast<int4*>(npos)[q] = make_int4(w4[0], w4[1], w4[2], w4[3]);
+        reinterpret_cast<int4*>(npos)[q] = w;
       }
       __syncthreads();
     }

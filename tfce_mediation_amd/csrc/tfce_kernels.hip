@@ -2119,11 +2119,12 @@ hipError_t launch_tfce_coop(const TfceCoopArgs& a, hipStream_t stream) {
   int dev = 0, cus = 0;
   if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
   if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
-  // one workgroup per 4,096 elements, at most 64 and one per compute unit:
+  // one workgroup per 4,096 elements, at most 24 and one per compute unit:
   // the grid barrier's cost grows with the workgroups it waits for (an
-  // order-7 hemisphere took 2.9 ms with 32 workgroups, 3.5 ms with 64 and
-  // 6.5 ms with 160)
-  int64_t want = std::min<int64_t>(64, ((int64_t)a.V + 4095) / 4096);
+  // order-7 hemisphere took 2.74 ms with 24 workgroups, 2.81 ms with 32 and
+  // 3.03 ms with 40; the 232k-voxel volume 6.50, 6.80 and 8.22 ms with 24,
+  // 32 and 56; 160 took the hemisphere to 6.5 ms)
+  int64_t want = std::min<int64_t>(24, ((int64_t)a.V + 4095) / 4096);
   int64_t cap = (int64_t)std::min(per_cu, 1) * cus;
   if (a.grid_limit > 0) cap = std::min<int64_t>(cap, a.grid_limit);
   const int grid = (int)std::max<int64_t>(1, std::min(want, cap));

@@ -96,8 +96,8 @@ GRAPHS = {
     "grid8": lambda: adjacency.voxel_csr(np.ones((1, 30, 40), bool), 26),
     # rows of width 12: two groups of 8 in the neighbour-position rows
     "ico4_w12": lambda: mesh_with_extra_edges(4, 3000, 2, 1),
-    # rows of width 20, padded to 32: the table path, pruning per group of 8,
-    # four groups joined at once
+    # rows of width 20, padded to 32: rows of joined positions found with the
+    # full row adjacency (the 26-connected volumes' path) on an irregular mesh
     "ico4_w20": lambda: mesh_with_extra_edges(4, 9000, 2, 2),
     "ico5": lambda: synth.icosphere_csr(5),
     "vox26": lambda: adjacency.voxel_csr(synth.ellipsoid_mask((21, 25, 19), (10, 12, 9), (9, 11, 8.5)), 26),

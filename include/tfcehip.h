@@ -97,6 +97,21 @@ int tfce_adj_voxel_csr(const uint8_t *d_mask, int32_t X, int32_t Y, int32_t Z, i
 int tfce_adj_mesh_csr(const int32_t *d_faces, int64_t F, int32_t n_vertices, int32_t *d_row_ptr,
                       int32_t *d_col, int64_t *nnz, hipStream_t stream);
 
+/* Replaces adjacency.compute(v, f, thresholds) and compute_distance_parallel
+ * (adjacency.pyx:44-79, :82-106: the exact geodesic distance of lib/geodesic
+ * from every vertex i to every vertex j > i): every pair (i < j) of a
+ * triangle mesh (float32 vertices [n_vertices][3], int32 faces
+ * [n_faces][3]) whose distance along the surface is below max_distance, with
+ * that distance, in (i, j) order.  Host code, n_threads host threads (0: all
+ * cores); the result is read with tfce_geodesic_count / _copy and freed with
+ * tfce_geodesic_destroy.  TFCE_EADJ if a face names a vertex out of range. */
+typedef struct tfce_geodesic tfce_geodesic_t;
+int tfce_geodesic_pairs(const float *vertices, int32_t n_vertices, const int32_t *faces, int32_t n_faces,
+                        double max_distance, int32_t n_threads, tfce_geodesic_t **out);
+int tfce_geodesic_count(const tfce_geodesic_t *pairs, int64_t *n_pairs);
+int tfce_geodesic_copy(const tfce_geodesic_t *pairs, int32_t *i, int32_t *j, double *distance);
+int tfce_geodesic_destroy(tfce_geodesic_t *pairs);
+
 /* Replaces CreateAdjSet.run(image, enhn) (tfce.pyx:44-45 -> fast_tfce.hpp:11-95):
  * enhn += TFCE(image) over V float32 elements (accumulate != 0), or
  * enhn = TFCE(image) (accumulate == 0).  dh == 0 gives the reference's step

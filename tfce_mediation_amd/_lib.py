@@ -39,6 +39,10 @@ SIGNATURES = {
     "tfce_set_single_image_mode": (c_int, [c_i32, c_i32]),
     "tfce_adj_voxel_csr": (c_int, [c_p, c_i32, c_i32, c_i32, c_i32, c_p, c_p, c_p, c_p, c_p]),
     "tfce_adj_mesh_csr": (c_int, [c_p, c_i64, c_i32, c_p, c_p, c_p, c_p]),
+    "tfce_geodesic_pairs": (c_int, [c_p, c_i32, c_p, c_i32, ctypes.c_double, c_i32, c_p]),
+    "tfce_geodesic_count": (c_int, [c_p, c_p]),
+    "tfce_geodesic_copy": (c_int, [c_p, c_p, c_p, c_p]),
+    "tfce_geodesic_destroy": (c_int, [c_p]),
     "tfce_set_lds_sweep": (c_int, [c_i32]),
     "tfce_set_pair_sweep": (c_int, [c_i32]),
     "tfce_batch_ex": (c_int, [c_p, c_p, c_i64, c_int, c_int, c_u32, c_p, c_p, c_p, c_p, c_p, c_p]),
@@ -101,9 +105,23 @@ def lib():
     return _lib
 
 
+_host = None
+
+
+def host_lib():
+    """The library for its host-only entry points (the geodesic adjacency),
+    which need no GPU."""
+    global _host
+    with _lock:
+        if _host is None:
+            _host = _lib if _lib is not None else load_library()
+    return _host
+
+
 def check(rc):
     if rc != 0:
-        msg = _lib.tfce_last_error().decode() if _lib is not None else ""
+        src = _lib if _lib is not None else _host
+        msg = src.tfce_last_error().decode() if src is not None else ""
         raise TfceHipError(f"libtfcehip error {rc}: {msg}")
     return rc
 

@@ -257,3 +257,49 @@ def faces_to_csr_device(faces, n_vertices, stream=None):
     _lib.check(lib.tfce_adj_mesh_csr(_lib.ptr(f), F, int(n_vertices), _lib.ptr(rp), _lib.ptr(col),
                                      ctypes.byref(nnz), _lib.stream_ptr(stream)))
     return rp, col[:nnz.value]
+
+
+def geodesic_pairs(vertices, faces, max_distance, n_threads=0):
+    """(i, j, distance) arrays of every vertex pair i < j of a triangle mesh
+    closer than max_distance along the surface, in (i, j) order: the exact
+    geodesic distances adjacency.pyx:44-106 takes from lib/geodesic, computed
+    by the library's window propagation on host threads (one-time set-up
+    work, like the reference's)."""
+    import ctypes
+    from . import _lib
+    v = np.ascontiguousarray(np.asarray(vertices, dtype=np.float32).reshape(-1, 3))
+    f = np.ascontiguousarray(np.asarray(faces, dtype=np.int32).reshape(-1, 3))
+    lib = _lib.host_lib()
+    h = ctypes.c_void_p()
+    _lib.check(lib.tfce_geodesic_pairs(v.ctypes.data, len(v), f.ctypes.data, len(f), float(max_distance),
+                                       int(n_threads), ctypes.byref(h)))
+    try:
+        n = ctypes.c_int64()
+        _lib.check(lib.tfce_geodesic_count(h, ctypes.byref(n)))
+        i = np.empty(n.value, np.int32)
+        j = np.empty(n.value, np.int32)
+        d = np.empty(n.value, np.float64)
+        _lib.check(lib.tfce_geodesic_copy(h, i.ctypes.data, j.ctypes.data, d.ctypes.data))
+    finally:
+        lib.tfce_geodesic_destroy(h)
+    return i, j, d
+
+
+def geodesic_adjacency(vertices, faces, thresholds, n_threads=0):
+    """adjacency.compute(v, f, thresholds) (adjacency.pyx:44-79): for each
+    threshold, the list per vertex of the vertices closer than it along the
+    surface, in increasing order (the order the reference's loops append
+    them in)."""
+    thresholds = np.atleast_1d(np.asarray(thresholds, dtype=np.float64))
+    nv = int(np.asarray(vertices).reshape(-1, 3).shape[0])
+    i, j, d = geodesic_pairs(vertices, faces, float(thresholds.max()), n_threads)
+    out = []
+    for t in thresholds:
+        keep = d < t
+        a = np.concatenate([i[keep], j[keep]])
+        b = np.concatenate([j[keep], i[keep]])
+        order = np.lexsort((b, a))
+        a, b = a[order], b[order]
+        starts = np.searchsorted(a, np.arange(nv + 1))
+        out.append([b[starts[v]:starts[v + 1]].tolist() for v in range(nv)])
+    return out

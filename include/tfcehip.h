@@ -102,12 +102,15 @@ int tfce_adj_mesh_csr(const int32_t *d_faces, int64_t F, int32_t n_vertices, int
  * from every vertex i to every vertex j > i): every pair (i < j) of a
  * triangle mesh (float32 vertices [n_vertices][3], int32 faces
  * [n_faces][3]) whose distance along the surface is below max_distance, with
- * that distance, in (i, j) order.  Host code, n_threads host threads (0: all
- * cores); the result is read with tfce_geodesic_count / _copy and freed with
- * tfce_geodesic_destroy.  TFCE_EADJ if a face names a vertex out of range. */
+ * that distance, in (i, j) order, for the sources i in
+ * [source_begin, source_end) (source_end <= 0: every vertex).  Host code,
+ * n_threads host threads (0: all cores); the result is read with
+ * tfce_geodesic_count / _copy and freed with tfce_geodesic_destroy.
+ * TFCE_EADJ if a face names a vertex out of range. */
 typedef struct tfce_geodesic tfce_geodesic_t;
 int tfce_geodesic_pairs(const float *vertices, int32_t n_vertices, const int32_t *faces, int32_t n_faces,
-                        double max_distance, int32_t n_threads, tfce_geodesic_t **out);
+                        double max_distance, int32_t source_begin, int32_t source_end, int32_t n_threads,
+                        tfce_geodesic_t **out);
 int tfce_geodesic_count(const tfce_geodesic_t *pairs, int64_t *n_pairs);
 int tfce_geodesic_copy(const tfce_geodesic_t *pairs, int32_t *i, int32_t *j, double *distance);
 int tfce_geodesic_destroy(tfce_geodesic_t *pairs);

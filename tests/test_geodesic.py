@@ -124,3 +124,14 @@ def test_rejects_bad_faces():
         adjacency.geodesic_pairs(v, f, 1.5)
     with pytest.raises(TfceHipError):
         adjacency.geodesic_pairs(v, grid(3, 3)[1], -1.0)
+
+
+def test_one_source_matches_the_full_run():
+    """compute_distance_parallel (adjacency.pyx:82-106) for one vertex: the
+    full run's pairs of that source."""
+    v, f = bumpy_sphere(3, 10.0, 0.03, 5)
+    i, j, d = adjacency.geodesic_pairs(v, f, 3.0)
+    for s in (0, 17, len(v) - 2):
+        got = adjacency.compute_distance_parallel(v, f, 3.0, s)
+        exp = [[int(a), int(b), float(x)] for a, b, x in zip(i, j, d) if a == s]
+        assert got == exp

@@ -39,7 +39,7 @@ SIGNATURES = {
     "tfce_set_single_image_mode": (c_int, [c_i32, c_i32]),
     "tfce_adj_voxel_csr": (c_int, [c_p, c_i32, c_i32, c_i32, c_i32, c_p, c_p, c_p, c_p, c_p]),
     "tfce_adj_mesh_csr": (c_int, [c_p, c_i64, c_i32, c_p, c_p, c_p, c_p]),
-    "tfce_geodesic_pairs": (c_int, [c_p, c_i32, c_p, c_i32, ctypes.c_double, c_i32, c_p]),
+    "tfce_geodesic_pairs": (c_int, [c_p, c_i32, c_p, c_i32, ctypes.c_double, c_i32, c_i32, c_i32, c_p]),
     "tfce_geodesic_count": (c_int, [c_p, c_p]),
     "tfce_geodesic_copy": (c_int, [c_p, c_p, c_p, c_p]),
     "tfce_geodesic_destroy": (c_int, [c_p]),

@@ -259,19 +259,23 @@ def faces_to_csr_device(faces, n_vertices, stream=None):
     return rp, col[:nnz.value]
 
 
-def geodesic_pairs(vertices, faces, max_distance, n_threads=0):
+def geodesic_pairs(vertices, faces, max_distance, n_threads=0, sources=None):
     """(i, j, distance) arrays of every vertex pair i < j of a triangle mesh
     closer than max_distance along the surface, in (i, j) order: the exact
     geodesic distances adjacency.pyx:44-106 takes from lib/geodesic, computed
     by the library's window propagation on host threads (one-time set-up
-    work, like the reference's)."""
+    work, like the reference's).  sources: (begin, end) range of i (default
+    every vertex)."""
     import ctypes
     from . import _lib
     v = np.ascontiguousarray(np.asarray(vertices, dtype=np.float32).reshape(-1, 3))
     f = np.ascontiguousarray(np.asarray(faces, dtype=np.int32).reshape(-1, 3))
     lib = _lib.host_lib()
     h = ctypes.c_void_p()
-    _lib.check(lib.tfce_geodesic_pairs(v.ctypes.data, len(v), f.ctypes.data, len(f), float(max_distance),
+    b, e = (0, 0) if sources is None else (int(sources[0]), int(sources[1]))
+    if sources is not None and e <= b:
+        return np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0, np.float64)
+    _lib.check(lib.tfce_geodesic_pairs(v.ctypes.data, len(v), f.ctypes.data, len(f), float(max_distance), b, e,
                                        int(n_threads), ctypes.byref(h)))
     try:
         n = ctypes.c_int64()
@@ -303,3 +307,10 @@ def geodesic_adjacency(vertices, faces, thresholds, n_threads=0):
         starts = np.searchsorted(a, np.arange(nv + 1))
         out.append([b[starts[v]:starts[v + 1]].tolist() for v in range(nv)])
     return out
+
+
+def compute_distance_parallel(v, f, threshold, i):
+    """adjacency.compute_distance_parallel (adjacency.pyx:82-106): [[i, j,
+    distance]] for every vertex j > i closer than threshold to vertex i."""
+    a, b, d = geodesic_pairs(v, f, float(np.max(threshold)), n_threads=1, sources=(int(i), int(i) + 1))
+    return [[int(x), int(y), float(z)] for x, y, z in zip(a, b, d)]

@@ -343,25 +343,29 @@ struct tfce_geodesic {
 extern "C" {
 
 int tfce_geodesic_pairs(const float* vertices, int32_t n_vertices, const int32_t* faces, int32_t n_faces,
-                        double max_distance, int32_t n_threads, tfce_geodesic_t** out) {
+                        double max_distance, int32_t source_begin, int32_t source_end, int32_t n_threads,
+                        tfce_geodesic_t** out) {
   if (!vertices || !faces || !out) return set_error(TFCE_EINVAL, "NULL argument");
   if (n_vertices < 1 || n_faces < 1) return set_error(TFCE_EINVAL, "need at least one vertex and one face");
   if (!(max_distance > 0.0) || !std::isfinite(max_distance))
     return set_error(TFCE_EINVAL, "max_distance must be positive and finite");
+  if (source_end <= 0) source_end = n_vertices;
+  if (source_begin < 0 || source_begin > source_end || source_end > n_vertices)
+    return set_error(TFCE_EINVAL, "source range out of [0, n_vertices]");
   Geo g;
   int rc = build(g, vertices, n_vertices, faces, n_faces);
   if (rc) return rc;
   int nt = n_threads > 0 ? n_threads : (int)std::max(1u, std::thread::hardware_concurrency());
   nt = std::min(nt, 256);
-  std::atomic<int> next(0);
+  std::atomic<int> next(source_begin);
   std::vector<tfce_geodesic> part(nt);
   auto worker = [&](int t) {
     Source src(g);
     constexpr int kChunk = 64;
     while (true) {
       const int s0 = next.fetch_add(kChunk);
-      if (s0 >= n_vertices) break;
-      for (int s = s0; s < std::min(n_vertices, s0 + kChunk); ++s) {
+      if (s0 >= source_end) break;
+      for (int s = s0; s < std::min(source_end, s0 + kChunk); ++s) {
         src.run(s, max_distance);
         std::vector<int> near;
         for (int v : src.touched())

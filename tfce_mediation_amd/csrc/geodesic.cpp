@@ -242,6 +242,17 @@ class Source {
     const double eps = 1e-12 * L;
     if (w.b0 <= eps) set(v0, w.sigma + std::sqrt(w.sx * w.sx + w.sy * w.sy));
     if (w.b1 >= L - eps) set(v1, w.sigma + std::sqrt((w.sx - L) * (w.sx - L) + w.sy * w.sy));
+    // A window longer at every point of its interval than the path through
+    // one end of the edge (that end's distance, then along the edge) only
+    // carries paths that the end's route beats: every ray continues straight
+    // from its point, and the route through the end to that point and on is
+    // a surface path at most as long.  Along the edge, window minus route is
+    // non-increasing for the route through v0 and non-decreasing for the one
+    // through v1, so one interval end decides each.
+    const double w1 = w.sigma + std::sqrt((w.b1 - w.sx) * (w.b1 - w.sx) + w.sy * w.sy);
+    const double w0 = w.sigma + std::sqrt((w.b0 - w.sx) * (w.b0 - w.sx) + w.sy * w.sy);
+    const double tol = 1e-12 * (w0 + w1);
+    if (w1 > d_[v0] + w.b1 + tol || w0 > d_[v1] + (L - w.b0) + tol) return;
     const int c = third(g_, w.f, w.e);
     if (c < 0) return;
     double cx, cy;

@@ -279,6 +279,31 @@ int tfce_ols_direct(const float *d_Y, int32_t n, int64_t V, int64_t ldy, const d
                     const double *invXX_se, double *d_coef, float *d_se, double *d_t, double *d_resid,
                     double *d_sse, double *d_sumy, double *d_sumyy, hipStream_t stream);
 
+/* Element-wise regressor (pyfunc.py:1024-1086 image_regression, called by
+ * STEP_1_vertex_tfce_multiple_regression.py:276-289 and
+ * STEP_1_voxel_tfce_multiple_regression.py:162-183): for every element v the
+ * design is the shared columns R (host n x r row-major, intercept first,
+ * 1 <= r <= 7) plus the element's own regressor Z[.][v]; the t-values of all
+ * r + 1 coefficients (FP64 least squares, sigma2 = SSE / (n - r - 1), se =
+ * sqrt(sigma2 * diag(inv(X'X))), t = a / se) go to d_t[j * ldt + v] as float,
+ * j = r for the element's regressor.  Elements whose regressor holds a NaN or
+ * has a population standard deviation below std_min (the reference: 0.01)
+ * get 0 in every column.  Y and Z are device [n][ldy] / [n][ldz] float;
+ * r + 1 < n <= 2048.  Blocking. */
+int tfce_image_regression(const float *d_Y, int64_t ldy, const float *d_Z, int64_t ldz, const double *R,
+                          int32_t n, int32_t r, int64_t V, double std_min, float *d_t, int64_t ldt,
+                          hipStream_t stream);
+
+/* Variance inflation image (pyfunc.py:1088-1114 image_reg_VIF): per element,
+ * R2 = 1 - RSS / TSS of y on X (host n x k row-major, intercept first,
+ * k <= 8, k < n <= 2048; RSS from the FP64 least-squares fit), VIF =
+ * 1 / (1 - R2), NaN set to 0, written to the device array d_vif[V] (double).
+ * d_Y is a device [n][ldy] array of float (y_f64 = 0) or double (y_f64 = 1);
+ * TSS = sum((y - mean)^2) is formed in that type, subject by subject, as
+ * numpy reduces the first axis.  Blocking. */
+int tfce_image_vif(const void *d_Y, int32_t y_f64, int64_t ldy, const double *X, int32_t n, int32_t k, int64_t V,
+                   double *d_vif, hipStream_t stream);
+
 /* se[j][v] = fl32(sqrt(sigma2[v] * invXX[j][j])) for j < k (cynumstats.pyx:47-52);
  * invXX is host k x k, sigma2 and se are device arrays.  Blocking. */
 int tfce_se_of_slope(const double *d_sigma2, const double *invXX, int32_t k, int64_t V, float *d_se,

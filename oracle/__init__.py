@@ -19,6 +19,8 @@ Contents
                ``pyfunc.py:107-126``).
 * ``graphs``   adjacency builders restated from ``pyfunc.py:37-76`` and
                ``tm_func.py:502-562`` plus brute-force BFS components.
+* ``imgreg``   the element-wise regressor and VIF image
+               (``pyfunc.py:264-271, 1024-1114``).
 
 Pins (each checked by a test that shares no code with the function)
 -------------------------------------------------------------------
@@ -39,4 +41,8 @@ Pins (each checked by a test that shares no code with the function)
                          6- and 26-connectivity) and a coordinate brute force
                          (``tests/test_adjacency.py``).
 * ``mlp``                central finite differences (``tests/test_oracle_mlp.py``).
+* ``imgreg.image_regression``   ``ols.tval_int`` when every element has the same
+                         regressor; Frisch-Waugh-Lovell with QR least squares
+                         (``tests/test_oracle_imgreg.py``).
+* ``imgreg.image_reg_VIF``      1 / (1 - r^2) of ``scipy.stats.linregress``.
 """

@@ -62,6 +62,9 @@ SIGNATURES = {
     "tfce_ols_direct": (c_int, [c_p, c_i32, c_i64, c_i64, c_p, c_i32, c_p, c_p, c_p, c_p, c_p, c_p, c_p,
                                 c_p, c_p]),
     "tfce_se_of_slope": (c_int, [c_p, c_p, c_i32, c_i64, c_p, c_p]),
+    "tfce_image_regression": (c_int, [c_p, c_i64, c_p, c_i64, c_p, c_i32, c_i32, c_i64, ctypes.c_double, c_p, c_i64,
+                                      c_p]),
+    "tfce_image_vif": (c_int, [c_p, c_i32, c_i64, c_p, c_i32, c_i32, c_i64, c_p, c_p]),
     "tfce_total_ss": (c_int, [c_p, c_i32, c_i64, c_i64, c_p, c_p]),
     "tfce_mlp_param_count": (c_int, []),
     "tfce_mlp_forward": (c_int, [c_p, c_p, c_i32, c_p, c_p]),

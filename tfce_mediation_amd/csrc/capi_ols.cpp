@@ -103,14 +103,16 @@ int upload(T* d, const T* h, size_t count, const char* what) {
   return 0;
 }
 
-// Lower Cholesky factor and its inverse (row-major k x k); false if not SPD.
+// Lower Cholesky factor and its inverse (row-major k x k); false if not SPD,
+// or if a pivot is below 1e-13 of its diagonal entry (a column that is a
+// combination of the ones before it, up to rounding).
 bool chol_inv(const std::vector<double>& G, int k, std::vector<double>& Li) {
   std::vector<double> L(k * k, 0.0);
   Li.assign(k * k, 0.0);
   for (int j = 0; j < k; ++j) {
     double d = G[j * k + j];
     for (int q = 0; q < j; ++q) d -= L[j * k + q] * L[j * k + q];
-    if (!(d > 0.0)) return false;
+    if (!(d > 1e-13 * G[j * k + j])) return false;
     d = std::sqrt(d);
     L[j * k + j] = d;
     for (int i = j + 1; i < k; ++i) {
@@ -675,6 +677,91 @@ int tfce_ols_direct(const float* d_Y, int32_t n, int64_t V, int64_t ldy, const d
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
   for (void* p : {(void*)dX, (void*)dA, (void*)dS}) (void)hipFree(p);
   if (e != hipSuccess) return set_error(-(int)e, "ols direct kernel");
+  return 0;
+}
+
+// Host FP64 inverse of D'D for a row-major n x m design D; false if singular.
+static bool gram_inverse(const double* D, int n, int m, std::vector<double>& inv) {
+  std::vector<double> G(m * m, 0.0), Li;
+  for (int a = 0; a < m; ++a)
+    for (int b = 0; b < m; ++b) {
+      double s = 0.0;
+      for (int i = 0; i < n; ++i) s += D[i * m + a] * D[i * m + b];
+      G[a * m + b] = s;
+    }
+  if (!chol_inv(G, m, Li)) return false;
+  inv.assign(m * m, 0.0);
+  for (int a = 0; a < m; ++a)
+    for (int b = 0; b < m; ++b) {
+      double s = 0.0;
+      for (int q = std::max(a, b); q < m; ++q) s += Li[q * m + a] * Li[q * m + b];
+      inv[a * m + b] = s;
+    }
+  return true;
+}
+
+int tfce_image_regression(const float* d_Y, int64_t ldy, const float* d_Z, int64_t ldz, const double* R,
+                          int32_t n, int32_t r, int64_t V, double std_min, float* d_t, int64_t ldt,
+                          hipStream_t stream) {
+  if (!d_Y || !d_Z || !R || !d_t) return set_error(TFCE_EINVAL, "NULL argument");
+  if (r < 1 || r + 1 > kOlsMaxK || n <= r + 1 || n > kOlsDirectMaxN || V < 1 || ldy < V || ldz < V || ldt < V)
+    return set_error(TFCE_EINVAL, "need 1 <= r <= 7, r + 1 < n <= 2048, V >= 1 and strides >= V");
+  std::vector<double> Ainv;
+  if (!gram_inverse(R, n, r, Ainv)) return set_error(TFCE_EINVAL, "R'R is singular");
+  double *dR = nullptr, *dA = nullptr;
+  int rc = 0;
+  if ((rc = dalloc(&dR, (size_t)n * r, "R")) || (rc = dalloc(&dA, (size_t)r * r, "Ainv")) ||
+      (rc = upload(dR, R, (size_t)n * r, "R")) || (rc = upload(dA, Ainv.data(), (size_t)r * r, "Ainv"))) {
+    for (void* p : {(void*)dR, (void*)dA}) if (p) (void)hipFree(p);
+    return rc;
+  }
+  ImgRegArgs a{};
+  a.Y = d_Y;
+  a.Z = d_Z;
+  a.ldy = ldy;
+  a.ldz = ldz;
+  a.n = n;
+  a.r = r;
+  a.V = V;
+  a.R = dR;
+  a.Ainv = dA;
+  a.std_min = std_min;
+  a.T = d_t;
+  a.ldt = ldt;
+  hipError_t e = launch_image_regression(a, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  for (void* p : {(void*)dR, (void*)dA}) (void)hipFree(p);
+  if (e != hipSuccess) return set_error(-(int)e, "image regression kernel");
+  return 0;
+}
+
+int tfce_image_vif(const void* d_Y, int32_t y_f64, int64_t ldy, const double* X, int32_t n, int32_t k, int64_t V,
+                   double* d_vif, hipStream_t stream) {
+  if (!d_Y || !X || !d_vif) return set_error(TFCE_EINVAL, "NULL argument");
+  if (k < 1 || k > kOlsMaxK || n <= k || n > kOlsDirectMaxN || V < 1 || ldy < V)
+    return set_error(TFCE_EINVAL, "need 1 <= k <= 8, k < n <= 2048, V >= 1 and ldy >= V");
+  std::vector<double> Ainv;
+  if (!gram_inverse(X, n, k, Ainv)) return set_error(TFCE_EINVAL, "X'X is singular");
+  double *dX = nullptr, *dA = nullptr;
+  int rc = 0;
+  if ((rc = dalloc(&dX, (size_t)n * k, "X")) || (rc = dalloc(&dA, (size_t)k * k, "Ainv")) ||
+      (rc = upload(dX, X, (size_t)n * k, "X")) || (rc = upload(dA, Ainv.data(), (size_t)k * k, "Ainv"))) {
+    for (void* p : {(void*)dX, (void*)dA}) if (p) (void)hipFree(p);
+    return rc;
+  }
+  ImgVifArgs a{};
+  a.Y = d_Y;
+  a.ldy = ldy;
+  a.n = n;
+  a.k = k;
+  a.V = V;
+  a.X = dX;
+  a.Ainv = dA;
+  a.vif = d_vif;
+  hipError_t e = launch_image_vif(a, y_f64 != 0, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  for (void* p : {(void*)dX, (void*)dA}) (void)hipFree(p);
+  if (e != hipSuccess) return set_error(-(int)e, "image VIF kernel");
   return 0;
 }
 

@@ -148,6 +148,35 @@ struct OlsDirectArgs {
 };
 
 hipError_t launch_ols_direct(const OlsDirectArgs& a, hipStream_t s);
+
+// Element-wise regressor (image_regression): t of the r shared columns and of
+// the element's own regressor, written to T[j][ldt] (j = r: the regressor).
+struct ImgRegArgs {
+  const float* Y;     // [n][ldy]
+  const float* Z;     // [n][ldz] the element-wise regressor
+  int64_t ldy, ldz;
+  int32_t n, r;       // subjects, shared columns (intercept first), r + 1 <= kOlsMaxK
+  int64_t V;
+  const double* R;    // [n][r] (device)
+  const double* Ainv; // [r][r] inverse of R'R (device)
+  double std_min;     // regressors with a smaller standard deviation give t = 0
+  float* T;           // [r + 1][ldt]
+  int64_t ldt;
+};
+
+// Variance inflation image (image_reg_VIF): 1 / (1 - R2) of y on X.
+struct ImgVifArgs {
+  const void* Y;      // [n][ldy] float or double
+  int64_t ldy;
+  int32_t n, k;
+  int64_t V;
+  const double* X;    // [n][k] (device), intercept first
+  const double* Ainv; // [k][k] inverse of X'X (device)
+  double* vif;        // [V]
+};
+
+hipError_t launch_image_regression(const ImgRegArgs& a, hipStream_t s);
+hipError_t launch_image_vif(const ImgVifArgs& a, bool f64, hipStream_t s);
 hipError_t launch_se_of_slope(const double* sigma2, const double* diag, int k, int64_t V, float* se,
                               hipStream_t s);
 hipError_t launch_total_ss(const float* Y, int32_t n, int64_t V, int64_t ldy, double* tss, double* mean,

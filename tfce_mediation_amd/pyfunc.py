@@ -8,6 +8,8 @@
 * ``write_perm_maxTFCE_vertex`` / ``_voxel``          pyfunc.py:107-126
 * ``calc_sobelz`` / ``calc_indirect``                 pyfunc.py:130-162, 2678-2709
 * ``find_nearest``                                    pyfunc.py:254-261
+* ``zscaler``                                         pyfunc.py:264-271
+* ``image_regression`` / ``image_reg_VIF``            pyfunc.py:1024-1114
 * ``glm_typeI``                                       pyfunc.py:2282-2404
 * ``check_blocks`` / ``rand_blocks``                  pyfunc.py:2711-2757
 The TFCE and statistics run on the GPU; the null writers append to the same
@@ -133,6 +135,74 @@ def write_perm_maxTFCE_voxel(statname, voxelStat, TFCEfunc):
     with open("perm_%s_TFCE_maxVoxel.csv" % statname, "a") as f:
         f.write("%1.4f\n" % maxval)
     return maxval
+
+
+def zscaler(X, axis=0, w_mean=True, w_std=True):
+    """Centre and/or scale X along ``axis`` by its mean and (population)
+    standard deviation, in X's dtype (pyfunc.py:264-271)."""
+    out = np.array(X, copy=True)
+    if w_mean:
+        out -= np.mean(out, axis)
+    if w_std:
+        out /= np.std(out, axis)
+    return out
+
+
+def image_regression(y, image_x, pred_x, covars=None, normalize=False, verbose=True, stream=None):
+    """Regression with an element-wise regressor (pyfunc.py:1024-1086).
+
+    ``y`` and ``image_x`` are [elements, subjects]; per element the design is
+    [1, pred_x, covars, image_x[element]] (covars optional).  Returns float32
+    ``(Tval, Timg)``: the t of the intercept and pred_x columns
+    (``arr[:, :len(pred_x.T) + 1]`` of the reference's per-element t array, so a
+    one-dimensional pred_x keeps every column) and the t of the element's
+    regressor; elements whose regressor holds a NaN or has std() < 0.01 are 0.
+    FP64 per element on the GPU (``tfce_image_regression``); with
+    ``normalize`` image_x is first z-scaled along axis 0 (``zscaler``).
+    """
+    import time
+    import torch
+    t0 = time.time()
+    y = np.asarray(y)
+    image_x = np.asarray(image_x)
+    pred_x = np.asarray(pred_x)
+    nv, n = y.shape
+    regressors = pred_x if covars is None else np.column_stack((pred_x, covars))
+    R = np.ascontiguousarray(np.column_stack([np.ones(n), regressors]), dtype=np.float64)
+    if normalize:
+        image_x = zscaler(image_x, axis=0, w_mean=True, w_std=True)
+    r = R.shape[1]
+    dy = torch.as_tensor(np.ascontiguousarray(y.T, dtype=np.float32)).cuda()
+    dz = torch.as_tensor(np.ascontiguousarray(image_x.T, dtype=np.float32)).cuda()
+    dt = torch.empty((r + 1, nv), dtype=torch.float32, device=dy.device)
+    lib = _lib.lib()
+    _lib.check(lib.tfce_image_regression(_lib.ptr(dy), nv, _lib.ptr(dz), nv, R.ctypes.data, n, r, nv, 0.01,
+                                         _lib.ptr(dt), nv, _lib.stream_ptr(stream)))
+    arr = dt.cpu().numpy().T
+    Tval = np.array(arr[:, :len(pred_x.T) + 1], dtype=np.float32)
+    Timg = np.array(np.squeeze(arr[:, -1:]), dtype=np.float32)
+    if verbose:
+        print("Finished. Image-wise independent variable regression took %.1f seconds" % (time.time() - t0))
+    return Tval, Timg
+
+
+def image_reg_VIF(y, regressors, stream=None):
+    """Variance inflation image (pyfunc.py:1088-1114): per element (column
+    of the [subjects, elements] array y) 1 / (1 - R2) of y on [1, regressors],
+    NaN set to 0, as float64.  The sum of squares about the element's mean is
+    formed in y's own type (float32 or float64) as numpy forms it; the fit in
+    FP64 on the GPU (``tfce_image_vif``)."""
+    import torch
+    y = np.asarray(y)
+    f64 = y.dtype == np.float64
+    X = np.ascontiguousarray(np.column_stack([np.ones(len(regressors)), regressors]), dtype=np.float64)
+    n, V = y.shape
+    dy = torch.as_tensor(np.ascontiguousarray(y, dtype=np.float64 if f64 else np.float32)).cuda()
+    dv = torch.empty(V, dtype=torch.float64, device=dy.device)
+    lib = _lib.lib()
+    _lib.check(lib.tfce_image_vif(_lib.ptr(dy), 1 if f64 else 0, V, X.ctypes.data, n, X.shape[1], V,
+                                  _lib.ptr(dv), _lib.stream_ptr(stream)))
+    return dv.cpu().numpy()
 
 
 def calc_indirect(ta, tb, alg="aroian"):

@@ -1,16 +1,18 @@
 """Observed-statistic images: the output half of the step-1 scripts.
 
 * ``vertex_regression_images``
-  STEP_1_vertex_tfce_multiple_regression.py:273-398 (no image regressor):
-  t of every coefficient by ``tval_int`` with inv(X'X), then per contrast
+  STEP_1_vertex_tfce_multiple_regression.py:273-398: t of every coefficient by
+  ``tval_int`` with inv(X'X), or with a vertex-wise regressor (:276-345) by
+  ``image_regression`` with the images ``tstat_imgcovar``,
+  ``negtstat_imgcovar`` and ``VIF_imgcovar`` (no TFCE) first; then per contrast
   j = 1..k-1 the images ``tstat_con<j>`` and ``negtstat_con<j>`` of both
   hemispheres through ``write_vertStat_img`` (TFCE of the statistic scattered
   into the whole hemisphere, rescaled by the masked maximum / 100 and by
   vdensity; its maximum appended to max_TFCE_contrast_values.csv).
 * ``voxel_regression_images``
-  STEP_1_voxel_tfce_multiple_regression.py:174-230: the same for one volume
-  (NaN t set to 0 first, :212), or with ``ancova`` the ``fstat`` image
-  sqrt(max(calcF, 0)) (:218-227).
+  STEP_1_voxel_tfce_multiple_regression.py:160-230: the same for one volume
+  (with a voxel-wise regressor :162-208; NaN t set to 0 first, :212), or with
+  ``ancova`` the ``fstat`` image sqrt(max(calcF, 0)) (:218-227).
 * ``fwep_image``  calculate_fweP_vertex.py:45-79 / calculate_fweP_voxel.py:
   1 - p(FWE) of every element > 0 of an observed TFCE image against the max-TFCE
   null (left-neighbour rule), 0 elsewhere; optionally -log10(1 - that).
@@ -26,7 +28,7 @@ import numpy as np
 
 from .cynumstats import calcF, tval_int
 from .fwe import calculate_fwep
-from .pyfunc import save_npy, write_vertStat_img, write_voxelStat_img
+from .pyfunc import image_reg_VIF, image_regression, save_npy, write_vertStat_img, write_voxelStat_img
 from .tfce import CreateAdjSet
 
 
@@ -44,10 +46,19 @@ def _in_dir(path):
         os.chdir(old)
 
 
+def _regressors(pred_x, covars):
+    return pred_x if covars is None else np.column_stack((pred_x, covars))
+
+
 def vertex_regression_images(merge_y, pred_x, adjac_lh, adjac_rh, bin_mask_lh, bin_mask_rh, optstfce,
-                             vdensity_lh=1, vdensity_rh=1, surface="area", out_dir=None, save=save_npy):
+                             vdensity_lh=1, vdensity_rh=1, surface="area", out_dir=None, save=save_npy,
+                             img_x=None, covars=None):
     """{(statname, hemi): (statistic image, scaled TFCE image)} for every
-    contrast and sign; merge_y is [n, V_lh + V_rh] (masked elements)."""
+    contrast and sign; merge_y is [n, V_lh + V_rh] (masked elements).  With
+    ``img_x`` ([n, V_lh + V_rh], the vertex-wise regressor) the t-values come
+    from ``image_regression`` on [1, pred_x, covars, img_x] and the regressor's
+    own images and the VIF image (of merge_y on pred_x and covars) are written
+    first."""
     merge_y = np.asarray(merge_y)
     n = merge_y.shape[0]
     bin_mask_lh = np.asarray(bin_mask_lh, dtype=bool)
@@ -56,8 +67,6 @@ def vertex_regression_images(merge_y, pred_x, adjac_lh, adjac_rh, bin_mask_lh, b
     num_vertex = merge_y.shape[1]
     X = np.column_stack([np.ones(n), pred_x])
     k = X.shape[1]
-    invXX = np.linalg.inv(np.dot(X.T, X))
-    tvals = tval_int(X, invXX, merge_y, n, k, num_vertex)
     H, E = float(optstfce[0]), float(optstfce[1])
     calc = {"lh": CreateAdjSet(H, E, adjac_lh), "rh": CreateAdjSet(H, E, adjac_rh)}
     masks = {"lh": bin_mask_lh, "rh": bin_mask_rh}
@@ -65,6 +74,21 @@ def vertex_regression_images(merge_y, pred_x, adjac_lh, adjac_rh, bin_mask_lh, b
     parts = {"lh": slice(0, nv_lh), "rh": slice(nv_lh, num_vertex)}
     out = {}
     with _in_dir(out_dir):
+        if img_x is not None:
+            tvals, timage = image_regression(merge_y.T.astype(np.float32), np.asarray(img_x).T.astype(np.float32),
+                                             pred_x, covars, verbose=False)
+            VIF = image_reg_VIF(merge_y, _regressors(pred_x, covars))
+            tvals = tvals.T
+            timage = timage.T
+            for name, stat, tfce in (("tstat_imgcovar", timage, True), ("negtstat_imgcovar", -timage, True),
+                                     ("VIF_imgcovar", VIF, False)):
+                for hemi in ("lh", "rh"):
+                    out[(name, hemi)] = write_vertStat_img(
+                        name, stat[parts[hemi]], None, None, surface, hemi, masks[hemi], calc[hemi],
+                        masks[hemi].shape[0], vds[hemi], TFCE=tfce, save=save)
+        else:
+            invXX = np.linalg.inv(np.dot(X.T, X))
+            tvals = tval_int(X, invXX, merge_y, n, k, num_vertex)
         for j in range(k - 1):
             tnum = j + 1
             for name, sgn in (("tstat_con%d" % tnum, 1), ("negtstat_con%d" % tnum, -1)):
@@ -79,10 +103,13 @@ def vertex_regression_images(merge_y, pred_x, adjac_lh, adjac_rh, bin_mask_lh, b
 
 
 def voxel_regression_images(y, pred_x, adjac, mask, optstfce, affine=None, imgext=".nii.gz", out_dir=None,
-                            ancova=False, save=save_npy):
+                            ancova=False, save=save_npy, image_x=None, covars=None):
     """{statname: (statistic, scaled TFCE)} over the in-mask voxels; y is
     [n, num_voxel], ``mask`` the boolean volume (data_index) whose True
-    elements, in C order, are the voxels of y and of the adjacency."""
+    elements, in C order, are the voxels of y and of the adjacency.  With
+    ``image_x`` ([n, num_voxel], the voxel-wise regressor) the t-values come
+    from ``image_regression`` and the regressor's images and the VIF image (of
+    image_x on pred_x and covars) are written first."""
     y = np.asarray(y)
     n, num_voxel = y.shape
     mask = np.asarray(mask, dtype=bool)
@@ -98,8 +125,20 @@ def voxel_regression_images(y, pred_x, adjac, mask, optstfce, affine=None, imgex
             out["fstat"] = write_voxelStat_img("fstat", np.sqrt(fvals), vol, mask, affine, calc, imgext,
                                                save=save)
             return out
-        invXX = np.linalg.inv(np.dot(X.T, X))
-        tvalues = tval_int(X, invXX, y, n, k, num_voxel)
+        if image_x is not None:
+            image_x = np.asarray(image_x, dtype=np.float32)
+            tvalues, timage = image_regression(y.T.astype(np.float32), image_x.T, pred_x, covars, verbose=False)
+            out["tstat_imgcovar"] = write_voxelStat_img("tstat_imgcovar", timage, vol, mask, affine, calc, imgext,
+                                                        save=save)
+            out["negtstat_imgcovar"] = write_voxelStat_img("negtstat_imgcovar", -timage, vol, mask, affine, calc,
+                                                           imgext, save=save)
+            tvalues = tvalues.T
+            VIF = image_reg_VIF(image_x, _regressors(pred_x, covars))
+            out["VIF_imgcovar"] = write_voxelStat_img("VIF_imgcovar", VIF, vol, mask, affine, calc, imgext,
+                                                      TFCE=False, save=save)
+        else:
+            invXX = np.linalg.inv(np.dot(X.T, X))
+            tvalues = tval_int(X, invXX, y, n, k, num_voxel)
         tvalues[np.isnan(tvalues)] = 0
         for j in range(k - 1):
             tnum = j + 1

@@ -139,6 +139,64 @@ def test_voxel_regression_images(torch, tmp_path, ancova):
         np.testing.assert_array_equal(vol[mask], stat)
 
 
+@pytest.mark.parametrize("medtype", ["M", "I"])
+def test_vertex_mediation_images(torch, tmp_path, medtype):
+    """STEP_1_vertex_tfce_mediation.py:279-288: SobelZ_<medtype> of both
+    hemispheres, the statistic against the oracle's calc_sobelz, the TFCE
+    image within 1 ulp of the oracle's TFCE of the same float32 image."""
+    from tfce_mediation_amd import step1
+    say(f"[gpu] step1.vertex_mediation_images {medtype} ico3 x2")
+    row_ptr, col = synth.icosphere_csr(3)
+    V = len(row_ptr) - 1
+    rng = np.random.default_rng(31)
+    m_lh, m_rh = rng.random(V) < 0.9, rng.random(V) < 0.8
+    n = 40
+    x = rng.standard_normal(n)
+    Y = np.concatenate([smooth_rows(row_ptr, col, n, 3)[:, m_lh], smooth_rows(row_ptr, col, n, 4)[:, m_rh]], 1)
+    Y[:, :60] += 0.6 * x[:, None].astype(np.float32)
+    dep = 0.5 * Y[:, :60].mean(1) + rng.standard_normal(n)
+    lists = [col[row_ptr[i]:row_ptr[i + 1]].tolist() for i in range(V)]
+    out = step1.vertex_mediation_images(Y, x, dep, medtype, lists, lists, m_lh, m_rh, [2, 1], surface="area",
+                                        out_dir=str(tmp_path))
+    z = ools.calc_sobelz(medtype, x, dep, Y.astype(np.float64), n, Y.shape[1])
+    nl = int(m_lh.sum())
+    assert sorted(out) == [("SobelZ_%s" % medtype, "lh"), ("SobelZ_%s" % medtype, "rh")]
+    for (name, hemi), (stat_img, tfce_img) in out.items():
+        m = m_lh if hemi == "lh" else m_rh
+        part = slice(0, nl) if hemi == "lh" else slice(nl, Y.shape[1])
+        np.testing.assert_allclose(stat_img[m], z[part].astype(np.float32), rtol=1e-6)
+        masked = stat_img[m].astype(np.float64)
+        exp = oracle_scaled_tfce(row_ptr, col, stat_img, masked, 2.0, 1.0)
+        assert ulp_diff(tfce_img, exp).max() <= 1, hemi
+        for suffix in ("_TFCE.npy", ".npy"):
+            assert os.path.exists(os.path.join(tmp_path, "%s_area_%s%s" % (name, hemi, suffix)))
+    lines = open(os.path.join(tmp_path, "max_TFCE_contrast_values.csv")).read().split()
+    assert [l.split(",")[0] for l in lines] == ["SobelZ_%s_area_lh" % medtype, "SobelZ_%s_area_rh" % medtype]
+
+
+def test_voxel_mediation_images(torch, tmp_path):
+    """STEP_1_voxel_tfce_mediation.py:97-104 (type M, 26-connected, E = 0.5)."""
+    from tfce_mediation_amd import step1
+    mask = synth.ellipsoid_mask((15, 17, 13), (7, 8, 6), (6, 7, 5.5))
+    row_ptr, col = adjacency.voxel_csr(mask, 26)
+    nv = int(mask.sum())
+    say(f"[gpu] step1.voxel_mediation_images M voxels={nv}")
+    n = 28
+    rng = np.random.default_rng(6)
+    x = rng.standard_normal(n)
+    y = smooth_rows(row_ptr, col, n, 12)
+    y[:, :150] += 0.7 * x[:, None].astype(np.float32)
+    dep = 0.6 * y[:, :150].mean(1) + rng.standard_normal(n)
+    out = step1.voxel_mediation_images(y, x, dep, "M", (row_ptr, col), mask, [2, 0.5], out_dir=str(tmp_path))
+    z = ools.calc_sobelz("M", x, dep, y.astype(np.float64), n, nv)
+    stat, tfce_vals = out["SobelZ_M"]
+    np.testing.assert_allclose(stat, z.astype(np.float32), rtol=1e-6)
+    img = stat.astype(np.float32)
+    enh, st, _ = otfce.tfce(row_ptr, col, img, 2.0, 0.5, exact_pow=True)
+    assert ulp_diff(tfce_vals, enh * (img.max() / 100)).max() <= 4
+    assert os.path.exists(os.path.join(tmp_path, "SobelZ_M_TFCE.npy"))
+
+
 def test_fwep_image(torch):
     from tfce_mediation_amd import step1
     say("[gpu] step1.fwep_image")

@@ -13,6 +13,10 @@
   STEP_1_voxel_tfce_multiple_regression.py:160-230: the same for one volume
   (with a voxel-wise regressor :162-208; NaN t set to 0 first, :212), or with
   ``ancova`` the ``fstat`` image sqrt(max(calcF, 0)) (:218-227).
+* ``vertex_mediation_images`` / ``voxel_mediation_images``
+  STEP_1_vertex_tfce_mediation.py:279-288 / STEP_1_voxel_tfce_mediation.py:
+  97-104: the Sobel-family z of ``calc_sobelz`` written as ``SobelZ_<medtype>``
+  (TFCE image and statistic) of both hemispheres or of the volume.
 * ``fwep_image``  calculate_fweP_vertex.py:45-79 / calculate_fweP_voxel.py:
   1 - p(FWE) of every element > 0 of an observed TFCE image against the max-TFCE
   null (left-neighbour rule), 0 elsewhere; optionally -log10(1 - that).
@@ -28,7 +32,8 @@ import numpy as np
 
 from .cynumstats import calcF, tval_int
 from .fwe import calculate_fwep
-from .pyfunc import image_reg_VIF, image_regression, save_npy, write_vertStat_img, write_voxelStat_img
+from .pyfunc import (calc_sobelz, image_reg_VIF, image_regression, save_npy, write_vertStat_img,
+                     write_voxelStat_img)
 from .tfce import CreateAdjSet
 
 
@@ -147,6 +152,44 @@ def voxel_regression_images(y, pred_x, adjac, mask, optstfce, affine=None, imgex
             out["negtstat_con%d" % tnum] = write_voxelStat_img("negtstat_con%d" % tnum, tvalues[tnum] * -1, vol,
                                                                mask, affine, calc, imgext, save=save)
     return out
+
+
+def vertex_mediation_images(merge_y, pred_x, depend_y, medtype, adjac_lh, adjac_rh, bin_mask_lh, bin_mask_rh,
+                            optstfce, vdensity_lh=1, vdensity_rh=1, surface="area", out_dir=None, save=save_npy,
+                            alg="aroian"):
+    """{("SobelZ_<medtype>", hemi): (statistic image, scaled TFCE image)}:
+    the observed z of mediation type I, M or Y over merge_y ([n, V_lh + V_rh],
+    masked elements, covariates already removed) and its TFCE images."""
+    merge_y = np.asarray(merge_y)
+    n, num_vertex = merge_y.shape
+    bin_mask_lh = np.asarray(bin_mask_lh, dtype=bool)
+    bin_mask_rh = np.asarray(bin_mask_rh, dtype=bool)
+    nv_lh = int(bin_mask_lh.sum())
+    H, E = float(optstfce[0]), float(optstfce[1])
+    sobel = calc_sobelz(medtype, pred_x, depend_y, merge_y, n, num_vertex, alg)
+    name = "SobelZ_%s" % medtype
+    out = {}
+    with _in_dir(out_dir):
+        for hemi, part, mask, adj, vd in (("lh", slice(0, nv_lh), bin_mask_lh, adjac_lh, vdensity_lh),
+                                          ("rh", slice(nv_lh, num_vertex), bin_mask_rh, adjac_rh, vdensity_rh)):
+            out[(name, hemi)] = write_vertStat_img(name, sobel[part], None, None, surface, hemi, mask,
+                                                   CreateAdjSet(H, E, adj), mask.shape[0], vd, save=save)
+    return out
+
+
+def voxel_mediation_images(y, pred_x, depend_y, medtype, adjac, mask, optstfce, affine=None, imgext=".nii.gz",
+                           out_dir=None, save=save_npy, alg="aroian"):
+    """{"SobelZ_<medtype>": (statistic, scaled TFCE)} over the in-mask voxels
+    of y ([n, num_voxel], covariates already removed)."""
+    y = np.asarray(y)
+    n, num_voxel = y.shape
+    mask = np.asarray(mask, dtype=bool)
+    calc = CreateAdjSet(float(optstfce[0]), float(optstfce[1]), adjac)
+    sobel = calc_sobelz(medtype, pred_x, depend_y, y, n, num_voxel, alg)
+    name = "SobelZ_%s" % medtype
+    with _in_dir(out_dir):
+        return {name: write_voxelStat_img(name, sobel, np.zeros(mask.shape, dtype=np.float32), mask, affine, calc,
+                                          imgext, save=save)}
 
 
 def fwep_image(tfce_image, perm_tfce_max, neglog10=False):

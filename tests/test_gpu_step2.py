@@ -296,3 +296,26 @@ def test_mmr_lr_from_tmi_temp(tmp_path, mediation):
         exp = np.loadtxt(os.path.join(ref, name))
         assert got.shape == exp.shape == ((6,) if mediation else (12,))
         np.testing.assert_allclose(got, exp, rtol=1e-4, atol=1e-6)
+
+
+def test_vertex_regression_from_step1_writer(tmp_path):
+    """The directory step1.save_vertex_handoff writes drives step 2 to the
+    nulls of the batched driver called directly."""
+    from tfce_mediation_amd import randomise, step1, step2
+    say("[gpu] step2.vertex_regression from step1.save_vertex_handoff")
+    rp, col = synth.icosphere_csr(3)
+    V = len(rp) - 1
+    rng = np.random.default_rng(2)
+    m_lh, m_rh = rng.random(V) < 0.85, rng.random(V) < 0.9
+    n = 26
+    merge_y = rng.standard_normal((n, int(m_lh.sum() + m_rh.sum()))).astype(np.float32)
+    pred_x = rng.standard_normal((n, 2))
+    vd = adjacency.vertex_density(rp)
+    step1.save_vertex_handoff("thickness", pred_x, merge_y, (rp, col), (rp, col), m_lh, m_rh, [2, 1], vd, vd,
+                              root=str(tmp_path))
+    got = step2.vertex_regression("thickness", 1, 6, root=str(tmp_path), seed=5, allow_pickle=True)
+    perms = randomise.permutation_indices(n, range(1, 7), seed=5)
+    ref = randomise.vertex_regress(merge_y, pred_x, ragged(rp, col), ragged(rp, col), m_lh, m_rh, [2.0, 1.0],
+                                   perms, vdensity_lh=vd, vdensity_rh=vd)
+    for c in (1, 2):
+        np.testing.assert_array_equal(got[c], ref[c])

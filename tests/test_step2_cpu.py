@@ -1,5 +1,8 @@
 """Step-1 hand-off loader (no GPU): numeric arrays load as saved; object
-arrays (ragged adjacency) are refused unless unpickling is asked for."""
+arrays (ragged adjacency) are refused unless unpickling is asked for; the
+step-1 writers save what the reference's step-1 scripts save."""
+import os
+
 import numpy as np
 import pytest
 
@@ -89,3 +92,65 @@ def test_type_y_path_a_with_covariates_matches_oracle():
         for p, perm in enumerate(perms):
             exp = ools.glm_typeI(right[perm], [left[perm]], dmy_covariates=cov)[2][1]
             assert abs(got[p] - exp) <= 1e-10 * abs(exp)
+
+
+# the variables each step-1 script saves (STEP_1_vertex_tfce_multiple_regression.py:248-266,
+# STEP_1_vertex_tfce_mediation.py:247-277, STEP_0_load_volumes.py:89-96 without the nibabel
+# header, STEP_1_voxel_tfce_multiple_regression.py:88-116, STEP_1_voxel_tfce_mediation.py:90-95)
+VERTEX_FILES = {"pred_x", "num_subjects", "all_vertex", "num_vertex", "num_vertex_lh", "num_vertex_rh",
+                "bin_mask_lh", "bin_mask_rh", "affine_mask_lh", "affine_mask_rh", "adjac_lh", "adjac_rh",
+                "merge_y", "optstfce", "vdensity_lh", "vdensity_rh"}
+VERTEX_MED_FILES = {"pred_x", "depend_y", "num_subjects", "num_vertex", "num_vertex_lh", "num_vertex_rh",
+                    "all_vertex", "bin_mask_lh", "bin_mask_rh", "adjac_lh", "adjac_rh", "optstfce", "vdensity_lh",
+                    "vdensity_rh", "merge_y"}
+VOLUME_FILES = {"raw_nonzero", "affine_mask", "data_mask", "num_voxel", "num_subjects"}
+VOXEL_FILES = VOLUME_FILES | {"covars", "adjac", "pred_x", "ancova", "optstfce", "raw_nonzero_corr"}
+VOXEL_MED_FILES = VOLUME_FILES | {"pred_x", "depend_y", "adjac", "medtype", "optstfce", "raw_nonzero_corr"}
+
+
+def _names(d):
+    return {f[:-4] for f in os.listdir(d) if f.endswith(".npy")}
+
+
+def test_handoff_writers_round_trip(tmp_path):
+    """The step-1 writers save the reference's variables under its names, and
+    step 2's reader gets them back (the adjacency only with allow_pickle)."""
+    from tfce_mediation_amd import adjacency, step1, step2, synth
+    rp, col = synth.icosphere_csr(2)
+    V = len(rp) - 1
+    rng = np.random.default_rng(4)
+    m_lh, m_rh = rng.random(V) < 0.9, rng.random(V) < 0.8
+    n = 12
+    y = rng.standard_normal((n, int(m_lh.sum() + m_rh.sum())))
+    x, dep = rng.standard_normal((n, 2)), rng.standard_normal(n)
+    vd = adjacency.vertex_density(rp)
+    d = step1.save_vertex_handoff("area", x, y, (rp, col), (rp, col), m_lh, m_rh, [2, 1], vd, vd, root=str(tmp_path))
+    assert _names(d) == VERTEX_FILES
+    with pytest.raises(ValueError, match="allow_pickle"):
+        step2.load_temp(d, ["adjac_lh"])
+    v = step2.load_temp(d, sorted(VERTEX_FILES), allow_pickle=True)
+    assert v["merge_y"].dtype == np.float32 and np.array_equal(v["merge_y"], y.astype(np.float32))
+    assert int(v["num_subjects"]) == n and int(v["all_vertex"]) == V
+    assert int(v["num_vertex_lh"]) == m_lh.sum() and int(v["num_vertex"]) == y.shape[1]
+    assert [list(r) for r in v["adjac_rh"]] == [col[rp[i]:rp[i + 1]].tolist() for i in range(V)]
+    np.testing.assert_array_equal(v["vdensity_lh"], vd)
+    np.testing.assert_array_equal(v["optstfce"], [2, 1])
+    d = step1.save_vertex_mediation_handoff("area", x[:, 0], dep, y, (rp, col), (rp, col), m_lh, m_rh, [2, 1],
+                                            root=str(tmp_path))
+    assert _names(d) == VERTEX_MED_FILES
+    np.testing.assert_array_equal(step2.load_temp(d, ["depend_y"])["depend_y"], dep)
+
+    mask = synth.ellipsoid_mask((7, 8, 6), (3, 3.5, 2.5), (3, 3.5, 2.5))
+    vrp, vcol = adjacency.voxel_csr(mask, 26)
+    nv = int(mask.sum())
+    yv = rng.standard_normal((n, nv))
+    d = step1.save_voxel_handoff(yv, x, (vrp, vcol), mask, [2, 0.5], ancova=0, covars=rng.standard_normal(n),
+                                 root=str(tmp_path / "vox"))
+    assert _names(d) == VOXEL_FILES
+    v = step2.load_temp(d, ["raw_nonzero_corr", "num_subjects", "num_voxel", "ancova", "data_mask"])
+    assert v["raw_nonzero_corr"].shape == (nv, n) and int(v["num_voxel"]) == nv and int(v["ancova"]) == 0
+    assert np.array_equal(v["data_mask"] > 0.99, mask)
+    d = step1.save_voxel_mediation_handoff(yv, x[:, 0], dep, "M", (vrp, vcol), mask, [2, 0.5],
+                                           root=str(tmp_path / "voxmed"))
+    assert _names(d) == VOXEL_MED_FILES
+    assert str(step2.load_temp(d, ["medtype"])["medtype"]) == "M"

@@ -17,6 +17,15 @@
   STEP_1_vertex_tfce_mediation.py:279-288 / STEP_1_voxel_tfce_mediation.py:
   97-104: the Sobel-family z of ``calc_sobelz`` written as ``SobelZ_<medtype>``
   (TFCE image and statistic) of both hemispheres or of the volume.
+* ``save_vertex_handoff`` / ``save_vertex_mediation_handoff`` /
+  ``save_voxel_handoff`` / ``save_voxel_mediation_handoff``
+  the step-1 -> step-2 directories as the step-1 scripts write them
+  (STEP_1_vertex_tfce_multiple_regression.py:245-266,
+  STEP_1_vertex_tfce_mediation.py:244-277,
+  STEP_1_voxel_tfce_multiple_regression.py:85-116 with STEP_0_load_volumes.py:
+  89-96, STEP_1_voxel_tfce_mediation.py:90-95): one ``np.save`` per variable
+  under the reference's names, ragged adjacency as an object array (which
+  ``step2`` reads back only with ``allow_pickle=True``).
 * ``fwep_image``  calculate_fweP_vertex.py:45-79 / calculate_fweP_voxel.py:
   1 - p(FWE) of every element > 0 of an observed TFCE image against the max-TFCE
   null (left-neighbour rule), 0 elsewhere; optionally -log10(1 - that).
@@ -190,6 +199,96 @@ def voxel_mediation_images(y, pred_x, depend_y, medtype, adjac, mask, optstfce, 
     with _in_dir(out_dir):
         return {name: write_voxelStat_img(name, sobel, np.zeros(mask.shape, dtype=np.float32), mask, affine, calc,
                                           imgext, save=save)}
+
+
+def _adjacency_object_array(adjac):
+    """Ragged neighbour lists as the object array np.save writes for them;
+    a CSR pair (row_ptr, col) is split into its rows first."""
+    if isinstance(adjac, tuple) and len(adjac) == 2:
+        row_ptr, col = (np.asarray(a) for a in adjac)
+        rows = [col[row_ptr[i]:row_ptr[i + 1]].tolist() for i in range(len(row_ptr) - 1)]
+    else:
+        rows = [list(r) for r in adjac]
+    arr = np.empty(len(rows), dtype=object)
+    for i, r in enumerate(rows):
+        arr[i] = r
+    return arr
+
+
+def _save_all(directory, items):
+    os.makedirs(directory, exist_ok=True)
+    for name, value in items:
+        np.save(os.path.join(directory, name), value)
+    return directory
+
+
+def save_vertex_handoff(surface, pred_x, merge_y, adjac_lh, adjac_rh, bin_mask_lh, bin_mask_rh, optstfce,
+                        vdensity_lh=1, vdensity_rh=1, affine_mask_lh=None, affine_mask_rh=None, root="."):
+    """``python_temp_<surface>/`` of the vertex GLM; merge_y is [n, V_lh + V_rh]
+    (masked elements), the masks are the full hemispheres'."""
+    merge_y = np.asarray(merge_y)
+    bin_mask_lh = np.asarray(bin_mask_lh, dtype=bool)
+    bin_mask_rh = np.asarray(bin_mask_rh, dtype=bool)
+    nl, nr = int(bin_mask_lh.sum()), int(bin_mask_rh.sum())
+    eye = np.eye(4)
+    return _save_all(os.path.join(root, "python_temp_%s" % surface), [
+        ("pred_x", pred_x), ("num_subjects", merge_y.shape[0]), ("all_vertex", bin_mask_lh.shape[0]),
+        ("num_vertex", nl + nr), ("num_vertex_lh", nl), ("num_vertex_rh", nr), ("bin_mask_lh", bin_mask_lh),
+        ("bin_mask_rh", bin_mask_rh), ("affine_mask_lh", eye if affine_mask_lh is None else affine_mask_lh),
+        ("affine_mask_rh", eye if affine_mask_rh is None else affine_mask_rh),
+        ("adjac_lh", _adjacency_object_array(adjac_lh)), ("adjac_rh", _adjacency_object_array(adjac_rh)),
+        ("merge_y", merge_y.astype(np.float32, order="C")), ("optstfce", np.asarray(optstfce)),
+        ("vdensity_lh", vdensity_lh), ("vdensity_rh", vdensity_rh)])
+
+
+def save_vertex_mediation_handoff(surface, pred_x, depend_y, merge_y, adjac_lh, adjac_rh, bin_mask_lh, bin_mask_rh,
+                                  optstfce, vdensity_lh=1, vdensity_rh=1, root="."):
+    """``python_temp_med_<surface>/`` of the vertex mediation."""
+    merge_y = np.asarray(merge_y)
+    bin_mask_lh = np.asarray(bin_mask_lh, dtype=bool)
+    bin_mask_rh = np.asarray(bin_mask_rh, dtype=bool)
+    nl, nr = int(bin_mask_lh.sum()), int(bin_mask_rh.sum())
+    return _save_all(os.path.join(root, "python_temp_med_%s" % surface), [
+        ("pred_x", pred_x), ("depend_y", depend_y), ("num_subjects", merge_y.shape[0]), ("num_vertex", nl + nr),
+        ("num_vertex_lh", nl), ("num_vertex_rh", nr), ("all_vertex", bin_mask_lh.shape[0]),
+        ("bin_mask_lh", bin_mask_lh), ("bin_mask_rh", bin_mask_rh),
+        ("adjac_lh", _adjacency_object_array(adjac_lh)), ("adjac_rh", _adjacency_object_array(adjac_rh)),
+        ("optstfce", np.asarray(optstfce)), ("vdensity_lh", vdensity_lh), ("vdensity_rh", vdensity_rh),
+        ("merge_y", merge_y.astype(np.float32, order="C"))])
+
+
+def _voxel_common(y, mask, affine_mask, raw=None):
+    y = np.asarray(y)
+    mask = np.asarray(mask)
+    return [("raw_nonzero", (y.T if raw is None else np.asarray(raw)).astype(np.float32, order="C")),
+            ("affine_mask", np.eye(4) if affine_mask is None else affine_mask),
+            ("data_mask", mask.astype(np.float32)), ("num_voxel", int((mask > 0.99).sum())),
+            ("num_subjects", y.shape[0])]
+
+
+def save_voxel_handoff(y, pred_x, adjac, mask, optstfce, ancova=0, covars=None, raw_nonzero=None,
+                       affine_mask=None, root="."):
+    """``python_temp/`` of the voxel GLM: STEP_0's volume files (raw_nonzero
+    = ``raw_nonzero`` or y transposed, affine, data_mask, num_voxel,
+    num_subjects) and step 1's (covars when given, adjac, pred_x, ancova,
+    optstfce, raw_nonzero_corr = y transposed); y is [n, num_voxel]."""
+    items = _voxel_common(y, mask, affine_mask, raw_nonzero)
+    if covars is not None:
+        items.append(("covars", covars))
+    items += [("adjac", _adjacency_object_array(adjac)), ("pred_x", pred_x), ("ancova", int(ancova)),
+              ("optstfce", np.asarray(optstfce)), ("raw_nonzero_corr", np.asarray(y).T.astype(np.float32, order="C"))]
+    return _save_all(os.path.join(root, "python_temp"), items)
+
+
+def save_voxel_mediation_handoff(y, pred_x, depend_y, medtype, adjac, mask, optstfce, raw_nonzero=None,
+                                 affine_mask=None, root="."):
+    """``python_temp/`` of the voxel mediation (STEP_0's volume files and
+    step 1's pred_x, depend_y, adjac, medtype, optstfce, raw_nonzero_corr)."""
+    items = _voxel_common(y, mask, affine_mask, raw_nonzero)
+    items += [("pred_x", pred_x), ("depend_y", depend_y), ("adjac", _adjacency_object_array(adjac)),
+              ("medtype", medtype), ("optstfce", np.asarray(optstfce)),
+              ("raw_nonzero_corr", np.asarray(y).T.astype(np.float32, order="C"))]
+    return _save_all(os.path.join(root, "python_temp"), items)
 
 
 def fwep_image(tfce_image, perm_tfce_max, neglog10=False):

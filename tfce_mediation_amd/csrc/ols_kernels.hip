@@ -631,8 +631,9 @@ __global__ __launch_bounds__(256) void ols_gemm_fp32_kernel(OlsGemmArgs a) {
 // (global_load_lds_dwordx4: each wave instruction moves 64 consecutive
 // 16-byte fragments, landing at consecutive LDS addresses), so every A and B
 // element crosses from L2 once per workgroup instead of once per wave.  Two
-// LDS stages: step s reads its fragments into registers, then issues the DMA
-// of step s+1 into the other stage, then runs its MFMAs while that DMA is in
+// LDS stages: step s issues the DMA of step s+1 into the other stage, then
+// reads its own fragments into registers and runs its MFMAs (each group of
+// products waits only for the fragments it uses) while that DMA is in
 // flight; one barrier per step separates the DMA of a stage from the reads
 // of the step before.
 template <int NP, int WR, int WC>
@@ -735,6 +736,12 @@ __global__ __launch_bounds__(64 * WR * WC) void ols_gemm_split_kernel(OlsGemmArg
     asm volatile("" ::: "memory");
     __builtin_amdgcn_s_barrier();
     asm volatile("" ::: "memory");
+    // the copy of a later stage goes first, into the buffer every wave
+    // finished reading before the barrier, so that the fragment reads below
+    // are the last LDS operations before the products and can be waited for
+    // in part
+    if (step + D - 1 < nsteps)
+      gemm_stage_copy<NP, WR, WC>(a, smem + ((step + D - 1) % D) * S::kBytes, 2 * (step + D - 1), mrow0, ncol0, w);
     const uint8_t* cur = smem + (step % D) * S::kBytes;
     typedef typename std::conditional<F16, f16x8, bf16x8>::type op8;
     op8 A[NP][4], B[NP][2];
@@ -749,8 +756,6 @@ __global__ __launch_bounds__(64 * WR * WC) void ols_gemm_split_kernel(OlsGemmArg
         B[q][ct] = *reinterpret_cast<const op8*>(cur + q * S::kPlane + S::kA +
                                                   (h * S::kCols + wc * 64 + 32 * ct + r) * 16);
     }
-    if (step + D - 1 < nsteps)
-      gemm_stage_copy<NP, WR, WC>(a, smem + ((step + D - 1) % D) * S::kBytes, 2 * (step + D - 1), mrow0, ncol0, w);
 #pragma unroll
     for (int rt = 0; rt < 4; ++rt)
 #pragma unroll

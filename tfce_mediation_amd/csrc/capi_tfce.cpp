@@ -2,10 +2,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <string>
 #include <utility>
+#include <unordered_map>
 #include <vector>
 
 #include "tfcehip.h"
@@ -24,7 +26,8 @@ struct tfce_adj {
   int32_t* d_col = nullptr;
   int32_t* d_ell = nullptr;     // [V][ell_width], -1 padded; null for long lists
   uint8_t* d_nadj = nullptr;    // [V][ell_width] adjacency among a row's neighbours (width <= 8)
-  uint32_t* d_adjw = nullptr;   // [V][ell_width] full row adjacency, rows wider than kNposMaxWidth
+  uint32_t* d_adjw = nullptr;   // [V or distinct rows][ell_width] full row adjacency, rows wider than kNposMaxWidth
+  uint16_t* d_adjid = nullptr;  // [V] row of d_adjw per element when d_adjw holds distinct rows only
   int32_t ell_width = 0;
   double* d_sizepow = nullptr;  // (double)s^E for s in [0, V] when E != 1
   SurfDev* d_surf = nullptr;    // one descriptor, offset 0
@@ -246,6 +249,7 @@ int tfce_adj_create_ex(const int32_t* row_ptr, const int32_t* col, int32_t V, in
     if (a->d_ell) (void)hipFree(a->d_ell);
     if (a->d_nadj) (void)hipFree(a->d_nadj);
     if (a->d_adjw) (void)hipFree(a->d_adjw);
+    if (a->d_adjid) (void)hipFree(a->d_adjid);
     if (a->d_sizepow) (void)hipFree(a->d_sizepow);
     if (a->d_surf) (void)hipFree(a->d_surf);
     delete a;
@@ -370,10 +374,45 @@ int tfce_adj_create_ex(const int32_t* row_ptr, const int32_t* col, int32_t V, in
               for (int32_t j = 0; j < width; ++j)
                 if (row[j] >= 0) mark[row[j]] = -1;
             }
-            const size_t wbytes = adjw.size() * sizeof(uint32_t);
+            // A grid's rows repeat (every interior voxel of a 26-connected
+            // volume has the same neighbour offsets in the same order): keep
+            // each distinct row once, in L2, and a 16-bit row id per element,
+            // when there are few enough distinct rows
+            std::vector<uint16_t> rid;
+            std::vector<uint32_t> dict;
+            {
+              std::unordered_map<uint64_t, std::vector<int32_t>> seen;  // row hash -> distinct rows
+              const size_t rb = (size_t)width * sizeof(uint32_t);
+              rid.resize(V);
+              for (int32_t v = 0; v < V && rid.size() == (size_t)V; ++v) {
+                const uint32_t* row = adjw.data() + (size_t)v * width;
+                uint64_t h = 1469598103934665603ull;
+                for (int32_t j = 0; j < width; ++j) h = (h ^ row[j]) * 1099511628211ull;
+                int32_t id = -1;
+                for (int32_t d : seen[h])
+                  if (std::memcmp(dict.data() + (size_t)d * width, row, rb) == 0) { id = d; break; }
+                if (id < 0) {
+                  id = (int32_t)(dict.size() / width);
+                  if (id >= kAdjDictMaxRows) { rid.clear(); break; }
+                  dict.insert(dict.end(), row, row + width);
+                  seen[h].push_back(id);
+                }
+                rid[v] = (uint16_t)id;
+              }
+            }
+            // (only when rows repeat: all-distinct rows would add an id load)
+            const bool use_dict = rid.size() == (size_t)V && 2 * (dict.size() / width) <= (size_t)V;
+            const std::vector<uint32_t>& wsrc = use_dict ? dict : adjw;
+            const size_t wbytes = wsrc.size() * sizeof(uint32_t);
             if ((e = hipMalloc(&a->d_adjw, wbytes)) != hipSuccess) return cleanup(-(int)e, "hipMalloc(adjw)");
-            if ((e = hipMemcpy(a->d_adjw, adjw.data(), wbytes, hipMemcpyHostToDevice)) != hipSuccess)
+            if ((e = hipMemcpy(a->d_adjw, wsrc.data(), wbytes, hipMemcpyHostToDevice)) != hipSuccess)
               return cleanup(-(int)e, "upload adjw");
+            if (use_dict) {
+              const size_t ibytes = rid.size() * sizeof(uint16_t);
+              if ((e = hipMalloc(&a->d_adjid, ibytes)) != hipSuccess) return cleanup(-(int)e, "hipMalloc(adjid)");
+              if ((e = hipMemcpy(a->d_adjid, rid.data(), ibytes, hipMemcpyHostToDevice)) != hipSuccess)
+                return cleanup(-(int)e, "upload adjid");
+            }
           }
         }
       }
@@ -392,6 +431,7 @@ int tfce_adj_create_ex(const int32_t* row_ptr, const int32_t* col, int32_t V, in
   sd.ell_width = a->ell_width;
   sd.nadj = a->d_nadj;
   sd.adjw = a->d_adjw;
+  sd.adjid = a->d_adjid;
   sd.sizepow = a->d_sizepow;
   sd.offset = 0;
   sd.V = V;
@@ -489,6 +529,7 @@ int tfce_adj_destroy(tfce_adj_t* a) {
   if (a->d_ell) (void)hipFree(a->d_ell);
   if (a->d_nadj) (void)hipFree(a->d_nadj);
   if (a->d_adjw) (void)hipFree(a->d_adjw);
+  if (a->d_adjid) (void)hipFree(a->d_adjid);
   if (a->d_sizepow) (void)hipFree(a->d_sizepow);
   (void)hipFree(a->d_surf);
   delete a;
@@ -809,6 +850,7 @@ int tfce_surfset_create(const tfce_adj_t* const* adjs, int32_t n, const int64_t*
     sd.ell_width = a->ell_width;
     sd.nadj = a->d_nadj;
     sd.adjw = a->d_adjw;
+    sd.adjid = a->d_adjid;
     sd.sizepow = a->d_sizepow;
     sd.offset = offsets[idx[k]];
     sd.V = a->V;

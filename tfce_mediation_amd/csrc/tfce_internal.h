@@ -47,6 +47,10 @@ constexpr int32_t kRingRowWords = 4;
 // more than four such candidates holds -2 - (its element) in the first word
 // and is joined from the table.
 constexpr int32_t kCandRowWords = 4;
+// Most distinct full-adjacency rows a surface keeps as a table of distinct
+// rows with a 16-bit row id per element (8 MB of rows at width 32); above
+// that every element keeps its own row.
+constexpr int32_t kAdjDictMaxRows = 65535;
 // Largest surface the LDS-resident sweep holds (12 bytes per element plus the
 // level tables in 160 KB of LDS), and the most levels it takes (a step of
 // max/100 gives at most 102).
@@ -63,6 +67,8 @@ struct SurfDev {
                            // null if not built
   const uint32_t* adjw;    // [V][ell_width] for rows wider than kNposMaxWidth: bit k of entry j set iff
                            // neighbours j and k are adjacent; null if not built
+  const uint16_t* adjid;   // [V] with adjw: the row of adjw that holds element v's masks (adjw then
+                           // holds each distinct row once), or null: row v
   const double* sizepow;   // [V+1] (double)s^E, null when E == 1
   int64_t offset;          // first element of this surface inside a map row
   int32_t V;

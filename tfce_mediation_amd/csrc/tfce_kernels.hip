@@ -769,7 +769,8 @@ __global__ __launch_bounds__(kTfceThreads, FULL ? 3 : 4) void tfce_batch_kernel(
         // the table row (one line), then every neighbour's position and the
         // row's adjacency masks (one line): two rounds of independent loads
         const int4* trow4 = reinterpret_cast<const int4*>(ell + (size_t)v * CW);
-        const uint4* arow4 = reinterpret_cast<const uint4*>(S.adjw + (size_t)v * CW);
+        const size_t arow = S.adjid ? (size_t)S.adjid[v] : (size_t)v;
+        const uint4* arow4 = reinterpret_cast<const uint4*>(S.adjw + arow * CW);
         int u[CW];
         unsigned m[CW];
 #pragma unroll
